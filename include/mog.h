@@ -1,0 +1,136 @@
+/*
+ * mog.h -- C ABI of the MI355X (gfx950) Mixture-of-Grids force step.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device addresses
+ * (for example torch.Tensor.data_ptr() of a CUDA/HIP tensor); `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).  Every call is
+ * asynchronous on `stream` unless stated otherwise.
+ *
+ * The reference (tbmo/mog-solver) has no FFI layer: its solver interface is
+ * the `Simulation` class of main.py, which drives GLSL compute shaders through
+ * moderngl.  Each entry point below names the reference code it replaces.
+ *
+ * Return codes: 0 = MOG_OK, negative = error (see mog_status); the message of
+ * the last error of a context is returned by mog_last_error(ctx), and of a
+ * failed mog_create by mog_last_error(NULL).
+ *
+ * Threading: a context is not re-entrant; one context per device and stream.
+ * Ownership: the caller owns particle buffers; the context owns density
+ * grids, force grids, transform copies and tables.
+ */
+#ifndef MOG_H
+#define MOG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOG_ABI_VERSION 1
+
+typedef enum mog_status {
+  MOG_OK = 0,
+  MOG_EINVAL = -1,        /* invalid argument (size, pointer, alignment, config) */
+  MOG_EHIP = -2,          /* HIP runtime error */
+  MOG_ENOTRANSFORMS = -3, /* mog_set_transforms was not called */
+  MOG_EUNSUPPORTED = -4,  /* valid but not implemented option */
+  MOG_ENOMEM = -5         /* device allocation failed */
+} mog_status;
+
+enum { MOG_COMPAT_REFERENCE = 0, MOG_COMPAT_PHYSICAL = 1 };
+enum { MOG_PRECISION_FP32 = 0, MOG_PRECISION_BF16X3 = 1, MOG_PRECISION_FP16 = 2 };
+enum { MOG_SOLVER_FFT = 0, MOG_SOLVER_CONVOLUTION = 1 };
+
+/* Solver configuration.  Keys mirror config.yaml as read by
+ * main.py:115-126 and main.py:662-667 (n_grids, grid_size, world_size, G,
+ * dt, damping, solver) plus the deltaTime multiplier `timescale`
+ * (main.py:111, 662) and the build's own options (compat, precision). */
+typedef struct mog_config {
+  int32_t n_grids;    /* k in [1, 65535] */
+  int32_t grid_size;  /* g, power of two in [2, 64] */
+  double world_size;  /* W > 0, periodic cube side */
+  double G;           /* gravitational constant of greens.glsl */
+  double dt;          /* time step */
+  double damping;     /* velocity factor per step */
+  double timescale;   /* deltaTime = dt * timescale */
+  int32_t compat;     /* MOG_COMPAT_*: reference keeps the x<-z force aliasing of
+                         main.py:638-654 at g in {4,16,64}; physical does not */
+  int32_t precision;  /* MOG_PRECISION_* of the DFT contractions */
+  int32_t solver;     /* MOG_SOLVER_*; the reference runs FFT for both */
+  int32_t reserved;
+} mog_config;
+
+typedef struct mog_ctx mog_ctx;
+
+/* ABI version of the loaded library (MOG_ABI_VERSION). */
+int mog_abi_version(void);
+
+/* Validate `cfg`, allocate the context's device buffers on the current HIP
+ * device and build the FP64-derived tables.  Replaces Simulation.__init__'s
+ * buffer/texture/shader set-up (main.py:108-126, 191-248). */
+int mog_create(const mog_config* cfg, mog_ctx** out);
+
+/* Release a context (NULL is ignored). */
+void mog_destroy(mog_ctx* ctx);
+
+/* Copy per-grid transforms from HOST memory (synchronous): rot is (k,12)
+ * float32 with rot[i*12 + 4*j + m] = M_i[j][m], off is (k,4) float32.
+ * Replaces the offset/rotation SSBO uploads of main.py:196-197 fed by
+ * compute_transforms (main.py:157-172). */
+int mog_set_transforms(mog_ctx* ctx, const float* rot_k12, const float* off_k4);
+
+/* One full force step on device particle state, in place:
+ * pos (n,4) float32 = (x, y, z, mass), vel (n,4) float32 = (vx, vy, vz, 0),
+ * both 16-byte aligned.  Replaces Simulation.step / step_fft
+ * (main.py:592-672): clear, deposit, spectral solve, gather + update. */
+int mog_step(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, void* stream);
+
+/* `nsteps` consecutive steps; with use_graph != 0 one step is captured in a
+ * hipGraph on first use for these (pos, vel, n, stream) and replayed. */
+int mog_run(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, int32_t nsteps,
+            int32_t use_graph, void* stream);
+
+/* Phase entry points (tests and callers that stage the step themselves). */
+
+/* Zero `grids_out` (k*g^3 int64, [grid][z][y][x]) and deposit the particles'
+ * milli-mass uint(mass*1000) by NGP.  Replaces clear_mass (main.py:558-562)
+ * + mass.glsl dispatch (main.py:599-608). */
+int mog_deposit(mog_ctx* ctx, const float* pos_n4, int64_t n, int64_t* grids_out, void* stream);
+
+/* Force grids (k*g^3 float4 = Fx, Fy, Fz, |F|) from density grids.  Replaces
+ * complex.glsl, run_fft_3d_batched, fourier.glsl, greens.glsl, the three
+ * inverse transforms and pack.glsl (main.py:610-655). */
+int mog_solve(mog_ctx* ctx, const int64_t* grids, float* force_out, void* stream);
+
+/* Averaged world-frame force avgForce (n,4; w = 0) at the particles, from
+ * force grids.  The gather half of update.glsl:65-81. */
+int mog_gather(mog_ctx* ctx, const float* pos_n4, int64_t n, const float* force,
+               float* avg_force_n4, void* stream);
+
+/* Kick, damp, drift, wrap from a given avgForce.  The update half of
+ * update.glsl:83-89. */
+int mog_update(mog_ctx* ctx, float* pos_n4, float* vel_n4, const float* avg_force_n4, int64_t n,
+               void* stream);
+
+/* mog_step with HIP events recorded between its kernels on `stream`; waits
+ * for the stream and writes the durations in milliseconds of the phases
+ * {grid clear, deposit, spectral solve, gather+update} to phase_ms[0..3].
+ * The reference has no timing facility; this exists for the benchmark. */
+int mog_step_profiled(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, float* phase_ms,
+                      void* stream);
+
+/* Device pointers of the context's own density grids (int64) and force
+ * grids (float4) as left by the last mog_step; NULL on error. */
+const int64_t* mog_grids_ptr(mog_ctx* ctx);
+const float* mog_force_ptr(mog_ctx* ctx);
+
+/* Message of the last error of `ctx` (or of the last failed mog_create when
+ * ctx is NULL).  Never NULL. */
+const char* mog_last_error(const mog_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOG_H */

@@ -1,0 +1,453 @@
+// C ABI of the MOG force step (declared in include/mog.h).
+//
+// Owns the per-context device state: packed transforms, int64 density grids,
+// complex scratch, float4 force grids and the FP64-derived tables, and
+// sequences the kernels of one step on the caller's stream.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/mog.h"
+#include "mog_common.h"
+
+using namespace mog;
+
+struct mog_ctx {
+  mog_config cfg;
+  Params P;
+  int device;
+  int need_x;  // x slot needs its own inverse transform
+  bool have_transforms;
+  float* d_rot;
+  float* d_off;
+  long long* d_grids;
+  float2* d_S1;
+  float2* d_S2;
+  float4* d_force;
+  float2* d_tw;
+  float* d_gtab;
+  float* d_kd;
+  DepositTune tune;
+  // graph cache of mog_run
+  hipStream_t cap_stream;
+  hipGraphExec_t gexec;
+  const void* g_pos;
+  const void* g_vel;
+  long long g_n;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(mog_ctx* ctx, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+int fail(mog_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx)
+    ctx->err = buf;
+  else
+    g_create_error = buf;
+  return code;
+}
+
+int hip_fail(mog_ctx* ctx, hipError_t e, const char* what) {
+  return fail(ctx, MOG_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int ilog2(int v) {
+  int r = 0;
+  while ((1 << r) < v) ++r;
+  return r;
+}
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// The reference's x force is overwritten by its z force when the number of
+// radix-2 dispatches per 3D transform, 3*(log2 g + 1), is odd
+// (main.py:564-590 returns the last destination; main.py:638-646 passes
+// complex_tex_b as destination of both the x and the z transform).
+bool aliases_x_with_z(int g) { return ((3 * (ilog2(g) + 1)) & 1) == 1; }
+
+int validate(const mog_config* c) {
+  if (!c) return fail(nullptr, MOG_EINVAL, "config is NULL");
+  if (c->n_grids < 1 || c->n_grids > 65535)
+    return fail(nullptr, MOG_EINVAL, "n_grids must be in [1, 65535], got %d", c->n_grids);
+  if (!is_pow2(c->grid_size) || c->grid_size < 2 || c->grid_size > 64)
+    return fail(nullptr, MOG_EINVAL, "grid_size must be a power of two in [2, 64], got %d",
+                c->grid_size);
+  if (!(isfinite(c->world_size) && c->world_size > 0.0))
+    return fail(nullptr, MOG_EINVAL, "world_size must be finite and > 0");
+  if (!isfinite(c->G)) return fail(nullptr, MOG_EINVAL, "G must be finite");
+  if (!isfinite(c->dt)) return fail(nullptr, MOG_EINVAL, "dt must be finite");
+  if (!isfinite(c->damping)) return fail(nullptr, MOG_EINVAL, "damping must be finite");
+  if (!isfinite(c->timescale)) return fail(nullptr, MOG_EINVAL, "timescale must be finite");
+  if (c->compat != MOG_COMPAT_REFERENCE && c->compat != MOG_COMPAT_PHYSICAL)
+    return fail(nullptr, MOG_EINVAL, "compat must be 0 (reference) or 1 (physical)");
+  if (c->precision < MOG_PRECISION_FP32 || c->precision > MOG_PRECISION_FP16)
+    return fail(nullptr, MOG_EINVAL, "precision must be 0 (fp32), 1 (bf16x3) or 2 (fp16)");
+  if (c->precision != MOG_PRECISION_FP32)
+    return fail(nullptr, MOG_EUNSUPPORTED, "precision %d is not built in this library",
+                c->precision);
+  if (c->solver != MOG_SOLVER_FFT && c->solver != MOG_SOLVER_CONVOLUTION)
+    return fail(nullptr, MOG_EINVAL, "solver must be 0 (fft) or 1 (convolution)");
+  const double cells = (double)c->n_grids * c->grid_size * c->grid_size * c->grid_size;
+  if (cells > 2147483647.0) return fail(nullptr, MOG_EINVAL, "n_grids * grid_size^3 exceeds 2^31");
+  return MOG_OK;
+}
+
+// float32 ``k2 > 1e-4`` of greens.glsl:32-45 (PI = 3.14159265359f,
+// k = float(shifted) * (TWO_PI / L), k2 = (kx*kx + ky*ky) + kz*kz).
+bool green_mask32(int ix, int iy, int iz, int g, double W) {
+  const float two_pi = 2.0f * 3.14159265359f;
+  const float step = two_pi / (float)W;
+  auto sh = [g](int i) { return i <= g / 2 ? i : i - g; };
+  volatile float kx = (float)sh(ix) * step;
+  volatile float ky = (float)sh(iy) * step;
+  volatile float kz = (float)sh(iz) * step;
+  volatile float a = kx * kx;
+  volatile float b = ky * ky;
+  volatile float c = kz * kz;
+  volatile float k2 = a + b;
+  k2 = k2 + c;
+  return k2 > 1e-4f;
+}
+
+void build_tables(const mog_config* c, float2* tw, float* gtab, float* kd) {
+  const int g = c->grid_size;
+  const double W = c->world_size;
+  for (int j = 0; j < g; ++j) {
+    const double th = 2.0 * M_PI * (double)j / (double)g;
+    tw[j] = make_float2((float)cos(th), (float)sin(th));
+  }
+  auto sh = [g](int i) { return i <= g / 2 ? i : i - g; };
+  for (int i = 0; i < g; ++i)
+    kd[i] = (i == g / 2 && g > 1) ? 0.0f : (float)((double)sh(i) * 2.0 * M_PI / W);
+  const double scale = 1.0 / (1000.0 * (double)g * g * g);
+  for (int iz = 0; iz < g; ++iz)
+    for (int iy = 0; iy < g; ++iy)
+      for (int ix = 0; ix < g; ++ix) {
+        double v = 0.0;
+        if ((ix | iy | iz) != 0 && green_mask32(ix, iy, iz, g, W)) {
+          const double kx = sh(ix) * 2.0 * M_PI / W, ky = sh(iy) * 2.0 * M_PI / W,
+                       kz = sh(iz) * 2.0 * M_PI / W;
+          const double k2 = kx * kx + ky * ky + kz * kz;
+          v = -2.0 * M_PI * c->G / k2 * scale;
+        }
+        gtab[(iz * g + iy) * g + ix] = (float)v;
+      }
+}
+
+Params make_params(const mog_config* c) {
+  Params P;
+  P.k = c->n_grids;
+  P.g = c->grid_size;
+  P.lg = ilog2(c->grid_size);
+  P.cells = c->grid_size * c->grid_size * c->grid_size;
+  P.W = (float)c->world_size;
+  P.half = P.W * 0.5f;
+  P.h = (float)(c->world_size / (double)c->grid_size);
+  P.dtp = (float)(c->dt * c->timescale);
+  P.damping = (float)c->damping;
+  P.fk = (float)c->n_grids;
+  return P;
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  return hipMalloc((void**)p, count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+}
+
+void release(mog_ctx* c) {
+  if (!c) return;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
+  (void)hipFree(c->d_rot);
+  (void)hipFree(c->d_off);
+  (void)hipFree(c->d_grids);
+  (void)hipFree(c->d_S1);
+  (void)hipFree(c->d_S2);
+  (void)hipFree(c->d_force);
+  (void)hipFree(c->d_tw);
+  (void)hipFree(c->d_gtab);
+  (void)hipFree(c->d_kd);
+}
+
+int check_particles(mog_ctx* ctx, const void* pos, const void* vel, int64_t n, bool need_vel) {
+  if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
+  if (n < 0) return fail(ctx, MOG_EINVAL, "n must be >= 0, got %lld", (long long)n);
+  if (n > (int64_t)2147483647) return fail(ctx, MOG_EINVAL, "n must be < 2^31, got %lld", (long long)n);
+  if (n == 0) return MOG_OK;
+  if (!pos) return fail(ctx, MOG_EINVAL, "pos is NULL");
+  if (!aligned16(pos)) return fail(ctx, MOG_EINVAL, "pos must be 16-byte aligned");
+  if (need_vel) {
+    if (!vel) return fail(ctx, MOG_EINVAL, "vel is NULL");
+    if (!aligned16(vel)) return fail(ctx, MOG_EINVAL, "vel must be 16-byte aligned");
+  }
+  return MOG_OK;
+}
+
+DepositTune default_tune(const Params& P) {
+  DepositTune t;
+  t.threads = 256;
+  const int per = (int)((64 * 1024) / ((size_t)P.cells * 4));
+  t.grids_per_wg = per < 1 ? 1 : per;
+  t.wg_target = 1024;
+  const char* env = getenv("MOG_DEPOSIT_TUNE");
+  if (env) {
+    int a = 0, b = 0, c = 0;
+    if (sscanf(env, "%d,%d,%d", &a, &b, &c) == 3 && a >= 64 && a <= 1024 && (a % 64) == 0 &&
+        b >= 1 && c >= 1) {
+      t.threads = a;
+      t.grids_per_wg = b;
+      t.wg_target = c;
+    }
+  }
+  return t;
+}
+
+hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, hipStream_t s) {
+  const Params& P = ctx->P;
+  const long long cells = (long long)P.k * P.cells;
+  hipError_t e = launch_fill_zero(ctx->d_grids, cells, s);
+  if (e != hipSuccess) return e;
+  e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off,
+                     (unsigned long long*)ctx->d_grids, ctx->tune, s);
+  if (e != hipSuccess) return e;
+  e = launch_solve(P, ctx->d_grids, ctx->d_S1, ctx->d_S2, ctx->d_force, ctx->d_tw, ctx->d_gtab,
+                   ctx->d_kd, ctx->need_x, s);
+  if (e != hipSuccess) return e;
+  return launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
+                              ctx->d_force, nullptr, 0, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mog_abi_version(void) { return MOG_ABI_VERSION; }
+
+int mog_create(const mog_config* cfg, mog_ctx** out) {
+  if (!out) return fail(nullptr, MOG_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = validate(cfg);
+  if (rc != MOG_OK) return rc;
+  mog_ctx* c = new mog_ctx();
+  c->cfg = *cfg;
+  c->P = make_params(cfg);
+  c->need_x = !(cfg->compat == MOG_COMPAT_REFERENCE && aliases_x_with_z(cfg->grid_size));
+  c->tune = default_tune(c->P);
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(nullptr, MOG_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
+  }
+  const size_t k = (size_t)cfg->n_grids, g = (size_t)cfg->grid_size, cells = k * g * g * g;
+  if (dalloc(&c->d_rot, k * 12) != hipSuccess || dalloc(&c->d_off, k * 4) != hipSuccess ||
+      dalloc(&c->d_grids, cells) != hipSuccess || dalloc(&c->d_S1, cells) != hipSuccess ||
+      dalloc(&c->d_S2, c->need_x ? cells : 1) != hipSuccess ||
+      dalloc(&c->d_force, cells) != hipSuccess || dalloc(&c->d_tw, g) != hipSuccess ||
+      dalloc(&c->d_gtab, g * g * g) != hipSuccess || dalloc(&c->d_kd, g) != hipSuccess) {
+    release(c);
+    delete c;
+    return fail(nullptr, MOG_ENOMEM, "device allocation failed for k=%d g=%d", cfg->n_grids,
+                cfg->grid_size);
+  }
+  float2* tw = (float2*)malloc(g * sizeof(float2));
+  float* gtab = (float*)malloc(g * g * g * sizeof(float));
+  float* kd = (float*)malloc(g * sizeof(float));
+  build_tables(cfg, tw, gtab, kd);
+  e = hipMemcpy(c->d_tw, tw, g * sizeof(float2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_gtab, gtab, g * g * g * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_kd, kd, g * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(c->d_grids, 0, cells * sizeof(long long));
+  free(tw);
+  free(gtab);
+  free(kd);
+  if (e != hipSuccess) {
+    release(c);
+    delete c;
+    return fail(nullptr, MOG_EHIP, "table upload: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return MOG_OK;
+}
+
+void mog_destroy(mog_ctx* ctx) {
+  if (!ctx) return;
+  release(ctx);
+  delete ctx;
+}
+
+int mog_set_transforms(mog_ctx* ctx, const float* rot_k12, const float* off_k4) {
+  if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
+  if (!rot_k12 || !off_k4) return fail(ctx, MOG_EINVAL, "rot/off is NULL");
+  const size_t k = (size_t)ctx->P.k;
+  hipError_t e = hipMemcpy(ctx->d_rot, rot_k12, k * 12 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ctx->d_off, off_k4, k * 4 * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(ctx, e, "mog_set_transforms");
+  ctx->have_transforms = true;
+  return MOG_OK;
+}
+
+int mog_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, void* stream) {
+  int rc = check_particles(ctx, pos, vel, n, true);
+  if (rc != MOG_OK) return rc;
+  if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
+  if (n == 0) return MOG_OK;
+  hipError_t e = enqueue_step(ctx, pos, vel, n, (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_step");
+}
+
+int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int32_t use_graph,
+            void* stream) {
+  int rc = check_particles(ctx, pos, vel, n, true);
+  if (rc != MOG_OK) return rc;
+  if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
+  if (nsteps < 0) return fail(ctx, MOG_EINVAL, "nsteps must be >= 0");
+  if (n == 0 || nsteps == 0) return MOG_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  if (!use_graph) {
+    for (int i = 0; i < nsteps; ++i)
+      if ((e = enqueue_step(ctx, pos, vel, n, s)) != hipSuccess) return hip_fail(ctx, e, "mog_run");
+    return MOG_OK;
+  }
+  if (!ctx->gexec || ctx->g_pos != pos || ctx->g_vel != vel || ctx->g_n != n) {
+    if (ctx->gexec) {
+      (void)hipGraphExecDestroy(ctx->gexec);
+      ctx->gexec = nullptr;
+    }
+    if (!ctx->cap_stream &&
+        (e = hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking)) != hipSuccess)
+      return hip_fail(ctx, e, "hipStreamCreate");
+    if ((e = hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hip_fail(ctx, e, "hipStreamBeginCapture");
+    hipError_t ee = enqueue_step(ctx, pos, vel, n, ctx->cap_stream);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(ctx->cap_stream, &graph);
+    if (ee != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return hip_fail(ctx, ee, "capture");
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      ctx->gexec = nullptr;
+      return hip_fail(ctx, e, "hipGraphInstantiate");
+    }
+    ctx->g_pos = pos;
+    ctx->g_vel = vel;
+    ctx->g_n = n;
+  }
+  for (int i = 0; i < nsteps; ++i)
+    if ((e = hipGraphLaunch(ctx->gexec, s)) != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+  return MOG_OK;
+}
+
+int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* phase_ms,
+                      void* stream) {
+  int rc = check_particles(ctx, pos, vel, n, true);
+  if (rc != MOG_OK) return rc;
+  if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
+  if (!phase_ms) return fail(ctx, MOG_EINVAL, "phase_ms is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  const Params& P = ctx->P;
+  hipEvent_t ev[5] = {};
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+  if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+  if (e == hipSuccess) e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
+  if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+  if (e == hipSuccess)
+    e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off,
+                       (unsigned long long*)ctx->d_grids, ctx->tune, s);
+  if (e == hipSuccess) e = hipEventRecord(ev[2], s);
+  if (e == hipSuccess)
+    e = launch_solve(P, ctx->d_grids, ctx->d_S1, ctx->d_S2, ctx->d_force, ctx->d_tw, ctx->d_gtab,
+                     ctx->d_kd, ctx->need_x, s);
+  if (e == hipSuccess) e = hipEventRecord(ev[3], s);
+  if (e == hipSuccess)
+    e = launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
+                             ctx->d_force, nullptr, 0, s);
+  if (e == hipSuccess) e = hipEventRecord(ev[4], s);
+  if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
+  for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventElapsedTime(&phase_ms[i], ev[i], ev[i + 1]);
+  for (int i = 0; i < 5; ++i)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_step_profiled");
+}
+
+int mog_deposit(mog_ctx* ctx, const float* pos, int64_t n, int64_t* grids_out, void* stream) {
+  int rc = check_particles(ctx, pos, nullptr, n, false);
+  if (rc != MOG_OK) return rc;
+  if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
+  if (!grids_out) return fail(ctx, MOG_EINVAL, "grids_out is NULL");
+  if (((uintptr_t)grids_out & 7u) != 0) return fail(ctx, MOG_EINVAL, "grids_out must be 8-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  const long long cells = (long long)ctx->P.k * ctx->P.cells;
+  hipError_t e = launch_fill_zero((long long*)grids_out, cells, s);
+  if (e == hipSuccess)
+    e = launch_deposit(ctx->P, (const float4*)pos, n, ctx->d_rot, ctx->d_off,
+                       (unsigned long long*)grids_out, ctx->tune, s);
+  return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_deposit");
+}
+
+int mog_solve(mog_ctx* ctx, const int64_t* grids, float* force_out, void* stream) {
+  if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
+  if (!grids || !force_out) return fail(ctx, MOG_EINVAL, "grids/force_out is NULL");
+  if (((uintptr_t)grids & 7u) != 0) return fail(ctx, MOG_EINVAL, "grids must be 8-byte aligned");
+  if (!aligned16(force_out)) return fail(ctx, MOG_EINVAL, "force_out must be 16-byte aligned");
+  hipError_t e = launch_solve(ctx->P, (const long long*)grids, ctx->d_S1, ctx->d_S2,
+                              (float4*)force_out, ctx->d_tw, ctx->d_gtab, ctx->d_kd, ctx->need_x,
+                              (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_solve");
+}
+
+int mog_gather(mog_ctx* ctx, const float* pos, int64_t n, const float* force, float* avg_out,
+               void* stream) {
+  int rc = check_particles(ctx, pos, nullptr, n, false);
+  if (rc != MOG_OK) return rc;
+  if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
+  if (n == 0) return MOG_OK;
+  if (!force || !avg_out) return fail(ctx, MOG_EINVAL, "force/avg_out is NULL");
+  if (!aligned16(force) || !aligned16(avg_out))
+    return fail(ctx, MOG_EINVAL, "force/avg_out must be 16-byte aligned");
+  hipError_t e = launch_gather_update(ctx->P, (float4*)pos, nullptr, n, ctx->d_rot, ctx->d_off,
+                                      (const float4*)force, (float4*)avg_out, 1,
+                                      (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_gather");
+}
+
+int mog_update(mog_ctx* ctx, float* pos, float* vel, const float* avg, int64_t n, void* stream) {
+  int rc = check_particles(ctx, pos, vel, n, true);
+  if (rc != MOG_OK) return rc;
+  if (n == 0) return MOG_OK;
+  if (!avg) return fail(ctx, MOG_EINVAL, "avg_force is NULL");
+  if (!aligned16(avg)) return fail(ctx, MOG_EINVAL, "avg_force must be 16-byte aligned");
+  hipError_t e = launch_update(ctx->P, (float4*)pos, (float4*)vel, (const float4*)avg, n,
+                               (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_update");
+}
+
+const int64_t* mog_grids_ptr(mog_ctx* ctx) { return ctx ? (const int64_t*)ctx->d_grids : nullptr; }
+
+const float* mog_force_ptr(mog_ctx* ctx) { return ctx ? (const float*)ctx->d_force : nullptr; }
+
+const char* mog_last_error(const mog_ctx* ctx) {
+  if (!ctx) return g_create_error.c_str();
+  return ctx->err.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// Force gather + integrator of the MOG step (replaces shaders/update.glsl).
+//
+// One thread per particle reads its position and velocity once, loops over
+// the k grids (the packed rotation rows and offsets are wave-uniform, so they
+// come through the scalar cache), recomputes the NGP cell with the same exact
+// float32 sequence as the deposit, fetches the float4 force of that cell,
+// rotates it back to the world frame (world_m = sum_j R[4m+j] F_j, i.e.
+// transpose(rot) * F of update.glsl:77), averages over k and applies
+// kick, damping, drift and the periodic wrap (update.glsl:81-89).
+#include "mog_common.h"
+
+namespace mog {
+
+__device__ __forceinline__ void integrate(float4& p, float4& v, float ax, float ay, float az,
+                                          const Params& P) {
+  // vel -= avgForce * deltaTime; vel *= damping; pos += vel * deltaTime;
+  // pos = mod(pos, worldSize)  (update.glsl:83-86), each op rounded.
+  float vx = v.x - ax * P.dtp;
+  float vy = v.y - ay * P.dtp;
+  float vz = v.z - az * P.dtp;
+  vx = vx * P.damping;
+  vy = vy * P.damping;
+  vz = vz * P.damping;
+  float x = p.x + vx * P.dtp;
+  float y = p.y + vy * P.dtp;
+  float z = p.z + vz * P.dtp;
+  p = make_float4(glsl_mod(x, P.W), glsl_mod(y, P.W), glsl_mod(z, P.W), p.w);
+  v = make_float4(vx, vy, vz, 0.0f);
+}
+
+// mode 0: update pos/vel in place; mode 1: write avgForce to avg_out.
+__global__ __launch_bounds__(256) void gather_update_kernel(
+    float4* __restrict__ pos, float4* __restrict__ vel, long long n,
+    const float* __restrict__ rot, const float* __restrict__ off,
+    const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = pos[i];
+  const float c0 = p.x - P.half;
+  const float c1 = p.y - P.half;
+  const float c2 = p.z - P.half;
+  float tx = 0.f, ty = 0.f, tz = 0.f;
+  for (int grid = 0; grid < P.k; ++grid) {
+    const float* R = rot + 12 * grid;
+    const int cell = cell_of(c0, c1, c2, R, off + 4 * grid, P);
+    const float4 F = force[(long long)grid * P.cells + cell];
+    tx += fmaf(R[0], F.x, fmaf(R[1], F.y, R[2] * F.z));
+    ty += fmaf(R[4], F.x, fmaf(R[5], F.y, R[6] * F.z));
+    tz += fmaf(R[8], F.x, fmaf(R[9], F.y, R[10] * F.z));
+  }
+  const float ax = tx / P.fk, ay = ty / P.fk, az = tz / P.fk;
+  if (mode == 1) {
+    avg_out[i] = make_float4(ax, ay, az, 0.0f);
+    return;
+  }
+  float4 v = vel[i];
+  integrate(p, v, ax, ay, az, P);
+  pos[i] = p;
+  vel[i] = v;
+}
+
+__global__ __launch_bounds__(256) void update_kernel(float4* __restrict__ pos,
+                                                     float4* __restrict__ vel,
+                                                     const float4* __restrict__ avg, long long n,
+                                                     Params P) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = pos[i];
+  float4 v = vel[i];
+  const float4 a = avg[i];
+  integrate(p, v, a.x, a.y, a.z, P);
+  pos[i] = p;
+  vel[i] = v;
+}
+
+hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
+                                const float* rot, const float* off, const float4* force,
+                                float4* avg_out, int mode, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long long blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(gather_update_kernel, dim3((unsigned)blocks), dim3(256), 0, s, pos, vel, n,
+                     rot, off, force, P, avg_out, mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(const Params& P, float4* pos, float4* vel, const float4* avg,
+                         long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long long blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(update_kernel, dim3((unsigned)blocks), dim3(256), 0, s, pos, vel, avg, n, P);
+  return hipGetLastError();
+}
+
+}  // namespace mog

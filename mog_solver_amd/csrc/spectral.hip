@@ -1,0 +1,222 @@
+// Spectral Poisson solve of the MOG step, FP32 vector path.
+//
+// Replaces complex.glsl (u32 -> float/1000), the 3-axis radix-2 transforms of
+// run_fft_3d_batched (main.py:564-590, fft_batched.glsl), the DC removal of
+// fourier.glsl, the Green's multiply and i*k gradients of greens.glsl:25-61,
+// the three inverse transforms (main.py:638-646) and pack.glsl.
+//
+// Each length-g transform is a dense DFT contraction with a twiddle table
+// (FP64-derived, stored FP32).  The 3D transform of a grid is split into
+// three passes over an L2-resident complex scratch buffer:
+//   A  per (grid, z) slab: real x-DFT, y-DFT                      -> S1
+//   B  per (grid, ky) slab: z-DFT, Green x i*k, inverse z-DFT      -> S1 (y+iz), S2 (x)
+//   C  per (grid, z) slab: inverse y-DFT, inverse x-DFT, pack float4
+// Two real force components share one complex inverse transform:
+// IDFT(A_y + i A_z) = F_y + i F_z, where A_d is the gradient spectrum with
+// its Nyquist plane along d removed.  That plane is anti-Hermitian and only
+// feeds the imaginary part the reference discards (pack.glsl:23-26), so the
+// real parts are unchanged.  The constant factors (1/1000 of complex.glsl,
+// -2*pi*G/k^2 of greens.glsl, 1/g^3 of pack.glsl) are folded into one FP64
+// table `gtab`.
+#include "mog_common.h"
+
+namespace mog {
+
+// e^{-i theta} * a for the forward sign; tw[j] = (cos, sin)(2*pi*j/g).
+__device__ __forceinline__ float2 cmul_conj_tw(float2 a, float2 w) {
+  // a * (w.x - i w.y)
+  return make_float2(fmaf(a.x, w.x, a.y * w.y), fmaf(a.y, w.x, -(a.x * w.y)));
+}
+__device__ __forceinline__ float2 cmul_tw(float2 a, float2 w) {
+  // a * (w.x + i w.y)
+  return make_float2(fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.y, w.x, a.x * w.y));
+}
+
+// Pass A: one workgroup per (grid, z) slab.
+__global__ __launch_bounds__(256) void spec_fwd_xy_kernel(const long long* __restrict__ mass,
+                                                          float2* __restrict__ S,
+                                                          const float2* __restrict__ tw, int g,
+                                                          int lg) {
+  extern __shared__ float2 sm[];
+  float2* twl = sm;
+  float* re = (float*)(sm + g);          // g*g reals
+  float2* B = sm + g + (g * g + 1) / 2;  // g*g complex
+  const int gg = g * g, m = g - 1;
+  const long long base = (long long)blockIdx.x * gg;
+  for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
+  for (int i = threadIdx.x; i < gg; i += blockDim.x) re[i] = (float)mass[base + i];
+  __syncthreads();
+  // x: B[y][kx] = sum_x re[y][x] e^{-2 pi i x kx / g}
+  for (int o = threadIdx.x; o < gg; o += blockDim.x) {
+    const int y = o >> lg, kx = o & m;
+    float ar = 0.f, ai = 0.f;
+    for (int x = 0; x < g; ++x) {
+      const float a = re[y * g + x];
+      const float2 w = twl[(x * kx) & m];
+      ar = fmaf(a, w.x, ar);
+      ai = fmaf(-a, w.y, ai);
+    }
+    B[o] = make_float2(ar, ai);
+  }
+  __syncthreads();
+  // y: S[ky][kx] = sum_y B[y][kx] e^{-2 pi i y ky / g}
+  for (int o = threadIdx.x; o < gg; o += blockDim.x) {
+    const int ky = o >> lg, kx = o & m;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int y = 0; y < g; ++y) {
+      const float2 v = cmul_conj_tw(B[y * g + kx], twl[(y * ky) & m]);
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    S[base + o] = acc;
+  }
+}
+
+// Pass B: one workgroup per (grid, ky): columns along z for every kx.
+__global__ __launch_bounds__(256) void spec_z_green_kernel(float2* __restrict__ S,
+                                                           float2* __restrict__ S2,
+                                                           const float2* __restrict__ tw,
+                                                           const float* __restrict__ gtab,
+                                                           const float* __restrict__ kd, int g,
+                                                           int lg, int need_x) {
+  extern __shared__ float2 sm[];
+  float2* twl = sm;
+  float2* L = sm + g;           // [z][kx]
+  float2* Z = L + g * g;        // [kz][kx]  (y + i z spectrum)
+  float2* X = Z + g * g;        // [kz][kx]  (x spectrum)
+  const int gg = g * g, m = g - 1;
+  const int grid = blockIdx.x >> lg;
+  const int ky = blockIdx.x & m;
+  const long long gbase = (long long)grid * g * gg;
+  for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
+  for (int i = threadIdx.x; i < gg; i += blockDim.x) {
+    const int z = i >> lg, kx = i & m;
+    L[i] = S[gbase + (long long)z * gg + ky * g + kx];
+  }
+  __syncthreads();
+  const float kyv = kd[ky];
+  for (int o = threadIdx.x; o < gg; o += blockDim.x) {
+    const int kz = o >> lg, kx = o & m;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int z = 0; z < g; ++z) {
+      const float2 v = cmul_conj_tw(L[z * g + kx], twl[(z * kz) & m]);
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    const float gf = gtab[(kz * g + ky) * g + kx];
+    const float pr = acc.x * gf, pi = acc.y * gf;
+    // Z = phi * (-kz' + i ky')
+    const float kzv = kd[kz];
+    Z[o] = make_float2(fmaf(-pr, kzv, -(pi * kyv)), fmaf(pr, kyv, -(pi * kzv)));
+    if (need_x) {
+      const float kxv = kd[kx];
+      X[o] = make_float2(-(pi * kxv), pr * kxv);
+    }
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < gg; o += blockDim.x) {
+    const int z = o >> lg, kx = o & m;
+    float2 az = make_float2(0.f, 0.f), ax = make_float2(0.f, 0.f);
+    for (int kz = 0; kz < g; ++kz) {
+      const float2 w = twl[(z * kz) & m];
+      const float2 v = cmul_tw(Z[kz * g + kx], w);
+      az.x += v.x;
+      az.y += v.y;
+      if (need_x) {
+        const float2 u = cmul_tw(X[kz * g + kx], w);
+        ax.x += u.x;
+        ax.y += u.y;
+      }
+    }
+    const long long at = gbase + (long long)z * gg + ky * g + kx;
+    S[at] = az;
+    if (need_x) S2[at] = ax;
+  }
+}
+
+// Pass C: one workgroup per (grid, z): inverse y, inverse x, pack float4.
+__global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restrict__ S,
+                                                          const float2* __restrict__ S2,
+                                                          float4* __restrict__ force,
+                                                          const float2* __restrict__ tw, int g,
+                                                          int lg, int need_x) {
+  extern __shared__ float2 sm[];
+  float2* twl = sm;
+  float2* Z = sm + g;       // [ky][kx]
+  float2* X = Z + g * g;    // [ky][kx]
+  float2* TZ = X + g * g;   // [y][kx]
+  float2* TX = TZ + g * g;  // [y][kx]
+  const int gg = g * g, m = g - 1;
+  const long long base = (long long)blockIdx.x * gg;
+  for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
+  for (int i = threadIdx.x; i < gg; i += blockDim.x) {
+    Z[i] = S[base + i];
+    if (need_x) X[i] = S2[base + i];
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < gg; o += blockDim.x) {
+    const int y = o >> lg, kx = o & m;
+    float2 az = make_float2(0.f, 0.f), ax = make_float2(0.f, 0.f);
+    for (int ky = 0; ky < g; ++ky) {
+      const float2 w = twl[(y * ky) & m];
+      const float2 v = cmul_tw(Z[ky * g + kx], w);
+      az.x += v.x;
+      az.y += v.y;
+      if (need_x) {
+        const float2 u = cmul_tw(X[ky * g + kx], w);
+        ax.x += u.x;
+        ax.y += u.y;
+      }
+    }
+    TZ[o] = az;
+    if (need_x) TX[o] = ax;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < gg; o += blockDim.x) {
+    const int y = o >> lg, x = o & m;
+    float fy = 0.f, fz = 0.f, fx = 0.f;
+    for (int kx = 0; kx < g; ++kx) {
+      const float2 w = twl[(x * kx) & m];
+      const float2 v = cmul_tw(TZ[y * g + kx], w);
+      fy += v.x;
+      fz += v.y;
+      if (need_x) {
+        const float2 u = TX[y * g + kx];
+        fx = fmaf(u.x, w.x, fmaf(-u.y, w.y, fx));
+      }
+    }
+    if (!need_x) fx = fz;
+    const float mag = sqrtf(fmaf(fx, fx, fmaf(fy, fy, fz * fz)));
+    force[base + o] = make_float4(fx, fy, fz, mag);
+  }
+}
+
+hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
+                        float4* force, const float2* tw, const float* gtab, const float* kd,
+                        int need_x, hipStream_t s) {
+  const int g = P.g, lg = P.lg, gg = g * g;
+  const unsigned slabs = (unsigned)(P.k * g);
+  const size_t ldsA = (size_t)(g + (gg + 1) / 2 + gg) * sizeof(float2);
+  const size_t ldsB = (size_t)(g + 3 * gg) * sizeof(float2);
+  const size_t ldsC = (size_t)(g + 4 * gg) * sizeof(float2);
+  hipError_t e;
+  if ((e = hipFuncSetAttribute((const void*)spec_fwd_xy_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)) != hipSuccess)
+    return e;
+  if ((e = hipFuncSetAttribute((const void*)spec_z_green_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)) != hipSuccess)
+    return e;
+  if ((e = hipFuncSetAttribute((const void*)spec_inv_xy_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(spec_fwd_xy_kernel, dim3(slabs), dim3(256), ldsA, s, grids, S1, tw, g, lg);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(spec_z_green_kernel, dim3(slabs), dim3(256), ldsB, s, S1, S2, tw, gtab, kd, g,
+                     lg, need_x);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(spec_inv_xy_kernel, dim3(slabs), dim3(256), ldsC, s, S1, S2, force, tw, g,
+                     lg, need_x);
+  return hipGetLastError();
+}
+
+}  // namespace mog

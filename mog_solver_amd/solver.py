@@ -1,0 +1,261 @@
+"""Python solver interface over the HIP C ABI.
+
+``MogSolver`` owns one ``mog_ctx`` (device grids, tables, transforms) and
+runs the phases or whole steps on device tensors.  ``Simulation`` is the
+headless counterpart of the reference's ``Simulation`` class
+(main.py:101-672): same configuration keys, attribute names
+(``cfg``, ``n_grids``, ``grid_size``, ``world_size``, ``voxel_size``,
+``num_particles``, ``G``, ``dt``, ``timescale``, ``offsets``, ``rotations``,
+``use_convolution``, ``pos_buf``, ``vel_buf``) and step methods
+(``step``, ``step_fft``, ``clear_mass``, ``compute_transforms``,
+``load_config``), without the window, renderer and key handling.
+
+Device tensors are torch tensors on a HIP device; torch supplies memory and
+the current stream only, every computation runs in ``libmog.so``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from .config import SolverConfig, load_config
+from .transforms import compute_transforms
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _stream_ptr(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _check_particles(t, name):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a device tensor")
+    if t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != 4 or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous (n,4) float32 tensor, got {tuple(t.shape)} {t.dtype}")
+
+
+class MogSolver:
+    """One MOG force-step context (``mog_create`` ... ``mog_destroy``)."""
+
+    def __init__(self, cfg: SolverConfig, rot=None, off=None, seed: int = 0):
+        self.lib = _abi.load()
+        self.cfg = cfg
+        c = _abi.MogConfig(
+            n_grids=cfg.n_grids,
+            grid_size=cfg.grid_size,
+            world_size=cfg.world_size,
+            G=cfg.G,
+            dt=cfg.dt,
+            damping=cfg.damping,
+            timescale=cfg.timescale,
+            compat=_abi.COMPAT[cfg.compat],
+            precision=_abi.PRECISION[cfg.precision],
+            solver=_abi.SOLVER[cfg.solver],
+            reserved=0,
+        )
+        h = ctypes.c_void_p()
+        rc = self.lib.mog_create(ctypes.byref(c), ctypes.byref(h))
+        _abi.check(self.lib, None, rc)
+        self._ctx = h
+        if rot is None or off is None:
+            off, rot = compute_transforms(cfg.n_grids, cfg.grid_size, cfg.world_size, seed=seed)
+        self.set_transforms(rot, off)
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self.lib.mog_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        _abi.check(self.lib, self._ctx, rc)
+
+    # -- configuration ------------------------------------------------------
+    def set_transforms(self, rot, off):
+        """Upload packed (k,12) rotations and (k,4) offsets (host arrays)."""
+        rot = np.ascontiguousarray(rot, dtype=np.float32)
+        off = np.ascontiguousarray(off, dtype=np.float32)
+        k = self.cfg.n_grids
+        if rot.shape != (k, 12) or off.shape != (k, 4):
+            raise ValueError(f"expected rot (k,12) and off (k,4) with k={k}, got {rot.shape}, {off.shape}")
+        self.rot = rot
+        self.off = off
+        self._check(self.lib.mog_set_transforms(self._ctx, rot.ctypes.data, off.ctypes.data))
+
+    @property
+    def cells(self):
+        g = self.cfg.grid_size
+        return self.cfg.n_grids * g * g * g
+
+    # -- whole step ---------------------------------------------------------
+    def step(self, pos, vel, stream=None):
+        _check_particles(pos, "pos")
+        _check_particles(vel, "vel")
+        if pos.shape[0] != vel.shape[0]:
+            raise ValueError("pos and vel must have the same length")
+        self._check(self.lib.mog_step(self._ctx, _ptr(pos), _ptr(vel), pos.shape[0], _stream_ptr(stream)))
+
+    def run(self, pos, vel, nsteps, use_graph=True, stream=None):
+        _check_particles(pos, "pos")
+        _check_particles(vel, "vel")
+        if pos.shape[0] != vel.shape[0]:
+            raise ValueError("pos and vel must have the same length")
+        self._check(
+            self.lib.mog_run(self._ctx, _ptr(pos), _ptr(vel), pos.shape[0], int(nsteps), int(bool(use_graph)),
+                             _stream_ptr(stream))
+        )
+
+    def step_profiled(self, pos, vel, stream=None):
+        """One step; returns the HIP-event durations (ms) of its phases
+        {clear, deposit, solve, gather_update} (synchronises the stream)."""
+        _check_particles(pos, "pos")
+        _check_particles(vel, "vel")
+        ms = (ctypes.c_float * 4)()
+        self._check(self.lib.mog_step_profiled(self._ctx, _ptr(pos), _ptr(vel), pos.shape[0],
+                                               ctypes.cast(ms, ctypes.c_void_p), _stream_ptr(stream)))
+        return dict(zip(("clear", "deposit", "solve", "gather_update"), list(ms)))
+
+    # -- phases -------------------------------------------------------------
+    def deposit(self, pos, out=None, stream=None):
+        """int64 milli-mass grids (k, g, g, g) [grid][z][y][x]."""
+        torch = _torch()
+        _check_particles(pos, "pos")
+        g = self.cfg.grid_size
+        if out is None:
+            out = torch.empty((self.cfg.n_grids, g, g, g), dtype=torch.int64, device=pos.device)
+        self._check(self.lib.mog_deposit(self._ctx, _ptr(pos), pos.shape[0], _ptr(out), _stream_ptr(stream)))
+        return out
+
+    def solve(self, grids, out=None, stream=None):
+        """float32 force grids (k, g, g, g, 4) = (Fx, Fy, Fz, |F|)."""
+        torch = _torch()
+        g = self.cfg.grid_size
+        if grids.dtype != torch.int64 or grids.numel() != self.cells or not grids.is_contiguous():
+            raise ValueError("grids must be a contiguous int64 tensor of k*g^3 elements")
+        if out is None:
+            out = torch.empty((self.cfg.n_grids, g, g, g, 4), dtype=torch.float32, device=grids.device)
+        self._check(self.lib.mog_solve(self._ctx, _ptr(grids), _ptr(out), _stream_ptr(stream)))
+        return out
+
+    def gather(self, pos, force, out=None, stream=None):
+        """avgForce (n,4) float32 (w = 0) from force grids."""
+        torch = _torch()
+        _check_particles(pos, "pos")
+        if force.dtype != torch.float32 or force.numel() != 4 * self.cells or not force.is_contiguous():
+            raise ValueError("force must be a contiguous float32 tensor of k*g^3*4 elements")
+        if out is None:
+            out = torch.empty_like(pos)
+        self._check(self.lib.mog_gather(self._ctx, _ptr(pos), pos.shape[0], _ptr(force), _ptr(out),
+                                        _stream_ptr(stream)))
+        return out
+
+    def update(self, pos, vel, avg_force, stream=None):
+        _check_particles(pos, "pos")
+        _check_particles(vel, "vel")
+        _check_particles(avg_force, "avg_force")
+        self._check(self.lib.mog_update(self._ctx, _ptr(pos), _ptr(vel), _ptr(avg_force), pos.shape[0],
+                                        _stream_ptr(stream)))
+
+    def forces(self, pos, stream=None):
+        """avgForce of the whole pipeline (deposit, solve, gather) at ``pos``."""
+        grids = self.deposit(pos, stream=stream)
+        force = self.solve(grids, stream=stream)
+        return self.gather(pos, force, stream=stream)
+
+
+class Simulation:
+    """Headless MOG simulation with the reference's configuration and names.
+
+    ``config`` is a dict of config.yaml keys or a path to such a file (the
+    reference's fallback dict when the path does not exist).  Particle state
+    is set with ``init_particles(pos, vel)`` from host arrays or device
+    tensors of shape (n,4).
+    """
+
+    def __init__(self, config=None, seed: int = 0, device=None):
+        self.cfg = config if isinstance(config, dict) else self.load_config(config)
+        sc = SolverConfig.from_dict(self.cfg)
+        self.solver_cfg = sc
+        self.timescale = sc.timescale
+        self.paused = False
+        self.n_grids = sc.n_grids
+        self.grid_size = sc.grid_size
+        self.world_size = sc.world_size
+        self.voxel_size = self.world_size / self.grid_size
+        self.num_particles = sc.num_particles
+        self.G = sc.G
+        self.dt = sc.dt
+        self.seed = seed
+        self.offsets, self.rotations = self.compute_transforms()
+        self.use_convolution = sc.use_convolution
+        self.device = device
+        self.solver = MogSolver(sc, rot=self.rotations, off=self.offsets)
+        self.pos_buf = None
+        self.vel_buf = None
+
+    @staticmethod
+    def load_config(path=None):
+        return load_config(path)
+
+    def compute_transforms(self):
+        """Packed (offsets (k,4), rotations (k,12)) from the seeded sampler."""
+        return compute_transforms(self.n_grids, self.grid_size, self.world_size, seed=self.seed)
+
+    def init_particles(self, pos, vel=None):
+        torch = _torch()
+        dev = self.device or torch.device("cuda", torch.cuda.current_device())
+        pos_t = torch.as_tensor(np.asarray(pos, dtype=np.float32) if not isinstance(pos, torch.Tensor) else pos)
+        pos_t = pos_t.to(device=dev, dtype=torch.float32).contiguous()
+        if vel is None:
+            vel_t = torch.zeros_like(pos_t)
+        else:
+            vel_t = torch.as_tensor(np.asarray(vel, dtype=np.float32) if not isinstance(vel, torch.Tensor) else vel)
+            vel_t = vel_t.to(device=dev, dtype=torch.float32).contiguous()
+        self.pos_buf, self.vel_buf = pos_t, vel_t
+        self.num_particles = pos_t.shape[0]
+
+    def clear_mass(self):
+        """Kept for interface parity: the step clears its grids itself."""
+
+    def step_fft(self):
+        if self.pos_buf is None:
+            raise RuntimeError("init_particles() first")
+        if self.solver.cfg.timescale != self.timescale:
+            raise RuntimeError("timescale changed: create a new Simulation or set_timescale()")
+        self.solver.step(self.pos_buf, self.vel_buf)
+
+    def step(self):
+        """The reference always runs the FFT solver (main.py:671-672)."""
+        if not self.paused:
+            self.step_fft()
+
+    def run(self, nsteps, use_graph=True):
+        if self.pos_buf is None:
+            raise RuntimeError("init_particles() first")
+        self.solver.run(self.pos_buf, self.vel_buf, nsteps, use_graph=use_graph)
