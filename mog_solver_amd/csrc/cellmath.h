@@ -1,0 +1,110 @@
+// NGP cell arithmetic of the MOG step, shared by the device kernels and a
+// host-side exhaustiveness check (plain C++, no HIP types).
+//
+// Exact path: the float32 sequence of mass.glsl:42-50 / update.glsl:71-76
+//   c = p - W/2;  r_m = (R[m]*c0 + R[4+m]*c1) + R[8+m]*c2;  l = (r + W/2) - off
+//   l = mod(mod(l, W) + W, W);  cell = int(l / h) % g
+// with every operation rounded separately (the library is built with
+// -ffp-contract=off) and IEEE division.
+//
+// Fast path: v_m = l_m / h evaluated as three FMAs with per-grid coefficients
+// F[4m+j] = RN(R[4j+m] / h), F[4m+3] = RN((W/2 - off_m) / h); the cell is
+// floor(v) mod g.  Both paths differ from the real-number value v* by at
+// most a bound E (derived in fast_margin below); when v lies farther than
+// M >= E_exact + E_fast from every integer, floor(v) mod g equals the exact
+// path's cell, so the fast result is used; otherwise the exact sequence runs.
+// The bound assumes |c_j| <= CB for the particle, which is checked per
+// particle (non-finite coordinates fail the check and take the exact path).
+#pragma once
+
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define MOG_HD __host__ __device__ __forceinline__
+#else
+#define MOG_HD inline
+#endif
+
+namespace mog {
+
+// GLSL mod(x, y) = x - y * floor(x / y).
+MOG_HD float glsl_mod_f(float x, float y) {
+  float t = floorf(x / y);
+  float p = y * t;
+  return x - p;
+}
+
+MOG_HD int axis_cell_exact(float l, float W, float h, int gmask) {
+  l = glsl_mod_f(l, W);
+  l = l + W;
+  l = glsl_mod_f(l, W);
+  float q = l / h;
+  int i = (int)q;
+  return i & gmask;
+}
+
+// Exact cell from the centred position and the packed rotation row R
+// (R[4j+m] = M[j][m]) and offset O of one grid.
+MOG_HD int cell_exact(float c0, float c1, float c2, const float* R, const float* O, float W,
+                      float half, float h, int lg) {
+  float r0 = R[0] * c0;
+  r0 = r0 + R[4] * c1;
+  r0 = r0 + R[8] * c2;
+  float r1 = R[1] * c0;
+  r1 = r1 + R[5] * c1;
+  r1 = r1 + R[9] * c2;
+  float r2 = R[2] * c0;
+  r2 = r2 + R[6] * c1;
+  r2 = r2 + R[10] * c2;
+  float l0 = (r0 + half) - O[0];
+  float l1 = (r1 + half) - O[1];
+  float l2 = (r2 + half) - O[2];
+  const int gm = (1 << lg) - 1;
+  int x = axis_cell_exact(l0, W, h, gm);
+  int y = axis_cell_exact(l1, W, h, gm);
+  int z = axis_cell_exact(l2, W, h, gm);
+  return (((z << lg) | y) << lg) | x;
+}
+
+// Fast cell; returns false when the exact path must decide.
+// F is the 12-float fast row of the grid, lim = 0.5 - M.
+MOG_HD bool cell_fast(float c0, float c1, float c2, const float* F, float lim, int lg, int* cell) {
+  const float v0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, F[3])));
+  const float v1 = fmaf(F[6], c2, fmaf(F[5], c1, fmaf(F[4], c0, F[7])));
+  const float v2 = fmaf(F[10], c2, fmaf(F[9], c1, fmaf(F[8], c0, F[11])));
+  const float f0 = floorf(v0), f1 = floorf(v1), f2 = floorf(v2);
+  const bool ok = (fabsf((v0 - f0) - 0.5f) < lim) && (fabsf((v1 - f1) - 0.5f) < lim) &&
+                  (fabsf((v2 - f2) - 0.5f) < lim);
+  const int gm = (1 << lg) - 1;
+  *cell = (((((int)f2 & gm) << lg) | ((int)f1 & gm)) << lg) | ((int)f0 & gm);
+  return ok;
+}
+
+// Per-particle admission to the fast path: every |c_j| <= CB (NaN fails).
+MOG_HD bool particle_fast(float c0, float c1, float c2, float CB) {
+  return (fabsf(c0) <= CB) && (fabsf(c1) <= CB) && (fabsf(c2) <= CB);
+}
+
+// Error margin M of the fast path (in cell units) for transforms whose
+// entries satisfy |R| <= rmax and |off| <= omax, particles with
+// |c_j| <= CB = 1.5 W, cell size h = W / g.  With u = 2^-24 and T = rmax*CB:
+//   exact path, l: products, partial sums, +W/2, -off     u (14 T + W + omax)
+//   exact path, mod chain (|floor| <= L/W, L = 3T + W/2 + omax)  u (L + 4 W)
+//   exact path, l / h                                   u g      (cell units)
+//   fast path, coefficient rounding                     u (3 T + W/2 + omax)
+//   fast path, three FMAs                               u (6 T + 3 (W/2 + omax))
+// M = 3 x (sum / h + u g); returns a value >= 0.05 when the fast path should
+// not be used.
+inline double fast_margin(double W, int g, double rmax, double omax) {
+  const double u = 1.0 / 16777216.0;
+  const double h = W / g;
+  const double CB = 1.5 * W;
+  const double T = rmax * CB;
+  const double L = 3.0 * T + 0.5 * W + omax;
+  const double exact_l = 14.0 * T + W + omax;
+  const double exact_mod = L + 4.0 * W;
+  const double fast = 3.0 * T + 0.5 * W + omax + 6.0 * T + 3.0 * (0.5 * W + omax);
+  return 3.0 * (u * (exact_l + exact_mod + fast) / h + u * g);
+}
+
+}  // namespace mog

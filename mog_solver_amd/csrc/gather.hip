@@ -2,8 +2,8 @@
 //
 // One thread per particle reads its position and velocity once, loops over
 // the k grids (the packed rotation rows and offsets are wave-uniform, so they
-// come through the scalar cache), recomputes the NGP cell with the same exact
-// float32 sequence as the deposit, fetches the float4 force of that cell,
+// come through the scalar cache), recomputes the NGP cell exactly as the
+// deposit does (same cell for the same position), fetches the float4 force of that cell,
 // rotates it back to the world frame (world_m = sum_j R[4m+j] F_j, i.e.
 // transpose(rot) * F of update.glsl:77), averages over k and applies
 // kick, damping, drift and the periodic wrap (update.glsl:81-89).
@@ -24,25 +24,31 @@ __device__ __forceinline__ void integrate(float4& p, float4& v, float ax, float 
   float x = p.x + vx * P.dtp;
   float y = p.y + vy * P.dtp;
   float z = p.z + vz * P.dtp;
-  p = make_float4(glsl_mod(x, P.W), glsl_mod(y, P.W), glsl_mod(z, P.W), p.w);
+  p = make_float4(glsl_mod_f(x, P.W), glsl_mod_f(y, P.W), glsl_mod_f(z, P.W), p.w);
   v = make_float4(vx, vy, vz, 0.0f);
 }
 
 // mode 0: update pos/vel in place; mode 1: write avgForce to avg_out.
+// With CACHE the cells come from cells_in[grid * n + i], stored by the
+// deposit of the same step for the same positions.
+template <bool CACHE>
 __global__ __launch_bounds__(256) void gather_update_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, long long n,
-    const float* __restrict__ rot, const float* __restrict__ off,
-    const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode) {
+    const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
+    const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode,
+    const unsigned short* __restrict__ cells_in) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 p = pos[i];
   const float c0 = p.x - P.half;
   const float c1 = p.y - P.half;
   const float c2 = p.z - P.half;
+  const bool pfast = particle_fast(c0, c1, c2, P.CB);
   float tx = 0.f, ty = 0.f, tz = 0.f;
   for (int grid = 0; grid < P.k; ++grid) {
     const float* R = rot + 12 * grid;
-    const int cell = cell_of(c0, c1, c2, R, off + 4 * grid, P);
+    const int cell = CACHE ? (int)cells_in[(long long)grid * n + i]
+                           : cell_index(c0, c1, c2, pfast, frot + 12 * grid, R, off + 4 * grid, P);
     const float4 F = force[(long long)grid * P.cells + cell];
     tx += fmaf(R[0], F.x, fmaf(R[1], F.y, R[2] * F.z));
     ty += fmaf(R[4], F.x, fmaf(R[5], F.y, R[6] * F.z));
@@ -74,12 +80,17 @@ __global__ __launch_bounds__(256) void update_kernel(float4* __restrict__ pos,
 }
 
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
-                                const float* rot, const float* off, const float4* force,
-                                float4* avg_out, int mode, hipStream_t s) {
+                                const float* rot, const float* off, const float* frot,
+                                const float4* force, float4* avg_out, int mode,
+                                const unsigned short* cells_in, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const long long blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(gather_update_kernel, dim3((unsigned)blocks), dim3(256), 0, s, pos, vel, n,
-                     rot, off, force, P, avg_out, mode);
+  if (cells_in)
+    hipLaunchKernelGGL(gather_update_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, pos,
+                       vel, n, rot, off, frot, force, P, avg_out, mode, cells_in);
+  else
+    hipLaunchKernelGGL(gather_update_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, pos,
+                       vel, n, rot, off, frot, force, P, avg_out, mode, cells_in);
   return hipGetLastError();
 }
 

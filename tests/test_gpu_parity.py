@@ -160,11 +160,13 @@ def test_forces_c1_within_tolerance(torch_cuda, compat):
     assert err < FORCE_TOL, err
 
 
-def test_step_matches_phases_and_graph(torch_cuda):
+@pytest.mark.parametrize("k,g", [(16, 16), (4, 32), (2, 64), (24, 8)])
+def test_step_matches_phases_and_graph(torch_cuda, k, g):
     """mog_step == deposit+solve+gather+update, and graph replay == eager."""
     torch = torch_cuda
-    k, g, W = 16, 16, 100.0
+    W = 100.0
     pos, vel = workloads.uniform_cube(65536, W, seed=1)
+    vel[:, :3] = np.random.default_rng(4).normal(0, 50, (pos.shape[0], 3))
     s, rot, off = make_solver(k, g, W)
     p1, v1 = dev(torch, pos), dev(torch, vel)
     s.step(p1, v1)
@@ -180,6 +182,24 @@ def test_step_matches_phases_and_graph(torch_cuda):
         s.run(p4, v4, 1, use_graph=True)
         torch.cuda.synchronize()
         assert torch.equal(p3, p4) and torch.equal(v3, v4)
+
+
+def test_particle_count_changes_between_steps(torch_cuda):
+    """The per-step cell cache grows with n; captured graphs are rebuilt."""
+    torch = torch_cuda
+    k, g, W = 8, 16, 100.0
+    s, rot, off = make_solver(k, g, W)
+    for n in (1000, 50000, 3000, 70000):
+        pos, vel = workloads.uniform_cube(n, W, seed=n)
+        pa, va = dev(torch, pos), dev(torch, vel)
+        pb, vb = dev(torch, pos), dev(torch, vel)
+        s.run(pa, va, 2, use_graph=True)
+        avg = s.forces(pb)
+        s.update(pb, vb, avg)
+        avg = s.forces(pb)
+        s.update(pb, vb, avg)
+        torch.cuda.synchronize()
+        assert torch.equal(pa, pb) and torch.equal(va, vb), n
 
 
 def test_trajectory_c1_10_steps(torch_cuda):
