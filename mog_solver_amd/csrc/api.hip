@@ -24,7 +24,7 @@ struct mog_ctx {
   bool have_transforms;
   float* d_rot;
   float* d_off;
-  float* d_frot;  // fast-path rows F[4m+j] = R[4j+m]/h, F[4m+3] = (W/2 - off_m)/h
+  float* d_frot;  // fast-path rows (cellmath.h make_fast_row)
   long long* d_grids;
   float2* d_S1;
   float2* d_S2;
@@ -33,11 +33,7 @@ struct mog_ctx {
   float* d_gtab;
   float* d_kd;
   DepositTune tune;
-  // uint16 cell of every (grid, particle) pair, written by the deposit of a
-  // step and read by its gather (g <= 32); capacity in particles
-  unsigned short* d_cells;
-  long long cells_cap;
-  bool use_cell_cache;
+
   // graph cache of mog_run
   hipStream_t cap_stream;
   hipGraphExec_t gexec;
@@ -190,7 +186,7 @@ void release(mog_ctx* c) {
   (void)hipFree(c->d_tw);
   (void)hipFree(c->d_gtab);
   (void)hipFree(c->d_kd);
-  (void)hipFree(c->d_cells);
+
 }
 
 int check_particles(mog_ctx* ctx, const void* pos, const void* vel, int64_t n, bool need_vel) {
@@ -228,37 +224,18 @@ DepositTune default_tune(const Params& P) {
   return t;
 }
 
-// Make room for the step's cell cache (g <= 32); outside any graph capture.
-// Returns the cache pointer, or NULL when the step recomputes cells.
-unsigned short* ensure_cells(mog_ctx* ctx, int64_t n) {
-  if (!ctx->use_cell_cache || ctx->P.g > 32 || n <= 0) return nullptr;
-  if (ctx->cells_cap >= n) return ctx->d_cells;
-  if (ctx->d_cells) (void)hipFree(ctx->d_cells);
-  ctx->d_cells = nullptr;
-  ctx->cells_cap = 0;
-  if (hipMalloc((void**)&ctx->d_cells, (size_t)n * ctx->P.k * sizeof(unsigned short)) != hipSuccess) {
-    (void)hipGetLastError();
-    ctx->d_cells = nullptr;
-    return nullptr;
-  }
-  ctx->cells_cap = n;
-  return ctx->d_cells;
-}
-
 hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, hipStream_t s) {
   const Params& P = ctx->P;
-  unsigned short* cells =
-      (ctx->use_cell_cache && ctx->P.g <= 32 && ctx->cells_cap >= n) ? ctx->d_cells : nullptr;
   hipError_t e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
   if (e != hipSuccess) return e;
   e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                     (unsigned long long*)ctx->d_grids, cells, ctx->tune, s);
+                     (unsigned long long*)ctx->d_grids, ctx->tune, s);
   if (e != hipSuccess) return e;
   e = launch_solve(P, ctx->d_grids, ctx->d_S1, ctx->d_S2, ctx->d_force, ctx->d_tw, ctx->d_gtab,
-                   ctx->d_kd, ctx->need_x, s);
+                   ctx->d_kd, ctx->need_x, ctx->d_rot, 1, s);
   if (e != hipSuccess) return e;
   return launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
-                              ctx->d_frot, ctx->d_force, nullptr, 0, cells, s);
+                              ctx->d_frot, ctx->d_force, 1, nullptr, 0, s);
 }
 
 }  // namespace
@@ -277,10 +254,7 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
   c->P = make_params(cfg);
   c->need_x = !(cfg->compat == MOG_COMPAT_REFERENCE && aliases_x_with_z(cfg->grid_size));
   c->tune = default_tune(c->P);
-  {
-    const char* cc = getenv("MOG_CELL_CACHE");
-    c->use_cell_cache = cc && cc[0] == '1';
-  }
+
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) {
     delete c;
@@ -343,12 +317,7 @@ int mog_set_transforms(mog_ctx* ctx, const float* rot_k12, const float* off_k4) 
   if (!finite) return fail(ctx, MOG_EINVAL, "rotations/offsets must be finite");
   float* frot = (float*)malloc(k * 12 * sizeof(float));
   if (!frot) return fail(ctx, MOG_ENOMEM, "host allocation failed");
-  for (size_t i = 0; i < k; ++i)
-    for (int m = 0; m < 3; ++m) {
-      for (int j = 0; j < 3; ++j)
-        frot[12 * i + 4 * m + j] = (float)((double)rot_k12[12 * i + 4 * j + m] / (double)P.h);
-      frot[12 * i + 4 * m + 3] = (float)(((double)P.half - (double)off_k4[4 * i + m]) / (double)P.h);
-    }
+  for (size_t i = 0; i < k; ++i) make_fast_row(rot_k12 + 12 * i, off_k4 + 4 * i, P.half, P.h, frot + 12 * i);
   const double margin = fast_margin(ctx->cfg.world_size, P.g, rmax, omax);
   hipError_t e = hipMemcpy(ctx->d_rot, rot_k12, k * 12 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ctx->d_off, off_k4, k * 4 * sizeof(float), hipMemcpyHostToDevice);
@@ -370,7 +339,6 @@ int mog_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, void* stream) {
   if (rc != MOG_OK) return rc;
   if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
   if (n == 0) return MOG_OK;
-  ensure_cells(ctx, n);
   hipError_t e = enqueue_step(ctx, pos, vel, n, (hipStream_t)stream);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_step");
 }
@@ -384,12 +352,6 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
   if (n == 0 || nsteps == 0) return MOG_OK;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
-  const unsigned short* cells_before = ctx->d_cells;
-  ensure_cells(ctx, n);
-  if (ctx->gexec && ctx->d_cells != cells_before) {  // captured pointer is stale
-    (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
   if (!use_graph) {
     for (int i = 0; i < nsteps; ++i)
       if ((e = enqueue_step(ctx, pos, vel, n, s)) != hipSuccess) return hip_fail(ctx, e, "mog_run");
@@ -436,9 +398,6 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
   if (!phase_ms) return fail(ctx, MOG_EINVAL, "phase_ms is NULL");
   hipStream_t s = (hipStream_t)stream;
   const Params& P = ctx->P;
-  if (n > 0) ensure_cells(ctx, n);
-  unsigned short* cells =
-      (ctx->use_cell_cache && P.g <= 32 && ctx->cells_cap >= n) ? ctx->d_cells : nullptr;
   hipEvent_t ev[5] = {};
   hipError_t e = hipSuccess;
   for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
@@ -447,15 +406,15 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
   if (e == hipSuccess) e = hipEventRecord(ev[1], s);
   if (e == hipSuccess)
     e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                       (unsigned long long*)ctx->d_grids, cells, ctx->tune, s);
+                       (unsigned long long*)ctx->d_grids, ctx->tune, s);
   if (e == hipSuccess) e = hipEventRecord(ev[2], s);
   if (e == hipSuccess)
     e = launch_solve(P, ctx->d_grids, ctx->d_S1, ctx->d_S2, ctx->d_force, ctx->d_tw, ctx->d_gtab,
-                     ctx->d_kd, ctx->need_x, s);
+                     ctx->d_kd, ctx->need_x, ctx->d_rot, 1, s);
   if (e == hipSuccess) e = hipEventRecord(ev[3], s);
   if (e == hipSuccess)
     e = launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
-                             ctx->d_frot, ctx->d_force, nullptr, 0, cells, s);
+                             ctx->d_frot, ctx->d_force, 1, nullptr, 0, s);
   if (e == hipSuccess) e = hipEventRecord(ev[4], s);
   if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
   for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventElapsedTime(&phase_ms[i], ev[i], ev[i + 1]);
@@ -475,7 +434,7 @@ int mog_deposit(mog_ctx* ctx, const float* pos, int64_t n, int64_t* grids_out, v
   hipError_t e = launch_fill_zero((long long*)grids_out, cells, s);
   if (e == hipSuccess)
     e = launch_deposit(ctx->P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                       (unsigned long long*)grids_out, nullptr, ctx->tune, s);
+                       (unsigned long long*)grids_out, ctx->tune, s);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_deposit");
 }
 
@@ -486,7 +445,7 @@ int mog_solve(mog_ctx* ctx, const int64_t* grids, float* force_out, void* stream
   if (!aligned16(force_out)) return fail(ctx, MOG_EINVAL, "force_out must be 16-byte aligned");
   hipError_t e = launch_solve(ctx->P, (const long long*)grids, ctx->d_S1, ctx->d_S2,
                               (float4*)force_out, ctx->d_tw, ctx->d_gtab, ctx->d_kd, ctx->need_x,
-                              (hipStream_t)stream);
+                              ctx->d_rot, 0, (hipStream_t)stream);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_solve");
 }
 
@@ -500,8 +459,8 @@ int mog_gather(mog_ctx* ctx, const float* pos, int64_t n, const float* force, fl
   if (!aligned16(force) || !aligned16(avg_out))
     return fail(ctx, MOG_EINVAL, "force/avg_out must be 16-byte aligned");
   hipError_t e = launch_gather_update(ctx->P, (float4*)pos, nullptr, n, ctx->d_rot, ctx->d_off,
-                                      ctx->d_frot, (const float4*)force, (float4*)avg_out, 1,
-                                      nullptr, (hipStream_t)stream);
+                                      ctx->d_frot, (const float4*)force, 0, (float4*)avg_out, 1,
+                                      (hipStream_t)stream);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_gather");
 }
 

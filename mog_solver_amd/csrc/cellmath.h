@@ -7,9 +7,9 @@
 // with every operation rounded separately (the library is built with
 // -ffp-contract=off) and IEEE division.
 //
-// Fast path: v_m = l_m / h evaluated as three FMAs with per-grid coefficients
-// F[4m+j] = RN(R[4j+m] / h), F[4m+3] = RN((W/2 - off_m) / h); the cell is
-// floor(v) mod g.  Both paths differ from the real-number value v* by at
+// Fast path: w_m = l_m / h - 1/2 evaluated as three FMAs with per-grid
+// coefficients F[4m+j] = RN(R[4j+m] / h), F[4m+3] = RN((W/2 - off_m)/h - 1/2);
+// the cell is floor(v) mod g = rint(w) mod g.  Both paths differ from the real-number value v* by at
 // most a bound E (derived in fast_margin below); when v lies farther than
 // M >= E_exact + E_fast from every integer, floor(v) mod g equals the exact
 // path's cell, so the fast result is used; otherwise the exact sequence runs.
@@ -66,23 +66,39 @@ MOG_HD int cell_exact(float c0, float c1, float c2, const float* R, const float*
   return (((z << lg) | y) << lg) | x;
 }
 
-// Fast cell; returns false when the exact path must decide.
-// F is the 12-float fast row of the grid, lim = 0.5 - M.
-MOG_HD bool cell_fast(float c0, float c1, float c2, const float* F, float lim, int lg, int* cell) {
-  const float v0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, F[3])));
-  const float v1 = fmaf(F[6], c2, fmaf(F[5], c1, fmaf(F[4], c0, F[7])));
-  const float v2 = fmaf(F[10], c2, fmaf(F[9], c1, fmaf(F[8], c0, F[11])));
-  const float f0 = floorf(v0), f1 = floorf(v1), f2 = floorf(v2);
-  const bool ok = (fabsf((v0 - f0) - 0.5f) < lim) && (fabsf((v1 - f1) - 0.5f) < lim) &&
-                  (fabsf((v2 - f2) - 0.5f) < lim);
+// Fast cell; returns 0 when the exact path must decide.  F is the 12-float
+// fast row of the grid (make_fast_row), lim = 0.5 - M.  With w = v - 0.5
+// (the -0.5 is folded into F[4m+3]) and r = rint(w): floor(v) = r and the
+// distance of v to the nearest integer is 0.5 - |w - r|.  Flags combine with
+// bitwise & so that no branch is taken per axis.
+MOG_HD int cell_fast(float c0, float c1, float c2, const float* F, float lim, int lg, int* cell) {
+  const float w0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, F[3])));
+  const float w1 = fmaf(F[6], c2, fmaf(F[5], c1, fmaf(F[4], c0, F[7])));
+  const float w2 = fmaf(F[10], c2, fmaf(F[9], c1, fmaf(F[8], c0, F[11])));
+  const float r0 = rintf(w0), r1 = rintf(w1), r2 = rintf(w2);
+  const int ok = (int)(fabsf(w0 - r0) < lim) & (int)(fabsf(w1 - r1) < lim) &
+                 (int)(fabsf(w2 - r2) < lim);
   const int gm = (1 << lg) - 1;
-  *cell = (((((int)f2 & gm) << lg) | ((int)f1 & gm)) << lg) | ((int)f0 & gm);
+  int c = (int)r2;
+  c = (c << lg) | ((int)r1 & gm);
+  c = (c << lg) | ((int)r0 & gm);
+  *cell = c & ((1 << (3 * lg)) - 1);
   return ok;
 }
 
+// The 12-float fast row of one grid: F[4m+j] = RN(R[4j+m] / h),
+// F[4m+3] = RN((W/2 - off_m) / h - 1/2), from the float32 values the
+// kernels use (half = float(W) * 0.5f, h = float(W / g)).
+inline void make_fast_row(const float* R, const float* O, float half, float h, float* F) {
+  for (int m = 0; m < 3; ++m) {
+    for (int j = 0; j < 3; ++j) F[4 * m + j] = (float)((double)R[4 * j + m] / (double)h);
+    F[4 * m + 3] = (float)(((double)half - (double)O[m]) / (double)h - 0.5);
+  }
+}
+
 // Per-particle admission to the fast path: every |c_j| <= CB (NaN fails).
-MOG_HD bool particle_fast(float c0, float c1, float c2, float CB) {
-  return (fabsf(c0) <= CB) && (fabsf(c1) <= CB) && (fabsf(c2) <= CB);
+MOG_HD int particle_fast(float c0, float c1, float c2, float CB) {
+  return (int)(fabsf(c0) <= CB) & (int)(fabsf(c1) <= CB) & (int)(fabsf(c2) <= CB);
 }
 
 // Error margin M of the fast path (in cell units) for transforms whose
@@ -91,8 +107,8 @@ MOG_HD bool particle_fast(float c0, float c1, float c2, float CB) {
 //   exact path, l: products, partial sums, +W/2, -off     u (14 T + W + omax)
 //   exact path, mod chain (|floor| <= L/W, L = 3T + W/2 + omax)  u (L + 4 W)
 //   exact path, l / h                                   u g      (cell units)
-//   fast path, coefficient rounding                     u (3 T + W/2 + omax)
-//   fast path, three FMAs                               u (6 T + 3 (W/2 + omax))
+//   fast path, coefficient rounding                     u (3 T + W/2 + omax + h/2)
+//   fast path, three FMAs                               u (6 T + 3 (W/2 + omax + h/2))
 // M = 3 x (sum / h + u g); returns a value >= 0.05 when the fast path should
 // not be used.
 inline double fast_margin(double W, int g, double rmax, double omax) {
@@ -103,7 +119,7 @@ inline double fast_margin(double W, int g, double rmax, double omax) {
   const double L = 3.0 * T + 0.5 * W + omax;
   const double exact_l = 14.0 * T + W + omax;
   const double exact_mod = L + 4.0 * W;
-  const double fast = 3.0 * T + 0.5 * W + omax + 6.0 * T + 3.0 * (0.5 * W + omax);
+  const double fast = 3.0 * T + 0.5 * W + omax + 0.5 * h + 6.0 * T + 3.0 * (0.5 * W + omax + 0.5 * h);
   return 3.0 * (u * (exact_l + exact_mod + fast) / h + u * g);
 }
 

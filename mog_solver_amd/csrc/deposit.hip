@@ -1,8 +1,8 @@
 // NGP mass deposit of the MOG step (replaces shaders/mass.glsl:27-55 and the
 // host-side clear of main.py:558-562).
 //
-// Work is tiled as (particle chunk) x (group of grids).  A workgroup keeps a
-// private 32-bit histogram of each grid of its group in LDS, streams its
+// Work is tiled as (particle chunk) x (group of G grids).  A workgroup keeps
+// a private 32-bit histogram of each of its G grids in LDS, streams its
 // particle chunk once, and for every (particle, grid) pair computes the
 // reference's cell (cellmath.h: three FMAs and a face-distance test, with the
 // exact float32 sequence as fallback near faces) and adds the particle's
@@ -32,50 +32,69 @@ hipError_t launch_fill_zero(long long* p, long long count, hipStream_t s) {
   return hipGetLastError();
 }
 
-// LDS-privatised deposit.  grid (chunks, grid_groups); dynamic LDS holds
-// ng * cells uint32 counters.  With CACHE the cell of every (grid, particle)
-// pair is also stored as uint16 in cells_out[grid * n + p] for the gather of
-// the same step (same positions, so the same cells).
-template <bool CACHE>
+// LDS-privatised deposit; grid (chunks, ceil(k / G)).  Dynamic LDS holds
+// G * cells uint32 counters.  The G fast rows are wave-uniform and live in
+// scalar registers for the whole chunk.
+template <int G>
 __global__ __launch_bounds__(1024) void deposit_lds_kernel(
-    const float4* __restrict__ pos, long long n, long long chunk, int G,
+    const float4* __restrict__ pos, long long n, long long chunk,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
-    Params P, unsigned long long lds_limit, unsigned long long* __restrict__ grids,
-    unsigned short* __restrict__ cells_out) {
+    Params P, unsigned long long lds_limit, unsigned long long* __restrict__ grids) {
   extern __shared__ unsigned int hist[];
   const int cells = P.cells;
   const int grid0 = blockIdx.y * G;
   const int ng = min(G, P.k - grid0);
   const int total = ng * cells;
   for (int i = threadIdx.x; i < total; i += blockDim.x) hist[i] = 0u;
+
+  float F[G][12];
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) {
+    const int gr = min(grid0 + gi, P.k - 1);
+#pragma unroll
+    for (int j = 0; j < 12; ++j) F[gi][j] = frot[12 * gr + j];
+  }
   __syncthreads();
 
   const long long p0 = (long long)blockIdx.x * chunk;
   const long long p1 = min(n, p0 + chunk);
-  for (long long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const float4 q = pos[p];
-    const float c0 = q.x - P.half;
-    const float c1 = q.y - P.half;
-    const float c2 = q.z - P.half;
-    const unsigned long long milli = milli_of(q.w);
-    if (milli == 0ull && !CACHE) continue;
-    const bool pfast = particle_fast(c0, c1, c2, P.CB);
-    if (milli <= lds_limit) {
-      const unsigned int mv = (unsigned int)milli;
-      for (int gi = 0; gi < ng; ++gi) {
-        const int grid = grid0 + gi;
-        const int cell =
-            cell_index(c0, c1, c2, pfast, frot + 12 * grid, rot + 12 * grid, off + 4 * grid, P);
-        atomicAdd(&hist[gi * cells + cell], mv);
-        if (CACHE) cells_out[(long long)grid * n + p] = (unsigned short)cell;
-      }
-    } else {
-      for (int gi = 0; gi < ng; ++gi) {
-        const int grid = grid0 + gi;
-        const int cell =
-            cell_index(c0, c1, c2, pfast, frot + 12 * grid, rot + 12 * grid, off + 4 * grid, P);
-        atomicAdd(&grids[(long long)grid * cells + cell], milli);
-        if (CACHE) cells_out[(long long)grid * n + p] = (unsigned short)cell;
+  // UP particles per thread per iteration: their loads are issued together.
+  constexpr int UP = 2;
+  const long long step = (long long)UP * blockDim.x;
+  for (long long pb = p0 + threadIdx.x; pb < p1; pb += step) {
+    float4 q[UP];
+#pragma unroll
+    for (int u = 0; u < UP; ++u) {
+      const long long p = pb + (long long)u * blockDim.x;
+      q[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);  // mass 0 is skipped
+    }
+#pragma unroll
+    for (int u = 0; u < UP; ++u) {
+      const unsigned long long milli = milli_of(q[u].w);
+      if (milli == 0ull) continue;
+      const float c0 = q[u].x - P.half;
+      const float c1 = q[u].y - P.half;
+      const float c2 = q[u].z - P.half;
+      const int pfast = particle_fast(c0, c1, c2, P.CB);
+      if (milli <= lds_limit) {
+        const unsigned int mv = (unsigned int)milli;
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi) {
+          if (gi < ng) {
+            const int gr = grid0 + gi;
+            const int cell = cell_index(c0, c1, c2, pfast, F[gi], rot + 12 * gr, off + 4 * gr, P);
+            atomicAdd(&hist[gi * cells + cell], mv);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi) {
+          if (gi < ng) {
+            const int gr = grid0 + gi;
+            const int cell = cell_index(c0, c1, c2, pfast, F[gi], rot + 12 * gr, off + 4 * gr, P);
+            atomicAdd(&grids[(long long)gr * cells + cell], milli);
+          }
+        }
       }
     }
   }
@@ -87,7 +106,7 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
   }
 }
 
-// Direct global deposit for grids too large to privatise in LDS.
+// Direct global deposit for grids too large to privatise in LDS (g = 64).
 __global__ __launch_bounds__(256) void deposit_global_kernel(
     const float4* __restrict__ pos, long long n, int G, const float* __restrict__ rot,
     const float* __restrict__ off, const float* __restrict__ frot, Params P,
@@ -97,25 +116,39 @@ __global__ __launch_bounds__(256) void deposit_global_kernel(
   const int grid0 = blockIdx.y * G;
   const int ng = min(G, P.k - grid0);
   const float4 q = pos[p];
+  const unsigned long long milli = milli_of(q.w);
+  if (milli == 0ull) return;
   const float c0 = q.x - P.half;
   const float c1 = q.y - P.half;
   const float c2 = q.z - P.half;
-  const unsigned long long milli = milli_of(q.w);
-  if (milli == 0ull) return;
-  const bool pfast = particle_fast(c0, c1, c2, P.CB);
+  const int pfast = particle_fast(c0, c1, c2, P.CB);
   for (int gi = 0; gi < ng; ++gi) {
-    const int grid = grid0 + gi;
-    const int cell =
-        cell_index(c0, c1, c2, pfast, frot + 12 * grid, rot + 12 * grid, off + 4 * grid, P);
-    atomicAdd(&grids[(long long)grid * P.cells + cell], milli);
+    const int gr = grid0 + gi;
+    const int cell = cell_index(c0, c1, c2, pfast, frot + 12 * gr, rot + 12 * gr, off + 4 * gr, P);
+    atomicAdd(&grids[(long long)gr * P.cells + cell], milli);
   }
 }
 
 static const size_t kMaxLds = 160 * 1024;
 
+template <int G>
+static hipError_t launch_lds(const Params& P, const float4* pos, long long n, long long chunks,
+                             long long chunk, int threads, const float* rot, const float* off,
+                             const float* frot, unsigned long long* grids, hipStream_t s) {
+  const int groups = (P.k + G - 1) / G;
+  const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)chunk;
+  const size_t lds = (size_t)G * P.cells * 4;
+  hipError_t e = hipFuncSetAttribute((const void*)deposit_lds_kernel<G>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(deposit_lds_kernel<G>, dim3((unsigned)chunks, (unsigned)groups),
+                     dim3(threads), lds, s, pos, n, chunk, rot, off, frot, P, lds_limit, grids);
+  return hipGetLastError();
+}
+
 hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const float* rot,
                           const float* off, const float* frot, unsigned long long* grids,
-                          unsigned short* cells_out, const DepositTune& t, hipStream_t s) {
+                          const DepositTune& t, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const size_t grid_bytes = (size_t)P.cells * 4;
   if (grid_bytes > kMaxLds) {
@@ -126,32 +159,24 @@ hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const
                        0, s, pos, n, G, rot, off, frot, P, grids);
     return hipGetLastError();
   }
-  int G = t.grids_per_wg;
-  const int gmax = (int)(kMaxLds / grid_bytes);
-  if (G > gmax) G = gmax;
-  if (G > P.k) G = P.k;
-  if (G < 1) G = 1;
+  // G in {1, 2, 4, 8}, within the LDS and at most k
+  int G = 1;
+  while (G * 2 <= t.grids_per_wg && G * 2 <= 8 && (size_t)(G * 2) * grid_bytes <= kMaxLds &&
+         G * 2 <= P.k)
+    G *= 2;
   const int groups = (P.k + G - 1) / G;
-  int threads = t.threads;
+  const int threads = t.threads;
   long long chunks = (t.wg_target + groups - 1) / groups;
   if (chunks < 1) chunks = 1;
   long long chunk = (n + chunks - 1) / chunks;
   if (chunk < threads) chunk = threads;
   chunks = (n + chunk - 1) / chunk;
-  const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)chunk;
-  const size_t lds = (size_t)G * grid_bytes;
-  const void* fn = cells_out ? (const void*)deposit_lds_kernel<true> : (const void*)deposit_lds_kernel<false>;
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
-  if (e != hipSuccess) return e;
-  if (cells_out)
-    hipLaunchKernelGGL(deposit_lds_kernel<true>, dim3((unsigned)chunks, (unsigned)groups),
-                       dim3(threads), lds, s, pos, n, chunk, G, rot, off, frot, P, lds_limit, grids,
-                       cells_out);
-  else
-    hipLaunchKernelGGL(deposit_lds_kernel<false>, dim3((unsigned)chunks, (unsigned)groups),
-                       dim3(threads), lds, s, pos, n, chunk, G, rot, off, frot, P, lds_limit, grids,
-                       cells_out);
-  return hipGetLastError();
+  switch (G) {
+    case 8: return launch_lds<8>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, s);
+    case 4: return launch_lds<4>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, s);
+    case 2: return launch_lds<2>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, s);
+    default: return launch_lds<1>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, s);
+  }
 }
 
 }  // namespace mog

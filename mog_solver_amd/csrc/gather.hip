@@ -1,12 +1,12 @@
 // Force gather + integrator of the MOG step (replaces shaders/update.glsl).
 //
 // One thread per particle reads its position and velocity once, loops over
-// the k grids (the packed rotation rows and offsets are wave-uniform, so they
-// come through the scalar cache), recomputes the NGP cell exactly as the
-// deposit does (same cell for the same position), fetches the float4 force of that cell,
-// rotates it back to the world frame (world_m = sum_j R[4m+j] F_j, i.e.
-// transpose(rot) * F of update.glsl:77), averages over k and applies
-// kick, damping, drift and the periodic wrap (update.glsl:81-89).
+// the k grids (the per-grid fast rows are wave-uniform, so they come through
+// the scalar cache), recomputes the NGP cell exactly as the deposit does
+// (same cell for the same position), fetches the force of that cell in the
+// world frame (M F: transpose(rot) * gridForce of update.glsl:77), averages
+// over k and applies kick, damping, drift and the periodic wrap
+// (update.glsl:81-89).
 #include "mog_common.h"
 
 namespace mog {
@@ -28,31 +28,64 @@ __device__ __forceinline__ void integrate(float4& p, float4& v, float ax, float 
   v = make_float4(vx, vy, vz, 0.0f);
 }
 
+// t += M F for a grid-frame force F of grid R: world_m = sum_j R[4m+j] F_j
+// (transpose(rot) * gridForce, update.glsl:77).
+__device__ __forceinline__ void accumulate_rotated(const float* __restrict__ R, const float4 F,
+                                                   float& tx, float& ty, float& tz) {
+  tx += fmaf(R[0], F.x, fmaf(R[1], F.y, R[2] * F.z));
+  ty += fmaf(R[4], F.x, fmaf(R[5], F.y, R[6] * F.z));
+  tz += fmaf(R[8], F.x, fmaf(R[9], F.y, R[10] * F.z));
+}
+
 // mode 0: update pos/vel in place; mode 1: write avgForce to avg_out.
-// With CACHE the cells come from cells_in[grid * n + i], stored by the
-// deposit of the same step for the same positions.
-template <bool CACHE>
+// WORLD: the force grids already hold M F (the step's solve rotates each
+// cell once), so a pair costs three adds instead of nine FMAs.  The grid
+// loop is unrolled by U so that U force fetches are in flight per wave.
+template <bool WORLD>
 __global__ __launch_bounds__(256) void gather_update_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, long long n,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
-    const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode,
-    const unsigned short* __restrict__ cells_in) {
+    const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode) {
+  constexpr int U = 4;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 p = pos[i];
   const float c0 = p.x - P.half;
   const float c1 = p.y - P.half;
   const float c2 = p.z - P.half;
-  const bool pfast = particle_fast(c0, c1, c2, P.CB);
+  const int pfast = particle_fast(c0, c1, c2, P.CB);
   float tx = 0.f, ty = 0.f, tz = 0.f;
-  for (int grid = 0; grid < P.k; ++grid) {
-    const float* R = rot + 12 * grid;
-    const int cell = CACHE ? (int)cells_in[(long long)grid * n + i]
-                           : cell_index(c0, c1, c2, pfast, frot + 12 * grid, R, off + 4 * grid, P);
+  int grid = 0;
+  for (; grid + U <= P.k; grid += U) {
+    float4 F[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int gr = grid + u;
+      const int cell = cell_index(c0, c1, c2, pfast, frot + 12 * gr, rot + 12 * gr, off + 4 * gr, P);
+      F[u] = force[(long long)gr * P.cells + cell];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (WORLD) {
+        tx += F[u].x;
+        ty += F[u].y;
+        tz += F[u].z;
+      } else {
+        accumulate_rotated(rot + 12 * (grid + u), F[u], tx, ty, tz);
+      }
+    }
+  }
+  for (; grid < P.k; ++grid) {
+    const int cell =
+        cell_index(c0, c1, c2, pfast, frot + 12 * grid, rot + 12 * grid, off + 4 * grid, P);
     const float4 F = force[(long long)grid * P.cells + cell];
-    tx += fmaf(R[0], F.x, fmaf(R[1], F.y, R[2] * F.z));
-    ty += fmaf(R[4], F.x, fmaf(R[5], F.y, R[6] * F.z));
-    tz += fmaf(R[8], F.x, fmaf(R[9], F.y, R[10] * F.z));
+    if (WORLD) {
+      tx += F.x;
+      ty += F.y;
+      tz += F.z;
+    } else {
+      accumulate_rotated(rot + 12 * grid, F, tx, ty, tz);
+    }
   }
   const float ax = tx / P.fk, ay = ty / P.fk, az = tz / P.fk;
   if (mode == 1) {
@@ -81,16 +114,16 @@ __global__ __launch_bounds__(256) void update_kernel(float4* __restrict__ pos,
 
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
                                 const float* rot, const float* off, const float* frot,
-                                const float4* force, float4* avg_out, int mode,
-                                const unsigned short* cells_in, hipStream_t s) {
+                                const float4* force, int world, float4* avg_out, int mode,
+                                hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const long long blocks = (n + 255) / 256;
-  if (cells_in)
+  if (world)
     hipLaunchKernelGGL(gather_update_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, pos,
-                       vel, n, rot, off, frot, force, P, avg_out, mode, cells_in);
+                       vel, n, rot, off, frot, force, P, avg_out, mode);
   else
     hipLaunchKernelGGL(gather_update_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, pos,
-                       vel, n, rot, off, frot, force, P, avg_out, mode, cells_in);
+                       vel, n, rot, off, frot, force, P, avg_out, mode);
   return hipGetLastError();
 }
 

@@ -28,15 +28,27 @@ struct Params {
   float lim;   // fast-path acceptance: |frac(v) - 0.5| < lim (0.5 - margin); < 0 disables
 };
 
+// The exact float32 sequence, kept out of line (one copy per translation
+// unit): it runs for a few percent of (particle, grid) pairs only.
+static __device__ __attribute__((noinline)) int cell_exact_call(float c0, float c1, float c2,
+                                                                const float* __restrict__ R,
+                                                                const float* __restrict__ O,
+                                                                float W, float half, float h,
+                                                                int lg) {
+  return cell_exact(c0, c1, c2, R, O, W, half, h, lg);
+}
+
 // Cell of the centred position (c0,c1,c2) in one grid: the fast path when
 // the particle is admitted (pfast) and v is clear of every cell face,
 // otherwise the exact float32 sequence of the reference (cellmath.h).
-__device__ __forceinline__ int cell_index(float c0, float c1, float c2, bool pfast,
-                                          const float* __restrict__ F, const float* __restrict__ R,
+// F points at the grid's fast row (registers or scalar memory), R and O at
+// its packed rotation row and offset in global memory.
+__device__ __forceinline__ int cell_index(float c0, float c1, float c2, int pfast, const float* F,
+                                          const float* __restrict__ R,
                                           const float* __restrict__ O, const Params& P) {
   int cell;
-  const bool ok = cell_fast(c0, c1, c2, F, P.lim, P.lg, &cell) && pfast;
-  if (!ok) cell = cell_exact(c0, c1, c2, R, O, P.W, P.half, P.h, P.lg);
+  const int ok = cell_fast(c0, c1, c2, F, P.lim, P.lg, &cell) & pfast;
+  if (!ok) cell = cell_exact_call(c0, c1, c2, R, O, P.W, P.half, P.h, P.lg);
   return cell;
 }
 
@@ -61,14 +73,16 @@ struct DepositTune {
 hipError_t launch_fill_zero(long long* p, long long count, hipStream_t s);
 hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const float* rot,
                           const float* off, const float* frot, unsigned long long* grids,
-                          unsigned short* cells_out, const DepositTune& t, hipStream_t s);
+                          const DepositTune& t, hipStream_t s);
+// world != 0: force grids hold M F (world frame) for the step's gather;
+// world == 0: grid-frame forces as pack.glsl writes them.
 hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
                         float4* force, const float2* tw, const float* gtab, const float* kd,
-                        int need_x, hipStream_t s);
+                        int need_x, const float* rot, int world, hipStream_t s);
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
                                 const float* rot, const float* off, const float* frot,
-                                const float4* force, float4* avg_out, int mode,
-                                const unsigned short* cells_in, hipStream_t s);
+                                const float4* force, int world, float4* avg_out, int mode,
+                                hipStream_t s);
 hipError_t launch_update(const Params& P, float4* pos, float4* vel, const float4* avg,
                          long long n, hipStream_t s);
 

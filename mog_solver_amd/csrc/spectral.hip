@@ -135,11 +135,15 @@ __global__ __launch_bounds__(256) void spec_z_green_kernel(float2* __restrict__ 
 }
 
 // Pass C: one workgroup per (grid, z): inverse y, inverse x, pack float4.
+// world != 0 stores M F (the grid's force rotated to the world frame,
+// world_m = sum_j R[4m+j] F_j) for the step's gather; |F| is unchanged.
 __global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restrict__ S,
                                                           const float2* __restrict__ S2,
                                                           float4* __restrict__ force,
                                                           const float2* __restrict__ tw, int g,
-                                                          int lg, int need_x) {
+                                                          int lg, int need_x,
+                                                          const float* __restrict__ rot,
+                                                          int world) {
   extern __shared__ float2 sm[];
   float2* twl = sm;
   float2* Z = sm + g;       // [ky][kx]
@@ -187,13 +191,21 @@ __global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restri
     }
     if (!need_x) fx = fz;
     const float mag = sqrtf(fmaf(fx, fx, fmaf(fy, fy, fz * fz)));
-    force[base + o] = make_float4(fx, fy, fz, mag);
+    if (world) {
+      const float* R = rot + 12 * (blockIdx.x >> lg);
+      const float wx = fmaf(R[0], fx, fmaf(R[1], fy, R[2] * fz));
+      const float wy = fmaf(R[4], fx, fmaf(R[5], fy, R[6] * fz));
+      const float wz = fmaf(R[8], fx, fmaf(R[9], fy, R[10] * fz));
+      force[base + o] = make_float4(wx, wy, wz, mag);
+    } else {
+      force[base + o] = make_float4(fx, fy, fz, mag);
+    }
   }
 }
 
 hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
                         float4* force, const float2* tw, const float* gtab, const float* kd,
-                        int need_x, hipStream_t s) {
+                        int need_x, const float* rot, int world, hipStream_t s) {
   const int g = P.g, lg = P.lg, gg = g * g;
   const unsigned slabs = (unsigned)(P.k * g);
   const size_t ldsA = (size_t)(g + (gg + 1) / 2 + gg) * sizeof(float2);
@@ -215,7 +227,7 @@ hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, flo
                      lg, need_x);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(spec_inv_xy_kernel, dim3(slabs), dim3(256), ldsC, s, S1, S2, force, tw, g,
-                     lg, need_x);
+                     lg, need_x, rot, world);
   return hipGetLastError();
 }
 

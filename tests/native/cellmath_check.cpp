@@ -49,10 +49,7 @@ static void make_grid(Grid& G, double W, int g) {
     G.O[j] = (float)(uni() * h);
   }
   G.O[3] = 0;
-  for (int m = 0; m < 3; ++m) {
-    for (int j = 0; j < 3; ++j) G.F[4 * m + j] = (float)((double)G.R[4 * j + m] / (double)h);
-    G.F[4 * m + 3] = (float)(((double)half - (double)G.O[m]) / (double)h);
-  }
+  mog::make_fast_row(G.R, G.O, half, h, G.F);
 }
 
 int main(int argc, char** argv) {
@@ -97,7 +94,7 @@ int main(int argc, char** argv) {
           const float c0 = p[0] - half, c1 = p[1] - half, c2 = p[2] - half;
           const int ce = mog::cell_exact(c0, c1, c2, G.R, G.O, Wf, half, h, lg);
           int cf = -1;
-          const bool ok = mog::particle_fast(c0, c1, c2, CB) && mog::cell_fast(c0, c1, c2, G.F, lim, lg, &cf);
+          const int ok = mog::cell_fast(c0, c1, c2, G.F, lim, lg, &cf) & mog::particle_fast(c0, c1, c2, CB);
           ++total;
           if (ok) {
             ++fast;
