@@ -32,6 +32,8 @@ struct mog_ctx {
   float2* d_tw;
   float* d_gtab;
   float* d_kd;
+  float* d_Wf;  // MFMA DFT tables (g >= 16): forward, inverse
+  float* d_Wi;
   DepositTune tune;
 
   // graph cache of mog_run
@@ -98,9 +100,10 @@ int validate(const mog_config* c) {
     return fail(nullptr, MOG_EINVAL, "compat must be 0 (reference) or 1 (physical)");
   if (c->precision < MOG_PRECISION_FP32 || c->precision > MOG_PRECISION_FP16)
     return fail(nullptr, MOG_EINVAL, "precision must be 0 (fp32), 1 (bf16x3) or 2 (fp16)");
-  if (c->precision != MOG_PRECISION_FP32)
-    return fail(nullptr, MOG_EUNSUPPORTED, "precision %d is not built in this library",
-                c->precision);
+  if (c->precision != MOG_PRECISION_FP32 && c->grid_size < 16)
+    return fail(nullptr, MOG_EUNSUPPORTED,
+                "precision %d (matrix-core DFT) needs grid_size >= 16, got %d", c->precision,
+                c->grid_size);
   if (c->solver != MOG_SOLVER_FFT && c->solver != MOG_SOLVER_CONVOLUTION)
     return fail(nullptr, MOG_EINVAL, "solver must be 0 (fft) or 1 (convolution)");
   const double cells = (double)c->n_grids * c->grid_size * c->grid_size * c->grid_size;
@@ -123,6 +126,29 @@ bool green_mask32(int ix, int iy, int iz, int g, double W) {
   volatile float k2 = a + b;
   k2 = k2 + c;
   return k2 > 1e-4f;
+}
+
+// DFT matrices for the matrix-core transforms, as the A-operand tables
+// [Re W | -Im W] and [Im W | Re W] (g rows of 2g floats each), forward
+// W = exp(-2 pi i jk/g) into Wf, inverse W = exp(+2 pi i jk/g) into Wi.
+void build_dft_tables(int g, float* Wf, float* Wi) {
+  const int g2 = 2 * g;
+  for (int j = 0; j < g; ++j)
+    for (int k = 0; k < g; ++k) {
+      const int jk = (j * k) % g;
+      const double th = 2.0 * M_PI * (double)jk / (double)g;
+      const float cs = (float)cos(th), sn = (float)sin(th);
+      // forward: Re W = cos, Im W = -sin
+      Wf[j * g2 + k] = cs;
+      Wf[j * g2 + g + k] = sn;
+      Wf[2 * g * g + j * g2 + k] = -sn;
+      Wf[2 * g * g + j * g2 + g + k] = cs;
+      // inverse: Re W = cos, Im W = sin
+      Wi[j * g2 + k] = cs;
+      Wi[j * g2 + g + k] = -sn;
+      Wi[2 * g * g + j * g2 + k] = sn;
+      Wi[2 * g * g + j * g2 + g + k] = cs;
+    }
 }
 
 void build_tables(const mog_config* c, float2* tw, float* gtab, float* kd) {
@@ -186,6 +212,8 @@ void release(mog_ctx* c) {
   (void)hipFree(c->d_tw);
   (void)hipFree(c->d_gtab);
   (void)hipFree(c->d_kd);
+  (void)hipFree(c->d_Wf);
+  (void)hipFree(c->d_Wi);
 
 }
 
@@ -224,6 +252,17 @@ DepositTune default_tune(const Params& P) {
   return t;
 }
 
+// Spectral solve: matrix-core passes for g >= 16, vector passes below.
+hipError_t solve_dispatch(mog_ctx* ctx, const long long* grids, float4* force, int world,
+                          hipStream_t s) {
+  const Params& P = ctx->P;
+  if (P.g >= 16)
+    return launch_solve_mfma(P, ctx->cfg.precision, grids, ctx->d_S1, ctx->d_S2, force, ctx->d_Wf,
+                             ctx->d_Wi, ctx->d_gtab, ctx->d_kd, ctx->need_x, ctx->d_rot, world, s);
+  return launch_solve(P, grids, ctx->d_S1, ctx->d_S2, force, ctx->d_tw, ctx->d_gtab, ctx->d_kd,
+                      ctx->need_x, ctx->d_rot, world, s);
+}
+
 hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, hipStream_t s) {
   const Params& P = ctx->P;
   hipError_t e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
@@ -231,8 +270,7 @@ hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, hipStre
   e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
                      (unsigned long long*)ctx->d_grids, ctx->tune, s);
   if (e != hipSuccess) return e;
-  e = launch_solve(P, ctx->d_grids, ctx->d_S1, ctx->d_S2, ctx->d_force, ctx->d_tw, ctx->d_gtab,
-                   ctx->d_kd, ctx->need_x, ctx->d_rot, 1, s);
+  e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
   if (e != hipSuccess) return e;
   return launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
                               ctx->d_frot, ctx->d_force, 1, nullptr, 0, s);
@@ -266,7 +304,8 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
       dalloc(&c->d_grids, cells) != hipSuccess || dalloc(&c->d_S1, cells) != hipSuccess ||
       dalloc(&c->d_S2, c->need_x ? cells : 1) != hipSuccess ||
       dalloc(&c->d_force, cells) != hipSuccess || dalloc(&c->d_tw, g) != hipSuccess ||
-      dalloc(&c->d_gtab, g * g * g) != hipSuccess || dalloc(&c->d_kd, g) != hipSuccess) {
+      dalloc(&c->d_gtab, g * g * g) != hipSuccess || dalloc(&c->d_kd, g) != hipSuccess ||
+      dalloc(&c->d_Wf, 4 * g * g) != hipSuccess || dalloc(&c->d_Wi, 4 * g * g) != hipSuccess) {
     release(c);
     delete c;
     return fail(nullptr, MOG_ENOMEM, "device allocation failed for k=%d g=%d", cfg->n_grids,
@@ -275,11 +314,18 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
   float2* tw = (float2*)malloc(g * sizeof(float2));
   float* gtab = (float*)malloc(g * g * g * sizeof(float));
   float* kd = (float*)malloc(g * sizeof(float));
+  float* Wf = (float*)malloc(4 * g * g * sizeof(float));
+  float* Wi = (float*)malloc(4 * g * g * sizeof(float));
   build_tables(cfg, tw, gtab, kd);
+  build_dft_tables((int)g, Wf, Wi);
   e = hipMemcpy(c->d_tw, tw, g * sizeof(float2), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_gtab, gtab, g * g * g * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_kd, kd, g * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_Wf, Wf, 4 * g * g * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_Wi, Wi, 4 * g * g * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(c->d_grids, 0, cells * sizeof(long long));
+  free(Wf);
+  free(Wi);
   free(tw);
   free(gtab);
   free(kd);
@@ -409,8 +455,7 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
                        (unsigned long long*)ctx->d_grids, ctx->tune, s);
   if (e == hipSuccess) e = hipEventRecord(ev[2], s);
   if (e == hipSuccess)
-    e = launch_solve(P, ctx->d_grids, ctx->d_S1, ctx->d_S2, ctx->d_force, ctx->d_tw, ctx->d_gtab,
-                     ctx->d_kd, ctx->need_x, ctx->d_rot, 1, s);
+    e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
   if (e == hipSuccess) e = hipEventRecord(ev[3], s);
   if (e == hipSuccess)
     e = launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
@@ -443,9 +488,8 @@ int mog_solve(mog_ctx* ctx, const int64_t* grids, float* force_out, void* stream
   if (!grids || !force_out) return fail(ctx, MOG_EINVAL, "grids/force_out is NULL");
   if (((uintptr_t)grids & 7u) != 0) return fail(ctx, MOG_EINVAL, "grids must be 8-byte aligned");
   if (!aligned16(force_out)) return fail(ctx, MOG_EINVAL, "force_out must be 16-byte aligned");
-  hipError_t e = launch_solve(ctx->P, (const long long*)grids, ctx->d_S1, ctx->d_S2,
-                              (float4*)force_out, ctx->d_tw, ctx->d_gtab, ctx->d_kd, ctx->need_x,
-                              ctx->d_rot, 0, (hipStream_t)stream);
+  hipError_t e = solve_dispatch(ctx, (const long long*)grids, (float4*)force_out, 0,
+                                (hipStream_t)stream);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_solve");
 }
 

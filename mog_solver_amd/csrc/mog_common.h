@@ -79,6 +79,13 @@ hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const
 hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
                         float4* force, const float2* tw, const float* gtab, const float* kd,
                         int need_x, const float* rot, int world, hipStream_t s);
+// MFMA spectral solve (g >= 16): precision 0 FP32, 1 BF16X3, 2 FP16.
+// Wf / Wi: forward / inverse DFT tables, each [Re W | -Im W] then
+// [Im W | Re W] as g rows of 2g floats.
+hipError_t launch_solve_mfma(const Params& P, int precision, const long long* grids, float2* S1,
+                             float2* S2, float4* force, const float* Wf, const float* Wi,
+                             const float* gtab, const float* kd, int need_x, const float* rot,
+                             int world, hipStream_t s);
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
                                 const float* rot, const float* off, const float* frot,
                                 const float4* force, int world, float4* avg_out, int mode,

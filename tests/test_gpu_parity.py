@@ -17,11 +17,12 @@ FORCE_TOL = 1e-5
 GRID_TOL = 2e-6
 
 
-def make_solver(k, g, W=100.0, G=100.0, dt=1e-5, damping=0.999, compat="reference", seed=0):
+def make_solver(k, g, W=100.0, G=100.0, dt=1e-5, damping=0.999, compat="reference", seed=0, precision="fp32"):
     from mog_solver_amd.solver import MogSolver
 
     cfg = SolverConfig.from_dict(
-        dict(n_grids=k, grid_size=g, world_size=W, G=G, dt=dt, damping=damping, solver="fft", compat=compat)
+        dict(n_grids=k, grid_size=g, world_size=W, G=G, dt=dt, damping=damping, solver="fft", compat=compat,
+             precision=precision)
     )
     rot, off = O.se3_transforms(k, g, W, seed)
     return MogSolver(cfg, rot=rot, off=off), rot, off
@@ -113,6 +114,32 @@ def test_solve_matches_fp64(torch_cuda, g, compat):
     assert mag < 1e-5
     if compat == "reference" and O.aliases_x_with_z(g):
         assert np.array_equal(got[..., 0], got[..., 2])
+
+
+# Split-precision matrix-core DFT: per-grid force error against FP64 and the
+# particle-force error after averaging over k grids.  Bounds: BF16X3 keeps
+# ~16 mantissa bits per operand (hi + lo), so a grid's force field carries a
+# few 1e-5 relative RMS and the k-average stays below 1e-5; scaled FP16 keeps
+# 11 bits, a few 1e-4 per grid.
+@pytest.mark.parametrize("g,precision,grid_tol,force_tol", [
+    (16, "bf16x3", 1e-4, 1e-5), (32, "bf16x3", 1e-4, 1e-5),
+    (16, "fp16", 5e-3, 2e-3), (32, "fp16", 5e-3, 2e-3)])
+def test_split_precision_dft(torch_cuda, g, precision, grid_tol, force_tol):
+    torch = torch_cuda
+    k, W, G = 24, 100.0, 100.0
+    pos, _ = workloads.plummer_spheres(60000, n_spheres=8, scale=5.0, world_size=W, seed=g)
+    s, rot, off = make_solver(k, g, W, G, precision=precision)
+    P = O.Params.make(k, g, W)
+    grids = O.deposit(pos, rot, off, P)
+    got = s.solve(dev(torch, grids)).cpu().numpy().astype(np.float64)
+    want = O.solve(grids, g, W, G)
+    errs = [O.rel_rms(got[i, ..., :3].reshape(-1, 3), want[i, ..., :3].reshape(-1, 3)) for i in range(k)]
+    assert max(errs) < grid_tol, errs
+    f = s.forces(dev(torch, pos)).cpu().numpy()
+    fw, _ = O.forces(pos, rot, off, k, g, W, G)
+    err = O.rel_rms(f[:, :3], fw)
+    print(f"{precision} g={g}: grid rel rms max {max(errs):.2e}, particle force rel rms {err:.2e}")
+    assert err < force_tol
 
 
 def test_gather_matches_fp64(torch_cuda):

@@ -110,6 +110,8 @@ def _cfg(**kw):
     (dict(compat=5), _abi.MOG_EINVAL),
     (dict(precision=9), _abi.MOG_EINVAL),
     (dict(solver=3), _abi.MOG_EINVAL),
+    (dict(precision=1, grid_size=8), _abi.MOG_EUNSUPPORTED),
+    (dict(precision=2, grid_size=4), _abi.MOG_EUNSUPPORTED),
 ])
 def test_create_rejects_invalid_config(kw, code):
     lib = _abi.load()
