@@ -41,8 +41,12 @@ def parse():
     ap.add_argument("--k", type=int, default=240)
     ap.add_argument("--g", type=int, default=16)
     ap.add_argument("--compat", default="reference", choices=["reference", "physical"])
+    ap.add_argument("--workload", default="two_disks", choices=["two_disks", "plummer", "uniform"],
+                    help="two_disks: n=1e6 colliding disks (k=240, g=16); plummer: 64 Plummer "
+                         "spheres, a=1.5 (use with --n 4000000 --g 32); uniform: U[0,W)^3")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "fp16"])
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--cpu-grids", type=int, default=48,
+    ap.add_argument("--cpu-grids", type=int, default=None,
                     help="grids of the oracle's timed CPU sample (all n particles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)
@@ -65,10 +69,18 @@ def main():
     W, G, dt, damping = 100.0, 100.0, 1e-5, 0.999
     n, k, g = args.n, args.k, args.g
     cfg_d = dict(n_grids=k, grid_size=g, world_size=W, G=G, dt=dt, damping=damping,
-                 solver="fft", compat=args.compat)
+                 solver="fft", compat=args.compat, precision=args.precision)
     cfg = SolverConfig.from_dict(cfg_d)
     off, rot = compute_transforms(k, g, W, seed=0)
-    pos_h, vel_h = workloads.two_disks(n, W, G, g, seed=2)
+    if args.workload == "two_disks":
+        pos_h, vel_h = workloads.two_disks(n, W, G, g, seed=2)
+        wl = "two colliding exponential disks"
+    elif args.workload == "plummer":
+        pos_h, vel_h = workloads.plummer_spheres(n, n_spheres=64, scale=1.5, world_size=W, seed=3)
+        wl = "64 Plummer spheres (a=1.5)"
+    else:
+        pos_h, vel_h = workloads.uniform_cube(n, W, seed=1)
+        wl = "uniform cube"
     print(f"[bench] workload ready n={n} k={k} g={g}", file=sys.stderr, flush=True)
 
     solver = MogSolver(cfg, rot=rot, off=off)
@@ -78,6 +90,8 @@ def main():
 
     # force error against the FP64 oracle on a grid sample (first cpu_grids
     # grids, every particle), from the initial state
+    if args.cpu_grids is None:  # about 10 s of single-core oracle work on this host
+        args.cpu_grids = max(1, int(48 * 1_000_000 / n))
     kk = max(1, min(args.cpu_grids, k))
     sample_cfg = SolverConfig.from_dict(dict(cfg_d, n_grids=kk))
     sample = MogSolver(sample_cfg, rot=rot[:kk], off=off[:kk])
@@ -164,11 +178,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32" if args.precision == "fp32" else f"fp32 state, {args.precision} DFT",
         "data": "synthetic",
         "config": {
-            "workload": f"two colliding exponential disks, n={n}, k={k} grids of g={g}, W={W}, "
-                        f"G={G}, dt={dt}, damping={damping}, compat={args.compat}",
+            "workload": f"{wl}, n={n}, k={k} grids of g={g}, W={W}, "
+                        f"G={G}, dt={dt}, damping={damping}, compat={args.compat}, "
+                        f"DFT precision {args.precision}",
             "n": n, "k": k, "g": g, "parallelism": "single-gpu",
             "graph": use_graph,
         },

@@ -35,6 +35,19 @@ struct mog_ctx {
   float* d_Wf;  // MFMA DFT tables (g >= 16): forward, inverse
   float* d_Wi;
   DepositTune tune;
+  // Morton visiting order of the gather, rebuilt every `resort_every` steps
+  // for the particle array it was built from
+  int* d_order;
+  int* d_idx;
+  unsigned* d_keys_a;
+  unsigned* d_keys_b;
+  void* d_sort_tmp;
+  size_t sort_tmp_bytes;
+  long long order_cap;
+  const void* order_pos;
+  long long order_n;
+  int steps_since_order;
+  int resort_every;
 
   // graph cache of mog_run
   hipStream_t cap_stream;
@@ -42,6 +55,7 @@ struct mog_ctx {
   const void* g_pos;
   const void* g_vel;
   long long g_n;
+  bool g_ordered;
   std::string err;
 };
 
@@ -212,6 +226,11 @@ void release(mog_ctx* c) {
   (void)hipFree(c->d_tw);
   (void)hipFree(c->d_gtab);
   (void)hipFree(c->d_kd);
+  (void)hipFree(c->d_order);
+  (void)hipFree(c->d_idx);
+  (void)hipFree(c->d_keys_a);
+  (void)hipFree(c->d_keys_b);
+  (void)hipFree(c->d_sort_tmp);
   (void)hipFree(c->d_Wf);
   (void)hipFree(c->d_Wi);
 
@@ -252,6 +271,63 @@ DepositTune default_tune(const Params& P) {
   return t;
 }
 
+// Particle counts below this visit in storage order (sorting costs more than
+// it saves).
+const long long kOrderMin = 32768;
+
+void free_order(mog_ctx* c) {
+  (void)hipFree(c->d_order);
+  (void)hipFree(c->d_idx);
+  (void)hipFree(c->d_keys_a);
+  (void)hipFree(c->d_keys_b);
+  (void)hipFree(c->d_sort_tmp);
+  c->d_order = c->d_idx = nullptr;
+  c->d_keys_a = c->d_keys_b = nullptr;
+  c->d_sort_tmp = nullptr;
+  c->order_cap = 0;
+  c->order_pos = nullptr;
+}
+
+// Buffers of the visiting order for n particles (outside graph capture).
+// Returns false (no ordering) when n is small or allocation fails.
+bool ensure_order(mog_ctx* c, int64_t n) {
+  if (n < kOrderMin || c->resort_every <= 0) return false;
+  if (c->order_cap >= n) return true;
+  free_order(c);
+  size_t tmp = 0;
+  if (order_temp_bytes(n, &tmp) != hipSuccess) return false;
+  const size_t b = (size_t)n * 4;
+  if (hipMalloc((void**)&c->d_order, b) != hipSuccess || hipMalloc((void**)&c->d_idx, b) != hipSuccess ||
+      hipMalloc((void**)&c->d_keys_a, b) != hipSuccess ||
+      hipMalloc((void**)&c->d_keys_b, b) != hipSuccess ||
+      hipMalloc(&c->d_sort_tmp, tmp > 0 ? tmp : 16) != hipSuccess) {
+    (void)hipGetLastError();
+    free_order(c);
+    return false;
+  }
+  c->sort_tmp_bytes = tmp;
+  c->order_cap = n;
+  return true;
+}
+
+// Rebuild the order when it is stale for (pos, n); returns the order to use
+// (NULL: storage order).
+hipError_t refresh_order(mog_ctx* c, const float* pos, int64_t n, hipStream_t s, const int** out) {
+  *out = nullptr;
+  if (!ensure_order(c, n)) return hipSuccess;
+  if (c->order_pos != pos || c->order_n != n || c->steps_since_order >= c->resort_every) {
+    hipError_t e = build_order((const float4*)pos, n, c->P.W, c->d_keys_a, c->d_keys_b, c->d_idx,
+                               c->d_order, c->d_sort_tmp, c->sort_tmp_bytes, s);
+    if (e != hipSuccess) return e;
+    c->order_pos = pos;
+    c->order_n = n;
+    c->steps_since_order = 0;
+  }
+  c->steps_since_order++;
+  *out = c->d_order;
+  return hipSuccess;
+}
+
 // Spectral solve: matrix-core passes for g >= 16, vector passes below.
 hipError_t solve_dispatch(mog_ctx* ctx, const long long* grids, float4* force, int world,
                           hipStream_t s) {
@@ -263,7 +339,8 @@ hipError_t solve_dispatch(mog_ctx* ctx, const long long* grids, float4* force, i
                       ctx->need_x, ctx->d_rot, world, s);
 }
 
-hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, hipStream_t s) {
+hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const int* order,
+                        hipStream_t s) {
   const Params& P = ctx->P;
   hipError_t e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
   if (e != hipSuccess) return e;
@@ -273,7 +350,7 @@ hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, hipStre
   e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
   if (e != hipSuccess) return e;
   return launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
-                              ctx->d_frot, ctx->d_force, 1, nullptr, 0, s);
+                              ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s);
 }
 
 }  // namespace
@@ -292,6 +369,7 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
   c->P = make_params(cfg);
   c->need_x = !(cfg->compat == MOG_COMPAT_REFERENCE && aliases_x_with_z(cfg->grid_size));
   c->tune = default_tune(c->P);
+  c->resort_every = 16;
 
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) {
@@ -385,7 +463,9 @@ int mog_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, void* stream) {
   if (rc != MOG_OK) return rc;
   if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
   if (n == 0) return MOG_OK;
-  hipError_t e = enqueue_step(ctx, pos, vel, n, (hipStream_t)stream);
+  const int* order = nullptr;
+  hipError_t e = refresh_order(ctx, pos, n, (hipStream_t)stream, &order);
+  if (e == hipSuccess) e = enqueue_step(ctx, pos, vel, n, order, (hipStream_t)stream);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_step");
 }
 
@@ -399,9 +479,21 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
   if (!use_graph) {
-    for (int i = 0; i < nsteps; ++i)
-      if ((e = enqueue_step(ctx, pos, vel, n, s)) != hipSuccess) return hip_fail(ctx, e, "mog_run");
+    for (int i = 0; i < nsteps; ++i) {
+      const int* order = nullptr;
+      if ((e = refresh_order(ctx, pos, n, s, &order)) != hipSuccess ||
+          (e = enqueue_step(ctx, pos, vel, n, order, s)) != hipSuccess)
+        return hip_fail(ctx, e, "mog_run");
+    }
     return MOG_OK;
+  }
+  // The captured step reads ctx->d_order; the order itself is rebuilt by
+  // direct launches between graph replays (the sort is not captured).
+  const int* const order_buf_before = ctx->d_order;
+  const bool ordered = ensure_order(ctx, n);
+  if (ctx->gexec && (ctx->d_order != order_buf_before || ordered != ctx->g_ordered)) {
+    (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
   }
   if (!ctx->gexec || ctx->g_pos != pos || ctx->g_vel != vel || ctx->g_n != n) {
     if (ctx->gexec) {
@@ -413,7 +505,7 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
       return hip_fail(ctx, e, "hipStreamCreate");
     if ((e = hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return hip_fail(ctx, e, "hipStreamBeginCapture");
-    hipError_t ee = enqueue_step(ctx, pos, vel, n, ctx->cap_stream);
+    hipError_t ee = enqueue_step(ctx, pos, vel, n, ordered ? ctx->d_order : nullptr, ctx->cap_stream);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(ctx->cap_stream, &graph);
     if (ee != hipSuccess) {
@@ -430,9 +522,15 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
     ctx->g_pos = pos;
     ctx->g_vel = vel;
     ctx->g_n = n;
+    ctx->g_ordered = ordered;
   }
-  for (int i = 0; i < nsteps; ++i)
+  for (int i = 0; i < nsteps; ++i) {
+    if (ordered) {
+      const int* order = nullptr;
+      if ((e = refresh_order(ctx, pos, n, s, &order)) != hipSuccess) return hip_fail(ctx, e, "mog_run");
+    }
     if ((e = hipGraphLaunch(ctx->gexec, s)) != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+  }
   return MOG_OK;
 }
 
@@ -444,6 +542,9 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
   if (!phase_ms) return fail(ctx, MOG_EINVAL, "phase_ms is NULL");
   hipStream_t s = (hipStream_t)stream;
   const Params& P = ctx->P;
+  const int* order = nullptr;
+  hipError_t eo = refresh_order(ctx, pos, n, s, &order);
+  if (eo != hipSuccess) return hip_fail(ctx, eo, "mog_step_profiled");
   hipEvent_t ev[5] = {};
   hipError_t e = hipSuccess;
   for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
@@ -459,7 +560,7 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
   if (e == hipSuccess) e = hipEventRecord(ev[3], s);
   if (e == hipSuccess)
     e = launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
-                             ctx->d_frot, ctx->d_force, 1, nullptr, 0, s);
+                             ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s);
   if (e == hipSuccess) e = hipEventRecord(ev[4], s);
   if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
   for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventElapsedTime(&phase_ms[i], ev[i], ev[i + 1]);
@@ -504,7 +605,7 @@ int mog_gather(mog_ctx* ctx, const float* pos, int64_t n, const float* force, fl
     return fail(ctx, MOG_EINVAL, "force/avg_out must be 16-byte aligned");
   hipError_t e = launch_gather_update(ctx->P, (float4*)pos, nullptr, n, ctx->d_rot, ctx->d_off,
                                       ctx->d_frot, (const float4*)force, 0, (float4*)avg_out, 1,
-                                      (hipStream_t)stream);
+                                      nullptr, (hipStream_t)stream);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_gather");
 }
 

@@ -41,14 +41,18 @@ __device__ __forceinline__ void accumulate_rotated(const float* __restrict__ R, 
 // WORLD: the force grids already hold M F (the step's solve rotates each
 // cell once), so a pair costs three adds instead of nine FMAs.  The grid
 // loop is unrolled by U so that U force fetches are in flight per wave.
+// `order` (optional) is the visiting order: thread t handles particle
+// order[t], so that the lanes of a wave are neighbours in space.
 template <bool WORLD>
 __global__ __launch_bounds__(256) void gather_update_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, long long n,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
-    const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode) {
+    const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode,
+    const int* __restrict__ order) {
   constexpr int U = 4;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long long i = order ? (long long)order[t] : t;
   float4 p = pos[i];
   const float c0 = p.x - P.half;
   const float c1 = p.y - P.half;
@@ -115,15 +119,15 @@ __global__ __launch_bounds__(256) void update_kernel(float4* __restrict__ pos,
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
                                 const float* rot, const float* off, const float* frot,
                                 const float4* force, int world, float4* avg_out, int mode,
-                                hipStream_t s) {
+                                const int* order, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const long long blocks = (n + 255) / 256;
   if (world)
     hipLaunchKernelGGL(gather_update_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, pos,
-                       vel, n, rot, off, frot, force, P, avg_out, mode);
+                       vel, n, rot, off, frot, force, P, avg_out, mode, order);
   else
     hipLaunchKernelGGL(gather_update_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, pos,
-                       vel, n, rot, off, frot, force, P, avg_out, mode);
+                       vel, n, rot, off, frot, force, P, avg_out, mode, order);
   return hipGetLastError();
 }
 

@@ -89,7 +89,14 @@ hipError_t launch_solve_mfma(const Params& P, int precision, const long long* gr
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
                                 const float* rot, const float* off, const float* frot,
                                 const float4* force, int world, float4* avg_out, int mode,
-                                hipStream_t s);
+                                const int* order, hipStream_t s);
+// Morton visiting order of the particles (order.hip): temp storage size of
+// the radix sort, and the order itself (keys_a/keys_b/idx_a: n-element
+// scratch; order: n-element output).
+hipError_t order_temp_bytes(long long n, size_t* bytes);
+hipError_t build_order(const float4* pos, long long n, float W, unsigned* keys_a,
+                       unsigned* keys_b, int* idx_a, int* order, void* tmp, size_t tmp_bytes,
+                       hipStream_t s);
 hipError_t launch_update(const Params& P, float4* pos, float4* vel, const float4* avg,
                          long long n, hipStream_t s);
 
