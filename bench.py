@@ -50,6 +50,8 @@ def parse():
                     help="grids of the oracle's timed CPU sample (all n particles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default tuning instead of autotuning first")
     return ap.parse_args()
 
 
@@ -97,6 +99,14 @@ def main():
     sample = MogSolver(sample_cfg, rot=rot[:kk], off=off[:kk])
     gpu_avg = sample.forces(pos).cpu().numpy()[:, :3].astype(np.float64)
     sample.close()
+
+    tuned = None
+    if not args.no_autotune:  # outside the timed region, on copies of the state
+        from mog_solver_amd.autotune import autotune
+
+        tuned = autotune(solver, pos, vel, cache_path=None)
+        print(f"[bench] autotuned ({tuned['evaluated']} candidates): {tuned['tuning']}",
+              file=sys.stderr, flush=True)
 
     use_graph = not args.no_graph
     solver.run(pos, vel, args.warmup, use_graph=use_graph)
@@ -190,6 +200,8 @@ def main():
         "interactions_per_s": round(n * k * steps_per_s, 1),
         "force_rel_rms_vs_fp64": None if force_err is None else float(f"{force_err:.3e}"),
         "force_sample_grids": kk,
+        "tuning": solver.get_tuning(),
+        "autotuned": tuned is not None,
         "phases_ms": {key: round(v, 4) for key, v in phases.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -61,6 +61,21 @@ typedef struct mog_config {
   int32_t reserved;
 } mog_config;
 
+/* Launch and algorithm choices of a context (defaults from mog_create;
+ * results do not depend on them, only speed does).  The reference has no
+ * counterpart: its dispatch sizes are fixed (main.py:585, 596, 607, 668). */
+typedef struct mog_tuning {
+  int32_t deposit_threads;      /* threads per deposit workgroup: 64..1024, multiple of 64 */
+  int32_t deposit_grids_per_wg; /* grids privatised per workgroup: 1, 2, 4 or 8 */
+  int32_t deposit_wg_target;    /* approximate number of deposit workgroups, >= 1 */
+  int32_t resort_every;         /* steps between rebuilds of the Morton visiting order; 0 = off */
+  int32_t sorted_deposit;       /* 1: deposit positions in visiting order, wave-aggregated adds */
+  int32_t cell_cache;           /* 1: the sorted deposit stores cells for the gather (g <= 32) */
+  int32_t spectral_mfma;        /* 1: matrix-core DFT for FP32 at g >= 16 (other precisions
+                                   always use it) */
+  int32_t reserved[9];
+} mog_tuning;
+
 typedef struct mog_ctx mog_ctx;
 
 /* ABI version of the loaded library (MOG_ABI_VERSION). */
@@ -119,6 +134,13 @@ int mog_update(mog_ctx* ctx, float* pos_n4, float* vel_n4, const float* avg_forc
  * The reference has no timing facility; this exists for the benchmark. */
 int mog_step_profiled(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, float* phase_ms,
                       void* stream);
+
+/* Current tuning of `ctx` (host struct, synchronous). */
+int mog_get_tuning(mog_ctx* ctx, mog_tuning* out);
+
+/* Replace the tuning of `ctx`; invalid fields are rejected with MOG_EINVAL
+ * and nothing changes.  Drops a captured mog_run graph. */
+int mog_set_tuning(mog_ctx* ctx, const mog_tuning* t);
 
 /* Device pointers of the context's own density grids (int64) and force
  * grids (float4) as left by the last mog_step; NULL on error. */

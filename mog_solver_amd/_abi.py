@@ -36,6 +36,8 @@ EXPORTED = (
     "mog_gather",
     "mog_update",
     "mog_step_profiled",
+    "mog_get_tuning",
+    "mog_set_tuning",
     "mog_grids_ptr",
     "mog_force_ptr",
     "mog_last_error",
@@ -58,6 +60,27 @@ class MogConfig(ctypes.Structure):
         ("solver", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
     ]
+
+
+class MogTuning(ctypes.Structure):
+    """Mirror of ``struct mog_tuning`` in include/mog.h."""
+
+    _fields_ = [
+        ("deposit_threads", ctypes.c_int32),
+        ("deposit_grids_per_wg", ctypes.c_int32),
+        ("deposit_wg_target", ctypes.c_int32),
+        ("resort_every", ctypes.c_int32),
+        ("sorted_deposit", ctypes.c_int32),
+        ("cell_cache", ctypes.c_int32),
+        ("spectral_mfma", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 9),
+    ]
+
+    FIELDS = ("deposit_threads", "deposit_grids_per_wg", "deposit_wg_target", "resort_every",
+              "sorted_deposit", "cell_cache", "spectral_mfma")
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f in self.FIELDS}
 
 
 class MogError(RuntimeError):
@@ -108,6 +131,10 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     lib.mog_update.argtypes = [vp, vp, vp, vp, i64, vp]
     lib.mog_step_profiled.restype = ctypes.c_int
     lib.mog_step_profiled.argtypes = [vp, vp, vp, i64, vp, vp]
+    lib.mog_get_tuning.restype = ctypes.c_int
+    lib.mog_get_tuning.argtypes = [vp, ctypes.POINTER(MogTuning)]
+    lib.mog_set_tuning.restype = ctypes.c_int
+    lib.mog_set_tuning.argtypes = [vp, ctypes.POINTER(MogTuning)]
     lib.mog_grids_ptr.restype = vp
     lib.mog_grids_ptr.argtypes = [vp]
     lib.mog_force_ptr.restype = vp

@@ -48,6 +48,13 @@ struct mog_ctx {
   long long order_n;
   int steps_since_order;
   int resort_every;
+  bool sorted_deposit;
+  bool cell_cache;
+  bool spectral_mfma;
+  // sorted-step buffers (g <= 32): positions in visiting order and the
+  // uint16 cell of every (grid, slot) pair
+  float4* d_spos;
+  unsigned short* d_cells;
 
   // graph cache of mog_run
   hipStream_t cap_stream;
@@ -250,24 +257,18 @@ int check_particles(mog_ctx* ctx, const void* pos, const void* vel, int64_t n, b
   return MOG_OK;
 }
 
-// Default deposit tiling: 512 threads and about 32 KB of LDS counters per
-// workgroup (several workgroups per CU), about 2048 workgroups in total.
+// Default deposit tiling (measured on MI355X, two-disk and Plummer
+// workloads): about 32 KB of LDS counters per workgroup of 512 threads and
+// about 2048 workgroups; when one grid alone needs more than 64 KB (g = 32)
+// the CU holds a single workgroup, so it gets 1024 threads and ~1024
+// workgroups.
 DepositTune default_tune(const Params& P) {
   DepositTune t;
-  t.threads = 512;
-  const int per = (int)((32 * 1024) / ((size_t)P.cells * 4));
-  t.grids_per_wg = per < 1 ? 1 : per;
-  t.wg_target = 2048;
-  const char* env = getenv("MOG_DEPOSIT_TUNE");
-  if (env) {
-    int a = 0, b = 0, c = 0;
-    if (sscanf(env, "%d,%d,%d", &a, &b, &c) == 3 && a >= 64 && a <= 1024 && (a % 64) == 0 &&
-        b >= 1 && c >= 1) {
-      t.threads = a;
-      t.grids_per_wg = b;
-      t.wg_target = c;
-    }
-  }
+  const size_t grid_bytes = (size_t)P.cells * 4;
+  const int per = (int)((32 * 1024) / grid_bytes);
+  t.grids_per_wg = per < 1 ? 1 : (per > 8 ? 8 : per);
+  t.threads = grid_bytes > 64 * 1024 ? 1024 : 512;
+  t.wg_target = grid_bytes > 64 * 1024 ? 1024 : 2048;
   return t;
 }
 
@@ -276,6 +277,10 @@ DepositTune default_tune(const Params& P) {
 const long long kOrderMin = 32768;
 
 void free_order(mog_ctx* c) {
+  (void)hipFree(c->d_spos);
+  (void)hipFree(c->d_cells);
+  c->d_spos = nullptr;
+  c->d_cells = nullptr;
   (void)hipFree(c->d_order);
   (void)hipFree(c->d_idx);
   (void)hipFree(c->d_keys_a);
@@ -307,6 +312,19 @@ bool ensure_order(mog_ctx* c, int64_t n) {
   }
   c->sort_tmp_bytes = tmp;
   c->order_cap = n;
+  // sorted deposit + cached gather need the copies; without them the step
+  // still uses the visiting order in the gather
+  if (c->P.g <= 32 && c->sorted_deposit) {
+    if (hipMalloc((void**)&c->d_spos, (size_t)n * sizeof(float4)) != hipSuccess ||
+        hipMalloc((void**)&c->d_cells,
+                  (size_t)((n + 63) / 64) * 64 * c->P.k * sizeof(unsigned short)) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(c->d_spos);
+      (void)hipFree(c->d_cells);
+      c->d_spos = nullptr;
+      c->d_cells = nullptr;
+    }
+  }
   return true;
 }
 
@@ -332,7 +350,7 @@ hipError_t refresh_order(mog_ctx* c, const float* pos, int64_t n, hipStream_t s,
 hipError_t solve_dispatch(mog_ctx* ctx, const long long* grids, float4* force, int world,
                           hipStream_t s) {
   const Params& P = ctx->P;
-  if (P.g >= 16)
+  if (P.g >= 16 && (ctx->spectral_mfma || ctx->cfg.precision != MOG_PRECISION_FP32))
     return launch_solve_mfma(P, ctx->cfg.precision, grids, ctx->d_S1, ctx->d_S2, force, ctx->d_Wf,
                              ctx->d_Wi, ctx->d_gtab, ctx->d_kd, ctx->need_x, ctx->d_rot, world, s);
   return launch_solve(P, grids, ctx->d_S1, ctx->d_S2, force, ctx->d_tw, ctx->d_gtab, ctx->d_kd,
@@ -342,6 +360,27 @@ hipError_t solve_dispatch(mog_ctx* ctx, const long long* grids, float4* force, i
 hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const int* order,
                         hipStream_t s) {
   const Params& P = ctx->P;
+  if (order && ctx->sorted_deposit && ctx->d_spos && ctx->d_cells) {
+    // sorted step: positions in visiting order, deposit with aggregated
+    // atomics (and a cell cache), gather from the cache or recomputed cells
+    const bool cache = ctx->cell_cache;
+    hipError_t e = launch_permute((const float4*)pos, order, n, ctx->d_spos, s);
+    if (e == hipSuccess) e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
+    if (e == hipSuccess)
+      e = launch_deposit_ex(P, ctx->d_spos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
+                            (unsigned long long*)ctx->d_grids, cache ? ctx->d_cells : nullptr,
+                            true, ctx->tune, s);
+    if (e == hipSuccess) e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
+    if (e == hipSuccess) {
+      if (cache)
+        e = launch_gather_cached(P, (float4*)pos, (float4*)vel, n, order, ctx->d_cells,
+                                 ctx->d_force, s);
+      else
+        e = launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
+                                 ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s);
+    }
+    return e;
+  }
   hipError_t e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
   if (e != hipSuccess) return e;
   e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
@@ -370,6 +409,9 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
   c->need_x = !(cfg->compat == MOG_COMPAT_REFERENCE && aliases_x_with_z(cfg->grid_size));
   c->tune = default_tune(c->P);
   c->resort_every = 16;
+  c->sorted_deposit = false;
+  c->cell_cache = false;
+  c->spectral_mfma = true;
 
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) {
@@ -547,20 +589,28 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
   if (eo != hipSuccess) return hip_fail(ctx, eo, "mog_step_profiled");
   hipEvent_t ev[5] = {};
   hipError_t e = hipSuccess;
+  const bool sorted = order && ctx->sorted_deposit && ctx->d_spos && ctx->d_cells;
+  const bool cache = sorted && ctx->cell_cache;
   for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
   if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+  if (e == hipSuccess && sorted) e = launch_permute((const float4*)pos, order, n, ctx->d_spos, s);
   if (e == hipSuccess) e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
   if (e == hipSuccess) e = hipEventRecord(ev[1], s);
   if (e == hipSuccess)
-    e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                       (unsigned long long*)ctx->d_grids, ctx->tune, s);
+    e = launch_deposit_ex(P, sorted ? ctx->d_spos : (const float4*)pos, n, ctx->d_rot, ctx->d_off,
+                          ctx->d_frot, (unsigned long long*)ctx->d_grids,
+                          cache ? ctx->d_cells : nullptr, sorted, ctx->tune, s);
   if (e == hipSuccess) e = hipEventRecord(ev[2], s);
-  if (e == hipSuccess)
-    e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
+  if (e == hipSuccess) e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
   if (e == hipSuccess) e = hipEventRecord(ev[3], s);
-  if (e == hipSuccess)
-    e = launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
-                             ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s);
+  if (e == hipSuccess) {
+    if (cache)
+      e = launch_gather_cached(P, (float4*)pos, (float4*)vel, n, order, ctx->d_cells, ctx->d_force,
+                               s);
+    else
+      e = launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
+                               ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s);
+  }
   if (e == hipSuccess) e = hipEventRecord(ev[4], s);
   if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
   for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventElapsedTime(&phase_ms[i], ev[i], ev[i + 1]);
@@ -618,6 +668,51 @@ int mog_update(mog_ctx* ctx, float* pos, float* vel, const float* avg, int64_t n
   hipError_t e = launch_update(ctx->P, (float4*)pos, (float4*)vel, (const float4*)avg, n,
                                (hipStream_t)stream);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_update");
+}
+
+int mog_get_tuning(mog_ctx* ctx, mog_tuning* out) {
+  if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
+  if (!out) return fail(ctx, MOG_EINVAL, "out is NULL");
+  memset(out, 0, sizeof *out);
+  out->deposit_threads = ctx->tune.threads;
+  out->deposit_grids_per_wg = ctx->tune.grids_per_wg;
+  out->deposit_wg_target = ctx->tune.wg_target;
+  out->resort_every = ctx->resort_every;
+  out->sorted_deposit = ctx->sorted_deposit ? 1 : 0;
+  out->cell_cache = ctx->cell_cache ? 1 : 0;
+  out->spectral_mfma = ctx->spectral_mfma ? 1 : 0;
+  return MOG_OK;
+}
+
+int mog_set_tuning(mog_ctx* ctx, const mog_tuning* t) {
+  if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
+  if (!t) return fail(ctx, MOG_EINVAL, "tuning is NULL");
+  if (t->deposit_threads < 64 || t->deposit_threads > 1024 || (t->deposit_threads % 64) != 0)
+    return fail(ctx, MOG_EINVAL, "deposit_threads must be a multiple of 64 in [64, 1024]");
+  const int G = t->deposit_grids_per_wg;
+  if (G != 1 && G != 2 && G != 4 && G != 8)
+    return fail(ctx, MOG_EINVAL, "deposit_grids_per_wg must be 1, 2, 4 or 8");
+  if (t->deposit_wg_target < 1) return fail(ctx, MOG_EINVAL, "deposit_wg_target must be >= 1");
+  if (t->resort_every < 0) return fail(ctx, MOG_EINVAL, "resort_every must be >= 0");
+  if ((t->sorted_deposit | t->cell_cache | t->spectral_mfma) & ~1)
+    return fail(ctx, MOG_EINVAL, "flags must be 0 or 1");
+  if (t->cell_cache && !t->sorted_deposit)
+    return fail(ctx, MOG_EINVAL, "cell_cache needs sorted_deposit");
+  ctx->tune.threads = t->deposit_threads;
+  ctx->tune.grids_per_wg = G;
+  ctx->tune.wg_target = t->deposit_wg_target;
+  const bool realloc = (t->sorted_deposit != 0) != ctx->sorted_deposit;
+  ctx->resort_every = t->resort_every;
+  ctx->sorted_deposit = t->sorted_deposit != 0;
+  ctx->cell_cache = t->cell_cache != 0;
+  ctx->spectral_mfma = t->spectral_mfma != 0;
+  if (realloc) free_order(ctx);  // buffers follow the sorted-deposit choice
+  ctx->order_pos = nullptr;      // rebuild the order on the next step
+  if (ctx->gexec) {
+    (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+  }
+  return MOG_OK;
 }
 
 const int64_t* mog_grids_ptr(mog_ctx* ctx) { return ctx ? (const int64_t*)ctx->d_grids : nullptr; }

@@ -129,26 +129,146 @@ __global__ __launch_bounds__(256) void deposit_global_kernel(
   }
 }
 
+// Sum of v over the 64 lanes of the wave (every lane must be active):
+// DPP within rows of 16 (quad swaps, half-row and row mirrors), then the four
+// row sums through readlane.
+__device__ __forceinline__ unsigned wave_sum(unsigned v) {
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);  // row_mirror
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 0) + (unsigned)__builtin_amdgcn_readlane((int)v, 16) +
+         (unsigned)__builtin_amdgcn_readlane((int)v, 32) + (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+}
+
+// hist[cell] += v over the wave (v == 0: no contribution).  Lanes that share
+// the cell of a leader lane are summed and added with one atomic (two
+// rounds); the rest add their own value.  Called with every lane active.
+__device__ __forceinline__ void wave_aggregate_add(unsigned int* hist, int cell, unsigned v) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(v != 0u);
+#pragma unroll
+  for (int round = 0; round < 2; ++round) {
+    if (pending == 0ull) return;
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lc = __builtin_amdgcn_readlane(cell, leader);
+    const bool mine = ((pending >> lane) & 1ull) && cell == lc;
+    const unsigned s = wave_sum(mine ? v : 0u);
+    if (lane == leader) atomicAdd(&hist[lc], s);
+    pending &= ~__ballot(mine);
+  }
+  if ((pending >> lane) & 1ull) atomicAdd(&hist[cell], v);
+}
+
+// Deposit over positions in Morton order (spos[t] = pos[order[t]]): the
+// lanes of a wave are neighbours in space and mostly share cells, so LDS
+// adds are aggregated per wave.  With CACHE the cell of every (grid, slot)
+// pair is stored in cells_out[cache_index(t, grid, k)] for the gather.
+template <int G, bool CACHE>
+__global__ __launch_bounds__(1024) void deposit_sorted_kernel(
+    const float4* __restrict__ spos, long long n, long long chunk,
+    const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
+    Params P, unsigned long long lds_limit, unsigned long long* __restrict__ grids,
+    unsigned short* __restrict__ cells_out) {
+  extern __shared__ unsigned int hist[];
+  const int cells = P.cells;
+  const int grid0 = blockIdx.y * G;
+  const int ng = min(G, P.k - grid0);
+  const int total = ng * cells;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) hist[i] = 0u;
+  float F[G][12];
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) {
+    const int gr = min(grid0 + gi, P.k - 1);
+#pragma unroll
+    for (int j = 0; j < 12; ++j) F[gi][j] = frot[12 * gr + j];
+  }
+  __syncthreads();
+  const long long p0 = (long long)blockIdx.x * chunk;
+  const long long p1 = min(n, p0 + chunk);
+  for (long long base = p0; base < p1; base += blockDim.x) {  // same trip count for all lanes
+    const long long p = base + threadIdx.x;
+    const bool live = p < p1;
+    const float4 q = live ? spos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const unsigned long long milli = milli_of(q.w);
+    const unsigned lds_v = (milli <= lds_limit) ? (unsigned)milli : 0u;
+    const bool glob = milli > lds_limit;
+    const float c0 = q.x - P.half;
+    const float c1 = q.y - P.half;
+    const float c2 = q.z - P.half;
+    const int pfast = particle_fast(c0, c1, c2, P.CB);
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+      if (gi < ng) {
+        const int gr = grid0 + gi;
+        const int cell = cell_index(c0, c1, c2, pfast, F[gi], rot + 12 * gr, off + 4 * gr, P);
+        if (CACHE && live) cells_out[cache_index(p, gr, P.k)] = (unsigned short)cell;
+        if (glob) atomicAdd(&grids[(long long)gr * cells + cell], milli);
+        wave_aggregate_add(hist + gi * cells, cell, lds_v);
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long* dst = grids + (long long)grid0 * cells;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const unsigned int v = hist[i];
+    if (v) atomicAdd(&dst[i], (unsigned long long)v);
+  }
+}
+
+__global__ __launch_bounds__(256) void permute_kernel(const float4* __restrict__ pos,
+                                                      const int* __restrict__ order, long long n,
+                                                      float4* __restrict__ spos) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) spos[t] = pos[order[t]];
+}
+
+hipError_t launch_permute(const float4* pos, const int* order, long long n, float4* spos,
+                          hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos,
+                     order, n, spos);
+  return hipGetLastError();
+}
+
 static const size_t kMaxLds = 160 * 1024;
 
 template <int G>
 static hipError_t launch_lds(const Params& P, const float4* pos, long long n, long long chunks,
                              long long chunk, int threads, const float* rot, const float* off,
-                             const float* frot, unsigned long long* grids, hipStream_t s) {
+                             const float* frot, unsigned long long* grids,
+                             unsigned short* cells_out, bool sorted, hipStream_t s) {
   const int groups = (P.k + G - 1) / G;
   const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)chunk;
   const size_t lds = (size_t)G * P.cells * 4;
-  hipError_t e = hipFuncSetAttribute((const void*)deposit_lds_kernel<G>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+  const void* fn = !sorted ? (const void*)deposit_lds_kernel<G>
+                   : cells_out ? (const void*)deposit_sorted_kernel<G, true>
+                               : (const void*)deposit_sorted_kernel<G, false>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(deposit_lds_kernel<G>, dim3((unsigned)chunks, (unsigned)groups),
-                     dim3(threads), lds, s, pos, n, chunk, rot, off, frot, P, lds_limit, grids);
+  const dim3 grid((unsigned)chunks, (unsigned)groups);
+  if (!sorted)
+    hipLaunchKernelGGL(deposit_lds_kernel<G>, grid, dim3(threads), lds, s, pos, n, chunk, rot, off,
+                       frot, P, lds_limit, grids);
+  else if (cells_out)
+    hipLaunchKernelGGL((deposit_sorted_kernel<G, true>), grid, dim3(threads), lds, s, pos, n, chunk,
+                       rot, off, frot, P, lds_limit, grids, cells_out);
+  else
+    hipLaunchKernelGGL((deposit_sorted_kernel<G, false>), grid, dim3(threads), lds, s, pos, n,
+                       chunk, rot, off, frot, P, lds_limit, grids, cells_out);
   return hipGetLastError();
 }
 
 hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const float* rot,
                           const float* off, const float* frot, unsigned long long* grids,
                           const DepositTune& t, hipStream_t s) {
+  return launch_deposit_ex(P, pos, n, rot, off, frot, grids, nullptr, false, t, s);
+}
+
+hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, const float* rot,
+                             const float* off, const float* frot, unsigned long long* grids,
+                             unsigned short* cells_out, bool sorted, const DepositTune& t,
+                             hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const size_t grid_bytes = (size_t)P.cells * 4;
   if (grid_bytes > kMaxLds) {
@@ -172,10 +292,18 @@ hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const
   if (chunk < threads) chunk = threads;
   chunks = (n + chunk - 1) / chunk;
   switch (G) {
-    case 8: return launch_lds<8>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, s);
-    case 4: return launch_lds<4>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, s);
-    case 2: return launch_lds<2>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, s);
-    default: return launch_lds<1>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, s);
+    case 8:
+      return launch_lds<8>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
+                           sorted, s);
+    case 4:
+      return launch_lds<4>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
+                           sorted, s);
+    case 2:
+      return launch_lds<2>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
+                           sorted, s);
+    default:
+      return launch_lds<1>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
+                           sorted, s);
   }
 }
 

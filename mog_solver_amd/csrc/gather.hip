@@ -102,6 +102,54 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
   vel[i] = v;
 }
 
+// Gather + update from the cell cache of the step's sorted deposit: slot t
+// reads its k cells with coalesced uint16 loads (no cell arithmetic), the
+// world-frame forces of those cells, then updates particle order[t].
+__global__ __launch_bounds__(256) void gather_cached_kernel(
+    float4* __restrict__ pos, float4* __restrict__ vel, long long n, const int* __restrict__ order,
+    const unsigned short* __restrict__ cells, const float4* __restrict__ force, Params P) {
+  constexpr int U = 8;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float tx = 0.f, ty = 0.f, tz = 0.f;
+  int grid = 0;
+  for (; grid + U <= P.k; grid += U) {
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = cells[cache_index(t, grid + u, P.k)];
+    float4 F[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) F[u] = force[(long long)(grid + u) * P.cells + c[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      tx += F[u].x;
+      ty += F[u].y;
+      tz += F[u].z;
+    }
+  }
+  for (; grid < P.k; ++grid) {
+    const float4 F = force[(long long)grid * P.cells + cells[cache_index(t, grid, P.k)]];
+    tx += F.x;
+    ty += F.y;
+    tz += F.z;
+  }
+  const long long i = order[t];
+  float4 p = pos[i];
+  float4 v = vel[i];
+  integrate(p, v, tx / P.fk, ty / P.fk, tz / P.fk, P);
+  pos[i] = p;
+  vel[i] = v;
+}
+
+hipError_t launch_gather_cached(const Params& P, float4* pos, float4* vel, long long n,
+                                const int* order, const unsigned short* cells,
+                                const float4* force, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_cached_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos,
+                     vel, n, order, cells, force, P);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void update_kernel(float4* __restrict__ pos,
                                                      float4* __restrict__ vel,
                                                      const float4* __restrict__ avg, long long n,

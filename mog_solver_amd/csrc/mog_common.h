@@ -52,6 +52,13 @@ __device__ __forceinline__ int cell_index(float c0, float c1, float c2, int pfas
   return cell;
 }
 
+// Position of the cached cell of (slot t, grid) in the uint16 cell cache:
+// tiles of 64 slots (one wave) x k grids, so a wave's deposit writes and its
+// gather reads whole contiguous rows.
+__device__ __forceinline__ long long cache_index(long long t, int grid, int k) {
+  return (((t >> 6) * k + grid) << 6) | (t & 63);
+}
+
 // uint(mass * 1000.0) of mass.glsl:54 as a 64-bit integer; negative and NaN
 // masses deposit 0.
 __device__ __forceinline__ unsigned long long milli_of(float m) {
@@ -76,6 +83,21 @@ hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const
                           const DepositTune& t, hipStream_t s);
 // world != 0: force grids hold M F (world frame) for the step's gather;
 // world == 0: grid-frame forces as pack.glsl writes them.
+// Deposit of positions already in Morton visiting order (sorted != 0; lanes
+// aggregate shared cells) with an optional uint16 cell cache
+// cells_out[cache_index(slot, grid, k)] (g <= 32 only, sorted only).
+hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, const float* rot,
+                             const float* off, const float* frot, unsigned long long* grids,
+                             unsigned short* cells_out, bool sorted, const DepositTune& t,
+                             hipStream_t s);
+// spos[t] = pos[order[t]]
+hipError_t launch_permute(const float4* pos, const int* order, long long n, float4* spos,
+                          hipStream_t s);
+// Gather + update of Morton-ordered slots from the cell cache: slot t is
+// particle order[t] and its cell in grid g is cells[cache_index(t, g, k)].
+hipError_t launch_gather_cached(const Params& P, float4* pos, float4* vel, long long n,
+                                const int* order, const unsigned short* cells,
+                                const float4* force, hipStream_t s);
 hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
                         float4* force, const float2* tw, const float* gtab, const float* kd,
                         int need_x, const float* rot, int world, hipStream_t s);

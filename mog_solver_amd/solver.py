@@ -108,6 +108,22 @@ class MogSolver:
         self.off = off
         self._check(self.lib.mog_set_transforms(self._ctx, rot.ctypes.data, off.ctypes.data))
 
+    # -- tuning -------------------------------------------------------------
+    def get_tuning(self) -> dict:
+        t = _abi.MogTuning()
+        self._check(self.lib.mog_get_tuning(self._ctx, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **fields):
+        """Update tuning fields (see ``struct mog_tuning``); others keep their value."""
+        cur = self.get_tuning()
+        unknown = set(fields) - set(cur)
+        if unknown:
+            raise KeyError(f"unknown tuning fields {sorted(unknown)}")
+        cur.update(fields)
+        t = _abi.MogTuning(**{k: int(v) for k, v in cur.items()})
+        self._check(self.lib.mog_set_tuning(self._ctx, ctypes.byref(t)))
+
     @property
     def cells(self):
         g = self.cfg.grid_size

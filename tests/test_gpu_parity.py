@@ -229,6 +229,50 @@ def test_particle_count_changes_between_steps(torch_cuda):
         assert torch.equal(pa, pb) and torch.equal(va, vb), n
 
 
+@pytest.mark.parametrize("g", [16, 32])
+def test_tuning_variants_agree(torch_cuda, g):
+    """Every tuning choice gives the same step: deposit/gather/visiting-order
+    variants bit-exactly, the vector DFT within FP32 rounding."""
+    torch = torch_cuda
+    k, W = 12, 100.0
+    pos, vel = workloads.two_disks(80000, W, 100.0, g, seed=6)
+    s, rot, off = make_solver(k, g, W)
+    base = s.get_tuning()
+    variants = [
+        dict(),
+        dict(resort_every=0),
+        dict(resort_every=3, sorted_deposit=1),
+        dict(resort_every=2, sorted_deposit=1, cell_cache=1),
+        dict(sorted_deposit=1, cell_cache=1, deposit_grids_per_wg=1, deposit_threads=256),
+        dict(deposit_grids_per_wg=4, deposit_threads=1024, deposit_wg_target=300),
+        dict(deposit_grids_per_wg=8, deposit_threads=64, deposit_wg_target=50),
+    ]
+    results = []
+    for v in variants:
+        s.set_tuning(**{**base, **v})
+        p, vv = dev(torch, pos), dev(torch, vel)
+        s.run(p, vv, 2, use_graph=True)
+        s.step(p, vv)
+        torch.cuda.synchronize()
+        results.append((p.cpu().numpy(), vv.cpu().numpy()))
+    for (p, vv), v in zip(results[1:], variants[1:]):
+        assert np.array_equal(p, results[0][0]) and np.array_equal(vv, results[0][1]), v
+    s.set_tuning(**{**base, "spectral_mfma": 0})
+    p, vv = dev(torch, pos), dev(torch, vel)
+    s.run(p, vv, 3, use_graph=False)
+    assert O.rel_rms(vv.cpu().numpy()[:, :3], results[0][1][:, :3]) < 1e-5
+
+
+def test_tuning_rejects_invalid(torch_cuda):
+    from mog_solver_amd._abi import MogError
+
+    s, _, _ = make_solver(4, 16)
+    for bad in (dict(deposit_threads=100), dict(deposit_grids_per_wg=3), dict(deposit_wg_target=0),
+                dict(resort_every=-1), dict(cell_cache=1, sorted_deposit=0), dict(spectral_mfma=2)):
+        with pytest.raises(MogError):
+            s.set_tuning(**bad)
+
+
 def test_trajectory_c1_10_steps(torch_cuda):
     """10 steps of the CPU-runnable config vs the oracle (FP64 forces,
     float32 state).  Per-particle agreement is statistical: a one-ulp force
