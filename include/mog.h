@@ -135,6 +135,21 @@ int mog_update(mog_ctx* ctx, float* pos_n4, float* vel_n4, const float* avg_forc
 int mog_step_profiled(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, float* phase_ms,
                       void* stream);
 
+/* Change deltaTime = dt * timescale and the damping factor between steps
+ * (main.py:662, 667 read them every step; the reference's [ ] \ keys change
+ * timescale at run time, main.py:739-750).  Drops a captured mog_run graph. */
+int mog_set_integrator(mog_ctx* ctx, double dt, double damping, double timescale);
+
+/* Perlin-fBm rejection-sampled initial conditions on the device:
+ * shaders/init_particles.glsl driven by Simulation.init_particles_gpu
+ * (main.py:283-310): positions in a sphere (spawn_shape 1) or cube (0) of the
+ * box shrunk by spawn_buffer * W, density pow(clamp(fbm4 + 0.5, 0, 1),
+ * density_contrast) at noise frequency noise_scale / W, masses base_mass,
+ * velocities uniform in [-W/2, W/2).  No context needed. */
+int mog_init_perlin(float* pos_n4, float* vel_n4, int64_t n, double world_size,
+                    double noise_scale, double density_contrast, uint32_t seed,
+                    double base_mass, double spawn_buffer, int32_t spawn_shape, void* stream);
+
 /* Current tuning of `ctx` (host struct, synchronous). */
 int mog_get_tuning(mog_ctx* ctx, mog_tuning* out);
 

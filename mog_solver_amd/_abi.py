@@ -36,6 +36,8 @@ EXPORTED = (
     "mog_gather",
     "mog_update",
     "mog_step_profiled",
+    "mog_set_integrator",
+    "mog_init_perlin",
     "mog_get_tuning",
     "mog_set_tuning",
     "mog_grids_ptr",
@@ -131,6 +133,11 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     lib.mog_update.argtypes = [vp, vp, vp, vp, i64, vp]
     lib.mog_step_profiled.restype = ctypes.c_int
     lib.mog_step_profiled.argtypes = [vp, vp, vp, i64, vp, vp]
+    lib.mog_set_integrator.restype = ctypes.c_int
+    lib.mog_set_integrator.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.mog_init_perlin.restype = ctypes.c_int
+    lib.mog_init_perlin.argtypes = [vp, vp, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_uint32, ctypes.c_double, ctypes.c_double, i32, vp]
     lib.mog_get_tuning.restype = ctypes.c_int
     lib.mog_get_tuning.argtypes = [vp, ctypes.POINTER(MogTuning)]
     lib.mog_set_tuning.restype = ctypes.c_int

@@ -670,6 +670,40 @@ int mog_update(mog_ctx* ctx, float* pos, float* vel, const float* avg, int64_t n
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_update");
 }
 
+int mog_set_integrator(mog_ctx* ctx, double dt, double damping, double timescale) {
+  if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
+  if (!isfinite(dt) || !isfinite(damping) || !isfinite(timescale))
+    return fail(ctx, MOG_EINVAL, "dt, damping and timescale must be finite");
+  ctx->cfg.dt = dt;
+  ctx->cfg.damping = damping;
+  ctx->cfg.timescale = timescale;
+  ctx->P.dtp = (float)(dt * timescale);
+  ctx->P.damping = (float)damping;
+  if (ctx->gexec) {  // kernel arguments of a captured step changed
+    (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+  }
+  return MOG_OK;
+}
+
+int mog_init_perlin(float* pos, float* vel, int64_t n, double world_size, double noise_scale,
+                    double density_contrast, uint32_t seed, double base_mass, double spawn_buffer,
+                    int32_t spawn_shape, void* stream) {
+  if (n < 0 || n > (int64_t)2147483647) return fail(nullptr, MOG_EINVAL, "n must be in [0, 2^31)");
+  if (n == 0) return MOG_OK;
+  if (!pos || !vel) return fail(nullptr, MOG_EINVAL, "pos/vel is NULL");
+  if (!aligned16(pos) || !aligned16(vel)) return fail(nullptr, MOG_EINVAL, "pos/vel must be 16-byte aligned");
+  if (!(isfinite(world_size) && world_size > 0.0) || !isfinite(noise_scale) ||
+      !isfinite(density_contrast) || !isfinite(base_mass) || !isfinite(spawn_buffer))
+    return fail(nullptr, MOG_EINVAL, "parameters must be finite (world_size > 0)");
+  if (spawn_shape != 0 && spawn_shape != 1) return fail(nullptr, MOG_EINVAL, "spawn_shape must be 0 or 1");
+  hipError_t e = launch_init_perlin((float4*)pos, (float4*)vel, n, (float)world_size, (float)noise_scale,
+                                    (float)density_contrast, seed, (float)base_mass,
+                                    (float)spawn_buffer, spawn_shape, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(nullptr, MOG_EHIP, "mog_init_perlin: %s", hipGetErrorString(e));
+  return MOG_OK;
+}
+
 int mog_get_tuning(mog_ctx* ctx, mog_tuning* out) {
   if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
   if (!out) return fail(ctx, MOG_EINVAL, "out is NULL");

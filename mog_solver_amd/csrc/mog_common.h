@@ -119,6 +119,9 @@ hipError_t order_temp_bytes(long long n, size_t* bytes);
 hipError_t build_order(const float4* pos, long long n, float W, unsigned* keys_a,
                        unsigned* keys_b, int* idx_a, int* order, void* tmp, size_t tmp_bytes,
                        hipStream_t s);
+hipError_t launch_init_perlin(float4* pos, float4* vel, long long n, float W, float noise_scale,
+                              float contrast, unsigned seed, float base_mass, float spawn_buffer,
+                              int sphere, hipStream_t s);
 hipError_t launch_update(const Params& P, float4* pos, float4* vel, const float4* avg,
                          long long n, hipStream_t s);
 

@@ -40,6 +40,19 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def init_perlin(pos, vel, world_size, noise_scale=4.0, density_contrast=2.0, seed=12345,
+                base_mass=1.0, spawn_buffer=0.1, spawn_shape=1, stream=None):
+    """Perlin-fBm initial conditions into device tensors pos, vel (n,4)
+    (shaders/init_particles.glsl through mog_init_perlin)."""
+    _check_particles(pos, "pos")
+    _check_particles(vel, "vel")
+    lib = _abi.load()
+    rc = lib.mog_init_perlin(_ptr(pos), _ptr(vel), pos.shape[0], float(world_size), float(noise_scale),
+                             float(density_contrast), int(seed) & 0xFFFFFFFF, float(base_mass),
+                             float(spawn_buffer), int(spawn_shape), _stream_ptr(stream))
+    _abi.check(lib, None, rc)
+
+
 def _check_particles(t, name):
     torch = _torch()
     if not isinstance(t, torch.Tensor) or not t.is_cuda:
@@ -107,6 +120,14 @@ class MogSolver:
         self.rot = rot
         self.off = off
         self._check(self.lib.mog_set_transforms(self._ctx, rot.ctypes.data, off.ctypes.data))
+
+    def set_integrator(self, dt=None, damping=None, timescale=None):
+        """Change dt, damping or timescale between steps."""
+        c = self.cfg
+        c.dt = c.dt if dt is None else float(dt)
+        c.damping = c.damping if damping is None else float(damping)
+        c.timescale = c.timescale if timescale is None else float(timescale)
+        self._check(self.lib.mog_set_integrator(self._ctx, c.dt, c.damping, c.timescale))
 
     # -- tuning -------------------------------------------------------------
     def get_tuning(self) -> dict:
@@ -256,14 +277,57 @@ class Simulation:
         self.pos_buf, self.vel_buf = pos_t, vel_t
         self.num_particles = pos_t.shape[0]
 
+    # Presets of the reference's keys 1/2/3 (main.py:722-731).
+    PRESETS = {1: (2.0, 1.5), 2: (4.0, 2.5), 3: (8.0, 4.0)}
+
+    def _alloc_state(self):
+        torch = _torch()
+        dev = self.device or torch.device("cuda", torch.cuda.current_device())
+        n = self.num_particles
+        if self.pos_buf is None or self.pos_buf.shape[0] != n:
+            self.pos_buf = torch.empty((n, 4), dtype=torch.float32, device=dev)
+            self.vel_buf = torch.empty((n, 4), dtype=torch.float32, device=dev)
+
+    def init_particles_gpu(self, noise_scale=4.0, density_contrast=2.0, seed=None, spawn_buffer=None):
+        """Perlin-fBm initial conditions on the device (main.py:283-310)."""
+        if seed is None:
+            seed = int(np.random.randint(0, 2**31))
+        if spawn_buffer is None:
+            spawn_buffer = self.cfg.get("spawn_buffer", 0.1)
+        self._alloc_state()
+        init_perlin(self.pos_buf, self.vel_buf, self.world_size, noise_scale, density_contrast, seed,
+                    float(self.cfg.get("particle_mass", 1.0)), float(spawn_buffer), 1)
+
+    def init_particles_default(self):
+        """The reference's start-up and key R state (main.py:280-281)."""
+        self.init_particles_gpu(noise_scale=4.0, density_contrast=2.0, seed=12345)
+
+    def preset(self, number, seed=None):
+        """Keys 1/2/3: clustering presets."""
+        ns, dc = self.PRESETS[int(number)]
+        self.init_particles_gpu(noise_scale=ns, density_contrast=dc, seed=seed)
+
+    def init_particles2(self, seed=None):
+        """Key T: two galaxies (main.py:312-357)."""
+        from .workloads import two_galaxies_reference
+
+        pos, vel = two_galaxies_reference(self.num_particles, self.world_size,
+                                          float(self.cfg.get("particle_mass", 1.0)), seed)
+        self.init_particles(pos, vel)
+
+    def set_timescale(self, timescale):
+        """deltaTime = dt * timescale (keys [ ] \\ of the reference)."""
+        self.timescale = float(timescale)
+        self.solver.set_integrator(timescale=self.timescale)
+
     def clear_mass(self):
         """Kept for interface parity: the step clears its grids itself."""
 
     def step_fft(self):
         if self.pos_buf is None:
             raise RuntimeError("init_particles() first")
-        if self.solver.cfg.timescale != self.timescale:
-            raise RuntimeError("timescale changed: create a new Simulation or set_timescale()")
+        if self.solver.cfg.timescale != self.timescale:  # attribute assigned directly
+            self.solver.set_integrator(timescale=self.timescale)
         self.solver.step(self.pos_buf, self.vel_buf)
 
     def step(self):

@@ -12,6 +12,9 @@ periodic cube ``[0, W)``.
   the MOG coupling ``G_eff = G * h^3 / 2`` (density is mass per cell and the
   Poisson factor is -2*pi*G, greens.glsl:40, complex.glsl:23).
 * ``plummer_spheres`` -- Plummer spheres at uniform-random centres, v = 0.
+* ``two_galaxies_reference`` -- the reference's key-T state, seeded.
+The Perlin-fBm sampler of the reference's key R/1/2/3 runs on the device
+(``solver.init_perlin``).
 """
 from __future__ import annotations
 
@@ -95,3 +98,46 @@ def plummer_spheres(n=4_000_000, n_spheres=64, scale=1.5, world_size=100.0, mass
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     xyz = centres[which] + r[:, None] * d
     return _pack(xyz, np.zeros((n, 3)), mass, world_size)
+
+
+def _spinning_blob(rs, count, world_size, centre, bulk, spin_axis):
+    """One blob of ``two_galaxies_reference``: radius ~ Exp(0.08 W), isotropic
+    direction (uniform azimuth, uniform cos polar), tangential speed
+    0.3 sqrt(r/W + 0.01) about ``spin_axis`` plus a bulk velocity."""
+    radius = world_size * rs.exponential(scale=0.08, size=count)
+    azimuth = rs.uniform(0, 2 * np.pi, count)
+    polar = np.arccos(rs.uniform(-1, 1, count))
+    offset = radius[:, None] * np.column_stack(
+        (np.sin(polar) * np.cos(azimuth), np.sin(polar) * np.sin(azimuth), np.cos(polar)))
+    axis = np.asarray(spin_axis, dtype=np.float64)
+    axis /= np.linalg.norm(axis)
+    tangent = np.cross(axis, offset)
+    tangent /= np.linalg.norm(tangent, axis=1, keepdims=True) + 1e-10
+    speed = 0.3 * np.sqrt(radius / world_size + 0.01)
+    return centre + offset, speed[:, None] * tangent + np.asarray(bulk)
+
+
+def two_galaxies_reference(n, world_size, particle_mass=1.0, seed=None):
+    """The state of the reference's key T (``Simulation.init_particles2``,
+    main.py:312-357): two spinning blobs of n//2 particles at W/2 -+ 0.2 W
+    along x with bulk velocities +-(0.3, 0.1, 0) and spin axes along
+    (0.3, 0.7, 0.5) and (-0.5, 0.2, 0.8).  The draws follow the reference's
+    legacy-RNG order (per blob: radii, azimuths, cos polar) from
+    ``RandomState(seed)``; the reference draws from the unseeded global
+    state.  Positions are not wrapped into the box (as in the reference)."""
+    rs = np.random.RandomState(seed)
+    half = n // 2
+    mid = world_size / 2.0
+    blobs = [
+        _spinning_blob(rs, half, world_size, np.array([mid - 0.2 * world_size, mid, mid]),
+                       (0.3, 0.1, 0.0), (0.3, 0.7, 0.5)),
+        _spinning_blob(rs, half, world_size, np.array([mid + 0.2 * world_size, mid, mid]),
+                       (-0.3, -0.1, 0.0), (-0.5, 0.2, 0.8)),
+    ]
+    pos = np.zeros((n, 4), dtype=np.float32)
+    vel = np.zeros((n, 4), dtype=np.float32)
+    for i, (xyz, v) in enumerate(blobs):
+        pos[i * half:(i + 1) * half, :3] = xyz
+        pos[i * half:(i + 1) * half, 3] = particle_mass
+        vel[i * half:(i + 1) * half, :3] = v
+    return pos, vel
