@@ -150,6 +150,44 @@ int mog_init_perlin(float* pos_n4, float* vel_n4, int64_t n, double world_size,
                     double noise_scale, double density_contrast, uint32_t seed,
                     double base_mass, double spawn_buffer, int32_t spawn_shape, void* stream);
 
+/* ---- Learned closure (no counterpart in the reference) ----------------
+ * Features of the ensemble of `ctx` (typically few grids, e.g. k = 24) at
+ * the particles: runs the context's clear, deposit and spectral solve, then
+ * writes feat (n,8) = (m/S, s/S, log(1 + rho/rho0), S) and mean (n,4) =
+ * (m, 0), with m the mean and s the across-grid standard deviation of the
+ * world-frame grid forces, rho the mean mass of the particle's cells and
+ * S = |m| + |s| a per-particle force scale. */
+int mog_closure_features(mog_ctx* ctx, const float* pos_n4, int64_t n, double rho0,
+                         float* feat_n8, float* mean_n4, void* stream);
+
+/* Number of fp32 parameters of the closure MLP (7 -> 64 -> 64 -> 64 -> 3,
+ * SiLU; W1 b1 W2 b2 W3 b3 W4 b4 with W[in][out] row-major). */
+int mog_mlp_param_count(void);
+
+/* force (n,4) = mean + S * MLP(x) on the matrix cores (bf16 != 0: BF16
+ * operands, else FP32; FP32 accumulation). */
+int mog_mlp_forward(const float* params, const float* feat_n8, const float* mean_n4, int64_t n,
+                    float* force_n4, int32_t bf16, void* stream);
+
+/* Fused forward + backward over n particles: adds dLoss/dparams to grad
+ * and the loss to *loss (device pointers; the caller clears them), with
+ * loss = wnorm * sum_p S_p^2 |MLP(x_p) - (target_p - m_p)/S_p|^2. */
+int mog_mlp_train_grad(const float* params, const float* feat_n8, const float* mean_n4,
+                       const float* target_n4, int64_t n, double wnorm, float* grad, float* loss,
+                       int32_t bf16, void* stream);
+
+/* FP64 direct summation audit: at the particles targets[0..nt) of pos,
+ * grad_out (nt,3) = G_eff * sum_j m_j d / (|d|^2 + softening^2)^1.5 with d the
+ * minimum-image separation in the periodic box of side world_size (the
+ * gradient the MOG avgForce approximates, G_eff = G h^3 / 2). */
+int mog_direct_sum(const float* pos_n4, int64_t n, const int32_t* targets, int32_t nt,
+                   double world_size, double G_eff, double softening, double* grad_out,
+                   void* stream);
+
+/* One bias-corrected Adam update of the parameters (step >= 1). */
+int mog_mlp_adam(float* params, const float* grad, float* m1, float* m2, double lr, double beta1,
+                 double beta2, double eps, int32_t step, void* stream);
+
 /* Current tuning of `ctx` (host struct, synchronous). */
 int mog_get_tuning(mog_ctx* ctx, mog_tuning* out);
 

@@ -38,6 +38,12 @@ EXPORTED = (
     "mog_step_profiled",
     "mog_set_integrator",
     "mog_init_perlin",
+    "mog_closure_features",
+    "mog_mlp_param_count",
+    "mog_mlp_forward",
+    "mog_mlp_train_grad",
+    "mog_mlp_adam",
+    "mog_direct_sum",
     "mog_get_tuning",
     "mog_set_tuning",
     "mog_grids_ptr",
@@ -138,6 +144,19 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     lib.mog_init_perlin.restype = ctypes.c_int
     lib.mog_init_perlin.argtypes = [vp, vp, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_uint32, ctypes.c_double, ctypes.c_double, i32, vp]
+    d = ctypes.c_double
+    lib.mog_closure_features.restype = ctypes.c_int
+    lib.mog_closure_features.argtypes = [vp, vp, i64, d, vp, vp, vp]
+    lib.mog_mlp_param_count.restype = ctypes.c_int
+    lib.mog_mlp_param_count.argtypes = []
+    lib.mog_mlp_forward.restype = ctypes.c_int
+    lib.mog_mlp_forward.argtypes = [vp, vp, vp, i64, vp, i32, vp]
+    lib.mog_mlp_train_grad.restype = ctypes.c_int
+    lib.mog_mlp_train_grad.argtypes = [vp, vp, vp, vp, i64, d, vp, vp, i32, vp]
+    lib.mog_mlp_adam.restype = ctypes.c_int
+    lib.mog_mlp_adam.argtypes = [vp, vp, vp, vp, d, d, d, d, i32, vp]
+    lib.mog_direct_sum.restype = ctypes.c_int
+    lib.mog_direct_sum.argtypes = [vp, i64, vp, i32, d, d, d, vp, vp]
     lib.mog_get_tuning.restype = ctypes.c_int
     lib.mog_get_tuning.argtypes = [vp, ctypes.POINTER(MogTuning)]
     lib.mog_set_tuning.restype = ctypes.c_int

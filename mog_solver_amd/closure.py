@@ -1,0 +1,269 @@
+"""Learned closure: a cheap k-grid ensemble plus a per-particle MLP that
+predicts the force of the full ensemble, with the full ensemble as fallback
+and judge.
+
+The reference always averages all k grids (update.glsl:67-81); this module
+is this build's extension.  Pieces:
+
+* ``ClosureMLP`` -- parameters (fp32 torch tensor) and Adam state; forward
+  pass, fused forward+backward gradient and the Adam update run in
+  ``libmog.so`` (``mog_mlp_forward``, ``mog_mlp_train_grad``,
+  ``mog_mlp_adam``), on the matrix cores in BF16 (or FP32) with FP32
+  accumulation.
+* ``make_snapshots`` -- runs the full-k solver and records, every few steps,
+  the closure features of the cheap ensemble (``mog_closure_features``) and
+  the full-k force (the training target).
+* ``train`` -- epochs of Adam over the snapshots.
+* ``ClosureSolver`` -- steps with the cheap ensemble + MLP; every
+  ``audit_every`` steps it also computes the full-k force and, when the
+  relative RMS error of the closure exceeds ``threshold``, uses the full-k
+  force for the next ``fallback_steps`` steps.
+* ``direct_sum`` -- FP64 Newtonian reference at a particle subsample.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .solver import MogSolver, _check_particles, _ptr, _stream_ptr
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+K_IN, K_H, K_OUT = 7, 64, 3
+LAYOUT = [("W1", (K_IN, K_H)), ("b1", (K_H,)), ("W2", (K_H, K_H)), ("b2", (K_H,)),
+          ("W3", (K_H, K_H)), ("b3", (K_H,)), ("W4", (K_H, K_OUT)), ("b4", (K_OUT,))]
+
+
+def rel_rms(a, b):
+    """sqrt(sum |a-b|^2 / sum |b|^2) over the first three components (torch)."""
+    d = (a[:, :3] - b[:, :3]).double()
+    return float(math.sqrt(float((d * d).sum()) / max(float((b[:, :3].double() ** 2).sum()), 1e-300)))
+
+
+class ClosureMLP:
+    """The closure network: 7 -> 64 -> 64 -> 64 -> 3, SiLU."""
+
+    def __init__(self, params=None, seed=0, bf16=True, device=None):
+        torch = _torch()
+        self.lib = _abi.load()
+        self.n_params = int(self.lib.mog_mlp_param_count())
+        self.bf16 = bool(bf16)
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        if params is None:
+            params = self.initial_params(seed)
+        self.params = torch.as_tensor(np.asarray(params, dtype=np.float32)).to(dev).contiguous()
+        if self.params.numel() != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters")
+        self.m1 = torch.zeros_like(self.params)
+        self.m2 = torch.zeros_like(self.params)
+        self.grad = torch.zeros_like(self.params)
+        self.loss_buf = torch.zeros(1, dtype=torch.float32, device=dev)
+        self.step_count = 0
+
+    @staticmethod
+    def initial_params(seed=0, out_scale=0.1):
+        """He-normal weights, zero biases; the output layer starts small so the
+        untrained closure predicts close to the cheap ensemble's mean."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for name, shape in LAYOUT:
+            if name.startswith("W"):
+                w = rng.standard_normal(shape) * math.sqrt(2.0 / shape[0])
+                parts.append((w * (out_scale if name == "W4" else 1.0)).ravel())
+            else:
+                parts.append(np.zeros(int(np.prod(shape))))
+        return np.concatenate(parts).astype(np.float32)
+
+    def _check(self, rc):
+        _abi.check(self.lib, None, rc)
+
+    def forward(self, feat, mean, out=None, stream=None):
+        """Predicted force (n,4) = mean + S * MLP(x)."""
+        torch = _torch()
+        n = feat.shape[0]
+        if out is None:
+            out = torch.empty((n, 4), dtype=torch.float32, device=feat.device)
+        self._check(self.lib.mog_mlp_forward(_ptr(self.params), _ptr(feat), _ptr(mean), n, _ptr(out),
+                                             int(self.bf16), _stream_ptr(stream)))
+        return out
+
+    @staticmethod
+    def wnorm(feat):
+        s = feat[:, 7].double()
+        return 1.0 / max(float((s * s).sum()), 1e-300)
+
+    def loss_and_grad(self, feat, mean, target, wnorm=None, stream=None):
+        """(loss, grad) of the fused training pass (grad is a fresh tensor)."""
+        self.grad.zero_()
+        self.loss_buf.zero_()
+        w = self.wnorm(feat) if wnorm is None else wnorm
+        self._check(self.lib.mog_mlp_train_grad(_ptr(self.params), _ptr(feat), _ptr(mean), _ptr(target),
+                                                feat.shape[0], float(w), _ptr(self.grad), _ptr(self.loss_buf),
+                                                int(self.bf16), _stream_ptr(stream)))
+        return float(self.loss_buf.item()), self.grad.clone()
+
+    def train_step(self, feat, mean, target, lr=1e-3, wnorm=None, stream=None):
+        self.grad.zero_()
+        self.loss_buf.zero_()
+        w = self.wnorm(feat) if wnorm is None else wnorm
+        self._check(self.lib.mog_mlp_train_grad(_ptr(self.params), _ptr(feat), _ptr(mean), _ptr(target),
+                                                feat.shape[0], float(w), _ptr(self.grad), _ptr(self.loss_buf),
+                                                int(self.bf16), _stream_ptr(stream)))
+        self.step_count += 1
+        self._check(self.lib.mog_mlp_adam(_ptr(self.params), _ptr(self.grad), _ptr(self.m1), _ptr(self.m2),
+                                          float(lr), 0.9, 0.999, 1e-8, self.step_count, _stream_ptr(stream)))
+        return self.loss_buf  # device scalar; read it when needed
+
+
+def closure_features(solver_small: MogSolver, pos, rho0, feat=None, mean=None, stream=None):
+    """(feat (n,8), mean (n,4)) of the cheap ensemble at ``pos`` (written into
+    ``feat``/``mean`` when given)."""
+    torch = _torch()
+    _check_particles(pos, "pos")
+    n = pos.shape[0]
+    if feat is None:
+        feat = torch.empty((n, 8), dtype=torch.float32, device=pos.device)
+    if mean is None:
+        mean = torch.empty((n, 4), dtype=torch.float32, device=pos.device)
+    if tuple(feat.shape) != (n, 8) or tuple(mean.shape) != (n, 4) or not (feat.is_contiguous() and mean.is_contiguous()):
+        raise ValueError("feat must be (n,8) and mean (n,4) contiguous float32")
+    solver_small._check(solver_small.lib.mog_closure_features(solver_small._ctx, _ptr(pos), n, float(rho0),
+                                                              _ptr(feat), _ptr(mean), _stream_ptr(stream)))
+    return feat, mean
+
+
+def direct_sum(pos, targets, world_size, G_eff, softening, stream=None):
+    """FP64 softened Newtonian gradient at particle indices ``targets``."""
+    torch = _torch()
+    _check_particles(pos, "pos")
+    t = torch.as_tensor(np.asarray(targets, dtype=np.int32)).to(pos.device).contiguous()
+    out = torch.empty((t.numel(), 3), dtype=torch.float64, device=pos.device)
+    lib = _abi.load()
+    _abi.check(lib, None, lib.mog_direct_sum(_ptr(pos), pos.shape[0], _ptr(t), t.numel(), float(world_size),
+                                             float(G_eff), float(softening), _ptr(out), _stream_ptr(stream)))
+    return out
+
+
+def default_rho0(n, mass, grid_size):
+    """Mean mass per cell."""
+    return max(float(n) * float(mass) / float(grid_size) ** 3, 1e-30)
+
+
+@dataclass
+class Snapshots:
+    feat: list = field(default_factory=list)
+    mean: list = field(default_factory=list)
+    target: list = field(default_factory=list)
+
+    def __len__(self):
+        return len(self.feat)
+
+
+def make_snapshots(full: MogSolver, small: MogSolver, pos, vel, count, every=1, rho0=None):
+    """Advance the full-k simulation and record ``count`` snapshots of
+    (cheap-ensemble features, full-k force).  pos/vel are advanced in place."""
+    rho0 = default_rho0(pos.shape[0], 1.0, full.cfg.grid_size) if rho0 is None else rho0
+    snaps = Snapshots()
+    for i in range(count):
+        if i:
+            full.run(pos, vel, every, use_graph=False)
+        f, m = closure_features(small, pos, rho0)
+        snaps.feat.append(f)
+        snaps.mean.append(m)
+        snaps.target.append(full.forces(pos))
+    return snaps
+
+
+def train(mlp: ClosureMLP, snaps: Snapshots, epochs=20, lr=2e-3, lr_final=2e-4, verbose=False):
+    """Adam over the snapshots (one step per snapshot per epoch) with a
+    geometric learning-rate decay; returns the per-epoch mean loss."""
+    torch = _torch()
+    history = []
+    total = max(1, epochs * len(snaps))
+    step = 0
+    for ep in range(epochs):
+        acc = torch.zeros(1, dtype=torch.float32, device=mlp.params.device)
+        for j in np.random.default_rng(ep).permutation(len(snaps)):
+            rate = lr * (lr_final / lr) ** (step / total)
+            acc += mlp.train_step(snaps.feat[j], snaps.mean[j], snaps.target[j], lr=rate)
+            step += 1
+        history.append(float(acc.item()) / len(snaps))
+        if verbose:
+            print(f"[closure] epoch {ep}: loss {history[-1]:.4e}", flush=True)
+    return history
+
+
+def evaluate(mlp: ClosureMLP, snaps: Snapshots):
+    """Mean relative RMS error vs the full-k force of the cheap ensemble's mean
+    and of the closure."""
+    plain, closure = [], []
+    for f, m, t in zip(snaps.feat, snaps.mean, snaps.target):
+        plain.append(rel_rms(m, t))
+        closure.append(rel_rms(mlp.forward(f, m), t))
+    return float(np.mean(plain)), float(np.mean(closure))
+
+
+class ClosureSolver:
+    """Steps with the cheap ensemble + MLP, audited against the full ensemble.
+
+    Every ``audit_every`` closure steps the full-k force is computed at the
+    same positions; if the closure's relative RMS error exceeds ``threshold``
+    the next ``fallback_steps`` steps use the full-k force.  ``audits`` keeps
+    (step, error, fallback) records.
+    """
+
+    def __init__(self, full: MogSolver, small: MogSolver, mlp: ClosureMLP, rho0, audit_every=10,
+                 threshold=0.05, fallback_steps=10):
+        a, b = full.cfg, small.cfg
+        if (a.grid_size, a.world_size, a.G, a.dt, a.damping, a.timescale, a.compat) != (
+                b.grid_size, b.world_size, b.G, b.dt, b.damping, b.timescale, b.compat):
+            raise ValueError("full and small ensembles must share grid size, box, G, integrator and compat")
+        self.full, self.small, self.mlp = full, small, mlp
+        self.rho0 = rho0
+        self.audit_every = audit_every
+        self.threshold = threshold
+        self.fallback_steps = fallback_steps
+        self.fallback_left = 0
+        self.steps = 0
+        self.audits = []
+        self._buf = None
+
+    def _buffers(self, pos):
+        torch = _torch()
+        n = pos.shape[0]
+        if self._buf is None or self._buf[0].shape[0] != n:
+            self._buf = (torch.empty((n, 8), dtype=torch.float32, device=pos.device),
+                         torch.empty((n, 4), dtype=torch.float32, device=pos.device),
+                         torch.empty((n, 4), dtype=torch.float32, device=pos.device))
+        return self._buf
+
+    def step(self, pos, vel):
+        if self.fallback_left > 0:
+            self.full.step(pos, vel)
+            self.fallback_left -= 1
+            self.steps += 1
+            return "full"
+        feat, mean, pred = self._buffers(pos)
+        f, m = closure_features(self.small, pos, self.rho0, feat=feat, mean=mean)
+        pred = self.mlp.forward(f, m, out=pred)
+        mode = "closure"
+        if self.audit_every and self.steps % self.audit_every == 0:
+            ref = self.full.forces(pos)
+            err = rel_rms(pred, ref)
+            fall = err > self.threshold
+            self.audits.append((self.steps, err, fall))
+            if fall:
+                self.fallback_left = self.fallback_steps
+                pred = ref
+                mode = "full"
+        self.small.update(pos, vel, pred)
+        self.steps += 1
+        return mode

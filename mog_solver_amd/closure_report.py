@@ -1,0 +1,161 @@
+"""Train and evaluate the learned closure on the two-colliding-disks workload.
+
+``python -m mog_solver_amd.closure_report [--n 1000000] [--snapshots 200] ...``
+
+1. Advance the k_full-grid simulation (n particles, two exponential disks)
+   and record ``--snapshots`` (features of the k_small ensemble, k_full
+   force) pairs, one every ``--every`` steps.
+2. Train the closure MLP on the GPU (fused BF16 training kernel + Adam).
+3. From the final state run ``--eval-steps`` closure steps.  At every step
+   the closure force is compared with the k_full force at the same positions
+   (relative RMS); every ``--direct-every`` steps the k_full, k_small and
+   closure forces are compared with FP64 direct summation (softening one
+   cell) at a ``--subsample`` of particles.
+4. Time a plain k_full step (hipGraph replay) and a closure step (features,
+   MLP, update), both with inputs resident on the device.
+
+Prints one JSON object.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+
+def _rel(a, b):
+    d = a - b
+    return float(np.linalg.norm(d) / max(np.linalg.norm(b), 1e-300))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--k-full", type=int, default=240)
+    ap.add_argument("--k-small", type=int, default=24)
+    ap.add_argument("--g", type=int, default=16)
+    ap.add_argument("--snapshots", type=int, default=200)
+    ap.add_argument("--every", type=int, default=5)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr-final", type=float, default=1e-4)
+    ap.add_argument("--eval-steps", type=int, default=100)
+    ap.add_argument("--direct-every", type=int, default=10)
+    ap.add_argument("--subsample", type=int, default=4096)
+    ap.add_argument("--time-steps", type=int, default=100)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+
+    import torch
+
+    from . import closure as CL
+    from . import workloads
+    from .config import SolverConfig
+    from .solver import MogSolver
+    from .transforms import compute_transforms
+
+    if not torch.cuda.is_available():
+        sys.exit("closure_report needs a HIP device")
+    W, G, dt, damping = 100.0, 100.0, 1e-5, 0.999
+
+    def make(k, seed):
+        cfg = SolverConfig.from_dict(dict(n_grids=k, grid_size=args.g, world_size=W, G=G, dt=dt, damping=damping,
+                                          solver="fft"))
+        off, rot = compute_transforms(k, args.g, W, seed=seed)
+        return MogSolver(cfg, rot=rot, off=off)
+
+    def log(msg):
+        print(f"[closure_report] {msg}", file=sys.stderr, flush=True)
+
+    n = args.n
+    full = make(args.k_full, args.seed + 1)
+    small = make(args.k_small, args.seed + 2)
+    pos, vel = workloads.two_disks(n=n, grid_size=args.g)
+    p = torch.from_numpy(pos).cuda()
+    v = torch.from_numpy(vel).cuda()
+    rho0 = CL.default_rho0(n, 1.0, args.g)
+    out = dict(workload=f"two_disks n={n}", k_full=args.k_full, k_small=args.k_small, g=args.g,
+               snapshots=args.snapshots, every=args.every, epochs=args.epochs)
+
+    t0 = time.perf_counter()
+    snaps = CL.make_snapshots(full, small, p, v, args.snapshots, every=args.every, rho0=rho0)
+    torch.cuda.synchronize()
+    out["snapshot_s"] = time.perf_counter() - t0
+    log(f"{len(snaps)} snapshots in {out['snapshot_s']:.1f}s")
+
+    mlp = CL.ClosureMLP(seed=args.seed, bf16=True)
+    t0 = time.perf_counter()
+    hist = CL.train(mlp, snaps, epochs=args.epochs, lr=args.lr, lr_final=args.lr_final)
+    torch.cuda.synchronize()
+    out["train_s"] = time.perf_counter() - t0
+    out["train_steps"] = args.epochs * len(snaps)
+    out["loss_first_epoch"], out["loss_last_epoch"] = hist[0], hist[-1]
+    plain_tr, closure_tr = CL.evaluate(mlp, snaps)
+    out["train_rel_rms"] = dict(k_small=plain_tr, closure=closure_tr)
+    log(f"trained in {out['train_s']:.1f}s: loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    del snaps
+    torch.cuda.empty_cache()
+
+    # evaluation on the steps that follow the training window
+    cs = CL.ClosureSolver(full, small, mlp, rho0, audit_every=0)
+    G_eff = workloads.effective_G(G, W, args.g)
+    rng = np.random.default_rng(args.seed)
+    sub = np.sort(rng.choice(n, size=min(args.subsample, n), replace=False)).astype(np.int32)
+    sub_t = torch.from_numpy(sub.astype(np.int64)).cuda()
+    vs_full, vs_full_plain = [], []
+    direct = {"k_full": [], "k_small": [], "closure": []}
+    for step in range(args.eval_steps):
+        feat, mean, pred = cs._buffers(p)
+        CL.closure_features(small, p, rho0, feat=feat, mean=mean)
+        mlp.forward(feat, mean, out=pred)
+        ref = full.forces(p)
+        vs_full.append(CL.rel_rms(pred, ref))
+        vs_full_plain.append(CL.rel_rms(mean, ref))
+        if args.direct_every and step % args.direct_every == 0:
+            nd = CL.direct_sum(p, sub, W, G_eff, W / args.g).cpu().numpy()
+            for name, t in (("k_full", ref), ("k_small", mean), ("closure", pred)):
+                direct[name].append(_rel(t[sub_t, :3].double().cpu().numpy(), nd))
+        small.update(p, v, pred)
+    out["eval_steps"] = args.eval_steps
+    out["closure_vs_k_full_rel_rms"] = dict(mean=float(np.mean(vs_full)), max=float(np.max(vs_full)))
+    out["k_small_vs_k_full_rel_rms"] = dict(mean=float(np.mean(vs_full_plain)), max=float(np.max(vs_full_plain)))
+    out["vs_fp64_direct_sum_rel_rms"] = {k2: float(np.mean(v2)) for k2, v2 in direct.items()}
+    out["direct_sum"] = dict(subsample=int(sub.size), softening=W / args.g, G_eff=G_eff)
+    log(f"eval: closure vs k_full {out['closure_vs_k_full_rel_rms']}, k_small {out['k_small_vs_k_full_rel_rms']}")
+
+    # timing, inputs resident on the device
+    def clock(fn, steps):
+        fn(3)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn(steps)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    K = args.time_steps
+    t_full = clock(lambda s: full.run(p, v, s, use_graph=True), K)
+    runner = CL.ClosureSolver(full, small, mlp, rho0, audit_every=0)
+
+    def closure_steps(s):
+        for _ in range(s):
+            runner.step(p, v)
+
+    t_closure = clock(closure_steps, K)
+    audited = CL.ClosureSolver(full, small, mlp, rho0, audit_every=10, threshold=math.inf)
+
+    def audited_steps(s):
+        for _ in range(s):
+            audited.step(p, v)
+
+    t_audit = clock(audited_steps, K)
+    out["ms_per_step"] = dict(k_full=t_full * 1e3, closure=t_closure * 1e3, closure_audit_every_10=t_audit * 1e3)
+    out["speedup_vs_k_full"] = dict(closure=t_full / t_closure, closure_audit_every_10=t_full / t_audit)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

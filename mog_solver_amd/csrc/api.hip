@@ -704,6 +704,84 @@ int mog_init_perlin(float* pos, float* vel, int64_t n, double world_size, double
   return MOG_OK;
 }
 
+int mog_closure_features(mog_ctx* ctx, const float* pos, int64_t n, double rho0, float* feat,
+                         float* mean, void* stream) {
+  int rc = check_particles(ctx, pos, nullptr, n, false);
+  if (rc != MOG_OK) return rc;
+  if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
+  if (n == 0) return MOG_OK;
+  if (!feat || !mean || !aligned16(feat) || !aligned16(mean))
+    return fail(ctx, MOG_EINVAL, "feat/mean must be 16-byte aligned device pointers");
+  if (!(isfinite(rho0) && rho0 > 0.0)) return fail(ctx, MOG_EINVAL, "rho0 must be finite and > 0");
+  hipStream_t s = (hipStream_t)stream;
+  const Params& P = ctx->P;
+  hipError_t e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
+  if (e == hipSuccess)
+    e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
+                       (unsigned long long*)ctx->d_grids, ctx->tune, s);
+  if (e == hipSuccess) e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
+  const int* order = nullptr;
+  if (e == hipSuccess) e = refresh_order(ctx, pos, n, s, &order);
+  if (e == hipSuccess)
+    e = launch_closure_features(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
+                                ctx->d_force, ctx->d_grids, (float)rho0, order, (float4*)feat,
+                                (float4*)mean, s);
+  return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_closure_features");
+}
+
+int mog_mlp_param_count(void) { return (int)mlp_param_count(); }
+
+int mog_mlp_forward(const float* params, const float* feat, const float* mean, int64_t n,
+                    float* force, int32_t bf16, void* stream) {
+  if (n < 0 || n > (int64_t)2147483647) return fail(nullptr, MOG_EINVAL, "n must be in [0, 2^31)");
+  if (n == 0) return MOG_OK;
+  if (!params || !feat || !mean || !force) return fail(nullptr, MOG_EINVAL, "NULL pointer");
+  if (!aligned16(feat) || !aligned16(mean) || !aligned16(force))
+    return fail(nullptr, MOG_EINVAL, "feat/mean/force must be 16-byte aligned");
+  hipError_t e = launch_mlp_forward(params, (const float4*)feat, (const float4*)mean, n,
+                                    (float4*)force, bf16 != 0, (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : fail(nullptr, MOG_EHIP, "mog_mlp_forward: %s", hipGetErrorString(e));
+}
+
+int mog_mlp_train_grad(const float* params, const float* feat, const float* mean,
+                       const float* target, int64_t n, double wnorm, float* grad, float* loss,
+                       int32_t bf16, void* stream) {
+  if (n < 0 || n > (int64_t)2147483647) return fail(nullptr, MOG_EINVAL, "n must be in [0, 2^31)");
+  if (n == 0) return MOG_OK;
+  if (!params || !feat || !mean || !target || !grad || !loss)
+    return fail(nullptr, MOG_EINVAL, "NULL pointer");
+  if (!aligned16(feat) || !aligned16(mean) || !aligned16(target))
+    return fail(nullptr, MOG_EINVAL, "feat/mean/target must be 16-byte aligned");
+  if (!(isfinite(wnorm) && wnorm > 0.0)) return fail(nullptr, MOG_EINVAL, "wnorm must be > 0");
+  hipError_t e = launch_mlp_train(params, (const float4*)feat, (const float4*)mean,
+                                  (const float4*)target, n, (float)wnorm, grad, loss, bf16 != 0,
+                                  (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : fail(nullptr, MOG_EHIP, "mog_mlp_train_grad: %s", hipGetErrorString(e));
+}
+
+int mog_direct_sum(const float* pos, int64_t n, const int32_t* targets, int32_t nt,
+                   double world_size, double G_eff, double softening, double* out, void* stream) {
+  if (n < 0 || n > (int64_t)2147483647) return fail(nullptr, MOG_EINVAL, "n must be in [0, 2^31)");
+  if (nt < 0) return fail(nullptr, MOG_EINVAL, "nt must be >= 0");
+  if (nt == 0) return MOG_OK;
+  if (!pos || !targets || !out) return fail(nullptr, MOG_EINVAL, "NULL pointer");
+  if (!aligned16(pos)) return fail(nullptr, MOG_EINVAL, "pos must be 16-byte aligned");
+  if (!(isfinite(world_size) && world_size > 0.0) || !isfinite(G_eff) || !isfinite(softening))
+    return fail(nullptr, MOG_EINVAL, "world_size > 0, G_eff and softening finite");
+  hipError_t e = launch_direct_sum((const float4*)pos, n, targets, nt, world_size, G_eff,
+                                   softening * softening, out, (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : fail(nullptr, MOG_EHIP, "mog_direct_sum: %s", hipGetErrorString(e));
+}
+
+int mog_mlp_adam(float* params, const float* grad, float* m1, float* m2, double lr, double beta1,
+                 double beta2, double eps, int32_t step, void* stream) {
+  if (!params || !grad || !m1 || !m2) return fail(nullptr, MOG_EINVAL, "NULL pointer");
+  if (step < 1) return fail(nullptr, MOG_EINVAL, "step must be >= 1");
+  hipError_t e = launch_adam(params, grad, m1, m2, (float)lr, (float)beta1, (float)beta2, (float)eps,
+                             step, (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : fail(nullptr, MOG_EHIP, "mog_mlp_adam: %s", hipGetErrorString(e));
+}
+
 int mog_get_tuning(mog_ctx* ctx, mog_tuning* out) {
   if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
   if (!out) return fail(ctx, MOG_EINVAL, "out is NULL");

@@ -1,0 +1,603 @@
+// Learned closure of the MOG step: features of a cheap k-grid ensemble and a
+// per-particle MLP that predicts the force of a larger ensemble.
+//
+// Features (per particle, from a context with few grids, e.g. k = 24):
+//   m   = mean over grids of the world-frame force M_i F_i(cell)      (3)
+//   s   = across-grid standard deviation of those forces, per axis     (3)
+//   rho = mean over grids of the mass of the particle's cell           (1)
+// Network input x = (m / S, s / S, log(1 + rho / rho0)), S = |m| + |s| + tiny
+// (a per-particle force scale, so the network is scale-free), output y (3);
+// predicted force F = m + S y.  Layers 7 -> 64 -> 64 -> 64 -> 3 with SiLU
+// activations; parameters fp32, row-major W[in][out], each followed by its
+// bias: W1 b1 W2 b2 W3 b3 W4 b4 (9027 floats).
+//
+// GEMMs run on the matrix cores over tiles of 64 particles held in LDS:
+// BF16 operands (v_mfma_f32_16x16x32_bf16) or FP32 operands
+// (v_mfma_f32_16x16x4_f32), FP32 accumulation either way.  The training
+// kernel is fused: forward (pre-activations kept in LDS), loss
+// sum_p w_p |y_p - t_p|^2 with t = (F_target - m) / S and w = S^2 / mean S^2,
+// backward, and gradient accumulation in LDS, flushed once per workgroup.
+#include "mog_common.h"
+
+namespace mog {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr int kIn = 7, kH = 64, kOut = 3;
+constexpr int oW1 = 0, ob1 = oW1 + kIn * kH, oW2 = ob1 + kH, ob2 = oW2 + kH * kH,
+              oW3 = ob2 + kH, ob3 = oW3 + kH * kH, oW4 = ob3 + kH, ob4 = oW4 + kH * kOut,
+              kParams = ob4 + kOut;
+constexpr int TILE = 64;  // particles per tile
+
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = sigm(z);
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// A matrix view: element (r, c) at p[r * rs + c * cs]; reads outside
+// (nr, nc) are zero; act applies SiLU on read.
+struct Mat {
+  const float* p;
+  int rs, cs, nr, nc;
+  bool act;
+  __device__ float at(int r, int c) const {
+    if (r >= nr || c >= nc) return 0.0f;
+    const float v = p[r * rs + c * cs];
+    return act ? silu(v) : v;
+  }
+};
+
+// C[M x N] (+)= A[M x K] . B[K x N] by the workgroup's waves; M, N are
+// multiples of 16 (padding reads zero), output written only inside
+// (cr, cc).  BF16: K handled in chunks of 32; FP32: chunks of 4.
+template <bool BF16>
+__device__ void wg_gemm(const Mat A, const Mat B, float* C, int ldc, int M, int N, int K, int cr,
+                        int cc, bool accumulate) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int tn = N / 16, nt = (M / 16) * tn;
+  for (int t = wave; t < nt; t += nw) {
+    const int r0 = (t / tn) * 16, c0 = (t % tn) * 16;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    const int ar = r0 + (lane & 15), bc = c0 + (lane & 15);
+    if (BF16) {
+      for (int k0 = 0; k0 < K; k0 += 32) {
+        v8bf a, b;
+        const int kb = k0 + 8 * (lane >> 4);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          a[j] = (__bf16)A.at(ar, kb + j);
+          b[j] = (__bf16)B.at(kb + j, bc);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+      }
+    } else {
+      for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + (lane >> 4);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A.at(ar, k), B.at(k, bc), acc, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = r0 + 4 * (lane >> 4) + i;
+      if (r < cr && bc < cc) {
+        if (accumulate)
+          C[r * ldc + bc] += acc[i];
+        else
+          C[r * ldc + bc] = acc[i];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Features of a k-grid ensemble (context grids and world-frame force grids)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void closure_features_kernel(
+    const float4* __restrict__ pos, long long n, const float* __restrict__ rot,
+    const float* __restrict__ off, const float* __restrict__ frot,
+    const float4* __restrict__ force, const long long* __restrict__ grids, Params P, float rho0,
+    const int* __restrict__ order, float4* __restrict__ feat, float4* __restrict__ mean_out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  // visiting order (spatially sorted when given): neighbouring lanes read
+  // neighbouring cells
+  const long long i = order ? (long long)order[t] : t;
+  const float4 p = pos[i];
+  const float c0 = p.x - P.half, c1 = p.y - P.half, c2 = p.z - P.half;
+  const int pfast = particle_fast(c0, c1, c2, P.CB);
+  // Sums are taken about the first grid's force (shifted data), so the
+  // variance does not cancel catastrophically when the spread is small.
+  float bx = 0.f, by = 0.f, bz = 0.f;
+  float sx = 0.f, sy = 0.f, sz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, sm = 0.f;
+  for (int g = 0; g < P.k; ++g) {
+    const int cell = cell_index(c0, c1, c2, pfast, frot + 12 * g, rot + 12 * g, off + 4 * g, P);
+    const long long at = (long long)g * P.cells + cell;
+    const float4 F = force[at];
+    if (g == 0) {
+      bx = F.x;
+      by = F.y;
+      bz = F.z;
+    }
+    const float ex = F.x - bx, ey = F.y - by, ez = F.z - bz;
+    sx += ex;
+    sy += ey;
+    sz += ez;
+    qx = fmaf(ex, ex, qx);
+    qy = fmaf(ey, ey, qy);
+    qz = fmaf(ez, ez, qz);
+    sm += (float)grids[at] * 1e-3f;
+  }
+  const float ik = 1.0f / (float)P.k;
+  const float ux = sx * ik, uy = sy * ik, uz = sz * ik;
+  const float mx = bx + ux, my = by + uy, mz = bz + uz;
+  const float vx = fmaxf(qx * ik - ux * ux, 0.f), vy = fmaxf(qy * ik - uy * uy, 0.f),
+              vz = fmaxf(qz * ik - uz * uz, 0.f);
+  const float dx = sqrtf(vx), dy = sqrtf(vy), dz = sqrtf(vz);
+  const float S = sqrtf(mx * mx + my * my + mz * mz) + sqrtf(vx + vy + vz) + 1e-30f;
+  const float iS = 1.0f / S;
+  feat[2 * i] = make_float4(mx * iS, my * iS, mz * iS, dx * iS);
+  feat[2 * i + 1] = make_float4(dy * iS, dz * iS, log1pf(sm * ik / rho0), S);
+  mean_out[i] = make_float4(mx, my, mz, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// MLP inference: F = m + S * MLP(x)
+// ---------------------------------------------------------------------------
+template <bool BF16>
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restrict__ prm,
+                                                          const float4* __restrict__ feat,
+                                                          const float4* __restrict__ mean,
+                                                          long long n, float4* __restrict__ out) {
+  __shared__ float X[TILE * 8];
+  __shared__ float H0[TILE * kH];
+  __shared__ float H1[TILE * kH];
+  __shared__ float Y[TILE * 16];
+  const long long ntile = (n + TILE - 1) / TILE;
+  for (long long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const long long base = tile * TILE;
+    for (int i = threadIdx.x; i < TILE * 8; i += blockDim.x) {
+      const long long p = base + i / 8;
+      const int f = i % 8;
+      float v = 0.f;
+      if (p < n && f < kIn) {
+        const float4 a = feat[2 * p + f / 4];
+        v = (&a.x)[f % 4];
+      }
+      X[i] = v;
+    }
+    __syncthreads();
+    // layer 1: Z1 = X W1 + b1 (bias added after); H0 holds Z1
+    wg_gemm<BF16>(Mat{X, 8, 1, TILE, kIn, false}, Mat{prm + oW1, kH, 1, kIn, kH, false}, H0, kH,
+                  TILE, kH, 32, TILE, kH, false);
+    __syncthreads();
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) H0[i] += prm[ob1 + i % kH];
+    __syncthreads();
+    wg_gemm<BF16>(Mat{H0, kH, 1, TILE, kH, true}, Mat{prm + oW2, kH, 1, kH, kH, false}, H1, kH,
+                  TILE, kH, kH, TILE, kH, false);
+    __syncthreads();
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) H1[i] += prm[ob2 + i % kH];
+    __syncthreads();
+    wg_gemm<BF16>(Mat{H1, kH, 1, TILE, kH, true}, Mat{prm + oW3, kH, 1, kH, kH, false}, H0, kH,
+                  TILE, kH, kH, TILE, kH, false);
+    __syncthreads();
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) H0[i] += prm[ob3 + i % kH];
+    __syncthreads();
+    wg_gemm<BF16>(Mat{H0, kH, 1, TILE, kH, true}, Mat{prm + oW4, kOut, 1, kH, kOut, false}, Y, 16,
+                  TILE, 16, kH, TILE, kOut, false);
+    __syncthreads();
+    for (int r = threadIdx.x; r < TILE; r += blockDim.x) {
+      const long long p = base + r;
+      if (p < n) {
+        const float sc = feat[2 * p + 1].w;
+        const float4 m = mean[p];
+        out[p] = make_float4(fmaf(sc, Y[r * 16 + 0] + prm[ob4 + 0], m.x),
+                             fmaf(sc, Y[r * 16 + 1] + prm[ob4 + 1], m.y),
+                             fmaf(sc, Y[r * 16 + 2] + prm[ob4 + 2], m.z), 0.f);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident BF16 inference (the production forward pass)
+// ---------------------------------------------------------------------------
+// One wave evaluates 16 particles per iteration with every weight held as a
+// v_mfma_f32_16x16x32_bf16 A-fragment in VGPRs (88 VGPRs) and the
+// activations never leaving registers.  The product is taken transposed,
+// H^T = W^T X^T: the weights are the A operand (rows = output features) and
+// the particles the B/C columns, so lane l's accumulators hold features
+// 16t + 4q + i (tile t, q = l/16, i = 0..3) of particle l % 16.  The next
+// layer's B operand needs, in lane group q, contraction indices 8q + j of a
+// 32-wide K chunk; the contraction order is free, so chunk c slot j of group
+// q is defined to be feature kperm(c, q, j) = 16(2c + j/4) + 4q + j%4 --
+// exactly the lane's own accumulator h[2c + j/4][j%4] -- and the weight
+// fragments are loaded in the same order.  No LDS round trip, no shuffles.
+// Layer 1's bias rides in the padded K slot 7 (input 1.0).
+__device__ __forceinline__ int kperm(int c, int q, int j) {
+  return 16 * (2 * c + (j >> 2)) + 4 * q + (j & 3);
+}
+
+// z * sigmoid(z) with the hardware exp2 and reciprocal (about 1 ulp each;
+// the result is rounded to bf16 next).  Written with the builtins because
+// this library is compiled with correctly rounded division, which would
+// expand a quotient into the full IEEE sequence.
+__device__ __forceinline__ float silu_fast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * z));
+}
+
+__device__ __forceinline__ v8bf silu_chunk(const v4f& lo, const v4f& hi) {
+  v8bf b;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    b[i] = (__bf16)silu_fast(lo[i]);
+    b[4 + i] = (__bf16)silu_fast(hi[i]);
+  }
+  return b;
+}
+
+__global__ __launch_bounds__(256) void mlp_forward_reg_kernel(const float* __restrict__ prm,
+                                                              const float4* __restrict__ feat,
+                                                              const float4* __restrict__ mean,
+                                                              long long n,
+                                                              float4* __restrict__ out) {
+  __shared__ float P[kParams];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) P[i] = prm[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  v8bf w1[4], w2[4][2], w3[4][2], w4[2];
+  v4f b2[4], b3[4], b4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = 8 * q + j;
+      const float v = k < kIn ? P[oW1 + k * kH + 16 * t + r] : (k == kIn ? P[ob1 + 16 * t + r] : 0.f);
+      w1[t][j] = (__bf16)v;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int f = kperm(c, q, j);
+        w2[t][c][j] = (__bf16)P[oW2 + f * kH + 16 * t + r];
+        w3[t][c][j] = (__bf16)P[oW3 + f * kH + 16 * t + r];
+      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      b2[t][i] = P[ob2 + 16 * t + 4 * q + i];
+      b3[t][i] = P[ob3 + 16 * t + 4 * q + i];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      w4[c][j] = (__bf16)(r < kOut ? P[oW4 + kperm(c, q, j) * kOut + r] : 0.f);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) b4[i] = (4 * q + i < kOut) ? P[ob4 + 4 * q + i] : 0.f;
+
+  const long long groups = (n + 15) >> 4;
+  const int wpb = blockDim.x >> 6;
+  const long long nwaves = (long long)gridDim.x * wpb;
+  for (long long gi = (long long)blockIdx.x * wpb + (threadIdx.x >> 6); gi < groups; gi += nwaves) {
+    const long long p = gi * 16 + r;
+    const bool live = p < n;
+    float4 fa = make_float4(0.f, 0.f, 0.f, 0.f), fb = fa;
+    if (live && q == 0) {
+      fa = feat[2 * p];
+      fb = feat[2 * p + 1];
+    }
+    v8bf x;
+    x[0] = (__bf16)fa.x;
+    x[1] = (__bf16)fa.y;
+    x[2] = (__bf16)fa.z;
+    x[3] = (__bf16)fa.w;
+    x[4] = (__bf16)fb.x;
+    x[5] = (__bf16)fb.y;
+    x[6] = (__bf16)fb.z;
+    x[7] = (__bf16)(q == 0 ? 1.f : 0.f);
+    v4f h[4], z[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      h[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[t], x, v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    {
+      const v8bf a0 = silu_chunk(h[0], h[1]), a1 = silu_chunk(h[2], h[3]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][0], a0, b2[t], 0, 0, 0);
+        z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][1], a1, z[t], 0, 0, 0);
+      }
+    }
+    {
+      const v8bf a0 = silu_chunk(z[0], z[1]), a1 = silu_chunk(z[2], z[3]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        h[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][0], a0, b3[t], 0, 0, 0);
+        h[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][1], a1, h[t], 0, 0, 0);
+      }
+    }
+    v4f y;
+    {
+      const v8bf a0 = silu_chunk(h[0], h[1]), a1 = silu_chunk(h[2], h[3]);
+      y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[0], a0, b4, 0, 0, 0);
+      y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[1], a1, y, 0, 0, 0);
+    }
+    if (live && q == 0) {
+      const float4 m = mean[p];
+      out[p] = make_float4(fmaf(fb.w, y[0], m.x), fmaf(fb.w, y[1], m.y), fmaf(fb.w, y[2], m.z), 0.f);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused training pass: forward, loss, backward, gradient accumulation
+// ---------------------------------------------------------------------------
+// grad (kParams floats) and loss (1 float) are accumulated with atomics; the
+// caller clears them.  wnorm = 1 / sum_p S_p^2 over the whole batch, so the
+// loss is sum_p S_p^2 |y_p - t_p|^2 / sum_p S_p^2 (the squared relative RMS
+// error of the predicted force when the batch is all particles).
+template <bool BF16>
+__global__ __launch_bounds__(256) void mlp_train_kernel(const float* __restrict__ prm,
+                                                        const float4* __restrict__ feat,
+                                                        const float4* __restrict__ mean,
+                                                        const float4* __restrict__ target,
+                                                        long long n, float wnorm,
+                                                        float* __restrict__ grad,
+                                                        float* __restrict__ loss) {
+  extern __shared__ float sm[];
+  float* X = sm;                  // TILE x 8
+  float* Z1 = X + TILE * 8;       // TILE x 64 (pre-activations)
+  float* Z2 = Z1 + TILE * kH;
+  float* Z3 = Z2 + TILE * kH;
+  float* DY = Z3 + TILE * kH;     // TILE x 16
+  float* D1 = DY + TILE * 16;     // TILE x 64 scratch
+  float* D2 = D1 + TILE * kH;     // TILE x 64 scratch
+  float* G = D2 + TILE * kH;      // kParams gradient accumulator
+  float* LS = G + kParams;        // 1 loss accumulator
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) G[i] = 0.f;
+  if (threadIdx.x == 0) LS[0] = 0.f;
+  __syncthreads();
+  const long long ntile = (n + TILE - 1) / TILE;
+  for (long long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const long long base = tile * TILE;
+    for (int i = threadIdx.x; i < TILE * 8; i += blockDim.x) {
+      const long long p = base + i / 8;
+      const int f = i % 8;
+      float v = 0.f;
+      if (p < n && f < kIn) {
+        const float4 a = feat[2 * p + f / 4];
+        v = (&a.x)[f % 4];
+      }
+      X[i] = v;
+    }
+    __syncthreads();
+    // forward
+    wg_gemm<BF16>(Mat{X, 8, 1, TILE, kIn, false}, Mat{prm + oW1, kH, 1, kIn, kH, false}, Z1, kH,
+                  TILE, kH, 32, TILE, kH, false);
+    __syncthreads();
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) Z1[i] += prm[ob1 + i % kH];
+    __syncthreads();
+    wg_gemm<BF16>(Mat{Z1, kH, 1, TILE, kH, true}, Mat{prm + oW2, kH, 1, kH, kH, false}, Z2, kH,
+                  TILE, kH, kH, TILE, kH, false);
+    __syncthreads();
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) Z2[i] += prm[ob2 + i % kH];
+    __syncthreads();
+    wg_gemm<BF16>(Mat{Z2, kH, 1, TILE, kH, true}, Mat{prm + oW3, kH, 1, kH, kH, false}, Z3, kH,
+                  TILE, kH, kH, TILE, kH, false);
+    __syncthreads();
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) Z3[i] += prm[ob3 + i % kH];
+    __syncthreads();
+    wg_gemm<BF16>(Mat{Z3, kH, 1, TILE, kH, true}, Mat{prm + oW4, kOut, 1, kH, kOut, false}, DY, 16,
+                  TILE, 16, kH, TILE, kOut, false);
+    __syncthreads();
+    // loss and dL/dy (zero rows beyond n)
+    for (int r = threadIdx.x; r < TILE; r += blockDim.x) {
+      const long long p = base + r;
+      float l = 0.f;
+      for (int c = 0; c < 16; ++c) {
+        float d = 0.f;
+        if (p < n && c < kOut) {
+          const float S = feat[2 * p + 1].w;
+          const float4 m = mean[p], tg = target[p];
+          const float t = ((&tg.x)[c] - (&m.x)[c]) / S;
+          const float w = S * S * wnorm;
+          const float e = DY[r * 16 + c] + prm[ob4 + c] - t;
+          l += w * e * e;
+          d = 2.0f * w * e;
+        }
+        DY[r * 16 + c] = d;
+      }
+      atomicAdd(LS, l);
+    }
+    __syncthreads();
+    // layer 4: dW4 += A3^T DY, db4 += sum DY, D1 = (DY W4^T) * silu'(Z3)
+    wg_gemm<BF16>(Mat{Z3, 1, kH, kH, TILE, true}, Mat{DY, 16, 1, TILE, kOut, false}, G + oW4, kOut,
+                  kH, 16, TILE, kH, kOut, true);
+    wg_gemm<BF16>(Mat{DY, 16, 1, TILE, kOut, false}, Mat{prm + oW4, 1, kOut, kOut, kH, false}, D1,
+                  kH, TILE, kH, 32, TILE, kH, false);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kOut; i += blockDim.x) {
+      float s = 0.f;
+      for (int r = 0; r < TILE; ++r) s += DY[r * 16 + i];
+      G[ob4 + i] += s;
+    }
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) D1[i] *= dsilu(Z3[i]);
+    __syncthreads();
+    // layer 3: dW3 += A2^T D1, db3 += sum D1, D2 = (D1 W3^T) * silu'(Z2)
+    wg_gemm<BF16>(Mat{Z2, 1, kH, kH, TILE, true}, Mat{D1, kH, 1, TILE, kH, false}, G + oW3, kH, kH,
+                  kH, TILE, kH, kH, true);
+    wg_gemm<BF16>(Mat{D1, kH, 1, TILE, kH, false}, Mat{prm + oW3, 1, kH, kH, kH, false}, D2, kH,
+                  TILE, kH, kH, TILE, kH, false);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+      float s = 0.f;
+      for (int r = 0; r < TILE; ++r) s += D1[r * kH + i];
+      G[ob3 + i] += s;
+    }
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) D2[i] *= dsilu(Z2[i]);
+    __syncthreads();
+    // layer 2: dW2 += A1^T D2, db2 += sum D2, D1 = (D2 W2^T) * silu'(Z1)
+    wg_gemm<BF16>(Mat{Z1, 1, kH, kH, TILE, true}, Mat{D2, kH, 1, TILE, kH, false}, G + oW2, kH, kH,
+                  kH, TILE, kH, kH, true);
+    wg_gemm<BF16>(Mat{D2, kH, 1, TILE, kH, false}, Mat{prm + oW2, 1, kH, kH, kH, false}, D1, kH,
+                  TILE, kH, kH, TILE, kH, false);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+      float s = 0.f;
+      for (int r = 0; r < TILE; ++r) s += D2[r * kH + i];
+      G[ob2 + i] += s;
+    }
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) D1[i] *= dsilu(Z1[i]);
+    __syncthreads();
+    // layer 1: dW1 += X^T D1, db1 += sum D1
+    wg_gemm<BF16>(Mat{X, 1, 8, kIn, TILE, false}, Mat{D1, kH, 1, TILE, kH, false}, G + oW1, kH, 16,
+                  kH, TILE, kIn, kH, true);
+    for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+      float s = 0.f;
+      for (int r = 0; r < TILE; ++r) s += D1[r * kH + i];
+      G[ob1 + i] += s;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) atomicAdd(&grad[i], G[i]);
+  if (threadIdx.x == 0) atomicAdd(loss, LS[0]);
+}
+
+// Adam on fp32 parameters (bias-corrected).
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ prm,
+                                                   const float* __restrict__ grad,
+                                                   float* __restrict__ m1, float* __restrict__ m2,
+                                                   int np, float lr, float b1, float b2, float eps,
+                                                   float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const float g = grad[i];
+  const float a = b1 * m1[i] + (1.f - b1) * g;
+  const float b = b2 * m2[i] + (1.f - b2) * g * g;
+  m1[i] = a;
+  m2[i] = b;
+  prm[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// FP64 direct summation of the softened Newtonian potential gradient at a
+// subsample of target particles, minimum-image in the periodic box:
+// grad_i = G_eff sum_j m_j d_ij / (|d_ij|^2 + eps^2)^(3/2), d_ij = r_i - r_j
+// wrapped into [-W/2, W/2).  One workgroup per target; sources are streamed
+// by the workgroup's threads and reduced in LDS.
+__global__ __launch_bounds__(256) void direct_sum_kernel(const float4* __restrict__ pos,
+                                                         long long n,
+                                                         const int* __restrict__ targets,
+                                                         double W, double G_eff, double eps2,
+                                                         double* __restrict__ out) {
+  __shared__ double red[3][256];
+  const int t = blockIdx.x;
+  const long long ti = targets[t];
+  if (ti < 0 || ti >= n) {  // invalid index: NaN result, no out-of-bounds read
+    if (threadIdx.x == 0)
+      for (int d = 0; d < 3; ++d) out[3 * t + d] = __builtin_nan("");
+    return;
+  }
+  const float4 pi = pos[ti];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  const double hw = 0.5 * W;
+  for (long long j = threadIdx.x; j < n; j += blockDim.x) {
+    if (j == ti) continue;  // no self term (it would be 0/0 without softening)
+    const float4 pj = pos[j];
+    double dx = (double)pi.x - (double)pj.x, dy = (double)pi.y - (double)pj.y,
+           dz = (double)pi.z - (double)pj.z;
+    dx -= W * floor((dx + hw) / W);
+    dy -= W * floor((dy + hw) / W);
+    dz -= W * floor((dz + hw) / W);
+    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+    const double inv = (double)pj.w / (r2 * sqrt(r2));
+    ax += dx * inv;
+    ay += dy * inv;
+    az += dz * inv;
+  }
+  red[0][threadIdx.x] = ax;
+  red[1][threadIdx.x] = ay;
+  red[2][threadIdx.x] = az;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 3; ++d) out[3 * t + d] = G_eff * red[d][0];
+}
+
+hipError_t launch_direct_sum(const float4* pos, long long n, const int* targets, int nt, double W,
+                             double G_eff, double eps2, double* out, hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  hipLaunchKernelGGL(direct_sum_kernel, dim3((unsigned)nt), dim3(256), 0, s, pos, n, targets, W,
+                     G_eff, eps2, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_closure_features(const Params& P, const float4* pos, long long n,
+                                   const float* rot, const float* off, const float* frot,
+                                   const float4* force, const long long* grids, float rho0,
+                                   const int* order, float4* feat, float4* mean, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(closure_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     pos, n, rot, off, frot, force, grids, P, rho0, order, feat, mean);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const float* prm, const float4* feat, const float4* mean,
+                              long long n, float4* out, bool bf16, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (bf16) {
+    // 4 waves per workgroup, 16 particles per wave iteration; the kernel
+    // holds ~230 registers per lane (2 waves per SIMD), so 512 workgroups
+    // (2 per CU, 36 KB of LDS each for the staged parameters) fill the chip.
+    const long long groups = (n + 15) / 16;
+    const long long want = (groups + 3) / 4;
+    const unsigned blocks = (unsigned)(want < 512 ? want : 512);
+    hipLaunchKernelGGL(mlp_forward_reg_kernel, dim3(blocks), dim3(256), 0, s, prm, feat, mean, n,
+                       out);
+  } else {
+    const long long tiles = (n + TILE - 1) / TILE;
+    const unsigned blocks = (unsigned)(tiles < 4096 ? tiles : 4096);
+    hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3(blocks), dim3(256), 0, s, prm, feat, mean, n,
+                       out);
+  }
+  return hipGetLastError();
+}
+
+size_t mlp_param_count() { return (size_t)kParams; }
+
+hipError_t launch_mlp_train(const float* prm, const float4* feat, const float4* mean,
+                            const float4* target, long long n, float wnorm, float* grad,
+                            float* loss, bool bf16, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const size_t lds = (size_t)(TILE * 8 + 3 * TILE * kH + TILE * 16 + 2 * TILE * kH + kParams + 1) *
+                     sizeof(float);
+  const void* fn = bf16 ? (const void*)mlp_train_kernel<true> : (const void*)mlp_train_kernel<false>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  const long long tiles = (n + TILE - 1) / TILE;
+  const unsigned blocks = (unsigned)(tiles < 256 ? tiles : 256);
+  if (bf16)
+    hipLaunchKernelGGL(mlp_train_kernel<true>, dim3(blocks), dim3(256), lds, s, prm, feat, mean,
+                       target, n, wnorm, grad, loss);
+  else
+    hipLaunchKernelGGL(mlp_train_kernel<false>, dim3(blocks), dim3(256), lds, s, prm, feat, mean,
+                       target, n, wnorm, grad, loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float* prm, const float* grad, float* m1, float* m2, float lr, float b1,
+                       float b2, float eps, int step, hipStream_t s) {
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, prm, grad, m1, m2,
+                     kParams, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+}  // namespace mog

@@ -122,6 +122,20 @@ hipError_t build_order(const float4* pos, long long n, float W, unsigned* keys_a
 hipError_t launch_init_perlin(float4* pos, float4* vel, long long n, float W, float noise_scale,
                               float contrast, unsigned seed, float base_mass, float spawn_buffer,
                               int sphere, hipStream_t s);
+hipError_t launch_closure_features(const Params& P, const float4* pos, long long n,
+                                   const float* rot, const float* off, const float* frot,
+                                   const float4* force, const long long* grids, float rho0,
+                                   const int* order, float4* feat, float4* mean, hipStream_t s);
+size_t mlp_param_count();
+hipError_t launch_direct_sum(const float4* pos, long long n, const int* targets, int nt, double W,
+                             double G_eff, double eps2, double* out, hipStream_t s);
+hipError_t launch_mlp_forward(const float* prm, const float4* feat, const float4* mean,
+                              long long n, float4* out, bool bf16, hipStream_t s);
+hipError_t launch_mlp_train(const float* prm, const float4* feat, const float4* mean,
+                            const float4* target, long long n, float wnorm, float* grad,
+                            float* loss, bool bf16, hipStream_t s);
+hipError_t launch_adam(float* prm, const float* grad, float* m1, float* m2, float lr, float b1,
+                       float b2, float eps, int step, hipStream_t s);
 hipError_t launch_update(const Params& P, float4* pos, float4* vel, const float4* avg,
                          long long n, hipStream_t s);
 

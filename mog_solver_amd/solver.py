@@ -16,6 +16,7 @@ the current stream only, every computation runs in ``libmog.so``.
 from __future__ import annotations
 
 import ctypes
+from pathlib import Path
 
 import numpy as np
 
@@ -235,7 +236,7 @@ class Simulation:
     tensors of shape (n,4).
     """
 
-    def __init__(self, config=None, seed: int = 0, device=None):
+    def __init__(self, config=None, seed: int = 0, device=None, transforms=None):
         self.cfg = config if isinstance(config, dict) else self.load_config(config)
         sc = SolverConfig.from_dict(self.cfg)
         self.solver_cfg = sc
@@ -249,7 +250,13 @@ class Simulation:
         self.G = sc.G
         self.dt = sc.dt
         self.seed = seed
-        self.offsets, self.rotations = self.compute_transforms()
+        self.step_count = 0
+        if transforms is None:
+            self.offsets, self.rotations = self.compute_transforms()
+        else:
+            rot, off = transforms
+            self.rotations = np.ascontiguousarray(rot, dtype=np.float32)
+            self.offsets = np.ascontiguousarray(off, dtype=np.float32)
         self.use_convolution = sc.use_convolution
         self.device = device
         self.solver = MogSolver(sc, rot=self.rotations, off=self.offsets)
@@ -334,8 +341,43 @@ class Simulation:
         """The reference always runs the FFT solver (main.py:671-672)."""
         if not self.paused:
             self.step_fft()
+            self.step_count += 1
 
     def run(self, nsteps, use_graph=True):
+        """``nsteps`` steps.  With ``record`` set in the config, a snapshot is
+        written every ``record_every`` steps (default 10) to ``record_dir``
+        (default ``snapshots``)."""
         if self.pos_buf is None:
             raise RuntimeError("init_particles() first")
-        self.solver.run(self.pos_buf, self.vel_buf, nsteps, use_graph=use_graph)
+        if self.paused:
+            return
+        if self.solver.cfg.timescale != self.timescale:
+            self.solver.set_integrator(timescale=self.timescale)
+        every = int(self.cfg.get("record_every", 10)) if self.cfg.get("record") else 0
+        done = 0
+        while done < nsteps:
+            chunk = nsteps - done if every <= 0 else min(nsteps - done, every - self.step_count % every)
+            self.solver.run(self.pos_buf, self.vel_buf, chunk, use_graph=use_graph)
+            done += chunk
+            self.step_count += chunk
+            if every > 0 and self.step_count % every == 0:
+                self.save(Path(self.cfg.get("record_dir", "snapshots")) / f"step_{self.step_count:08d}.npz")
+
+    def save(self, path):
+        """Write the current state as a snapshot (``mog_solver_amd.snapshot``)."""
+        from .snapshot import save_snapshot
+
+        save_snapshot(path, self.pos_buf, self.vel_buf, self.rotations, self.offsets, self.step_count,
+                      self.cfg)
+
+    @classmethod
+    def from_snapshot(cls, path, device=None):
+        """A simulation restored from a snapshot: configuration, transforms,
+        particle state and step counter."""
+        from .snapshot import load_snapshot
+
+        snap = load_snapshot(path)
+        sim = cls(snap["config"], device=device, transforms=(snap["rot"], snap["off"]))
+        sim.init_particles(snap["pos"], snap["vel"])
+        sim.step_count = snap["step"]
+        return sim

@@ -1,0 +1,262 @@
+"""GPU parity of the learned-closure kernels against the FP64 closure oracle.
+
+Bars (tolerances stated per test): features within float32 rounding of the
+oracle's; MLP forward within 1e-5 (FP32 matrix cores) / 3e-2 (BF16
+operands) of the oracle in units of the output's RMS; gradients within
+1e-4 / 5e-2 relative norm; Adam bit-for-bit formula within 1e-6; FP64
+direct summation within 1e-12 of a NumPy FP64 sum.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import closure_oracle as C
+from oracle import mog_oracle as O
+from mog_solver_amd.config import SolverConfig
+from mog_solver_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def solver(k, g, W=100.0, G=100.0, compat="reference", seed=0):
+    from mog_solver_amd.solver import MogSolver
+
+    cfg = SolverConfig.from_dict(dict(n_grids=k, grid_size=g, world_size=W, G=G, dt=1e-5, damping=0.999,
+                                      solver="fft", compat=compat))
+    rot, off = O.se3_transforms(k, g, W, seed)
+    return MogSolver(cfg, rot=rot, off=off), rot, off
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def clustered(n, seed=5, W=100.0):
+    return workloads.plummer_spheres(n=n, n_spheres=4, scale=6.0, world_size=W, seed=seed)
+
+
+@pytest.mark.parametrize("compat", ["reference", "physical"])
+def test_features_match_oracle(torch_cuda, compat):
+    from mog_solver_amd.closure import closure_features
+
+    torch = torch_cuda
+    k, g, W, G = 4, 16, 100.0, 100.0
+    pos, _ = clustered(4096)
+    s, rot, off = solver(k, g, W, G, compat=compat, seed=3)
+    rho0 = 4096 / g**3
+    feat, mean = closure_features(s, dev(torch, pos), rho0)
+    feat = feat.cpu().numpy().astype(np.float64)
+    mean = mean.cpu().numpy().astype(np.float64)
+    want_f, want_m = C.features(pos, rot, off, k, g, W, G, rho0, compat=compat)
+    scale = np.sqrt(np.mean(np.sum(want_m**2, 1)))
+    assert np.sqrt(np.mean(np.sum((mean[:, :3] - want_m) ** 2, 1))) / scale < 1e-5
+    assert np.all(mean[:, 3] == 0)
+    # scale-free features: absolute error (values are O(1))
+    assert np.max(np.abs(feat[:, 0:3] - want_f[:, 0:3])) < 1e-4
+    assert np.max(np.abs(feat[:, 3:6] - want_f[:, 3:6])) < 2e-3
+    np.testing.assert_allclose(feat[:, 6], want_f[:, 6], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(feat[:, 7], want_f[:, 7], rtol=1e-4)
+
+
+def random_inputs(n, seed):
+    rng = np.random.default_rng(seed)
+    feat = np.zeros((n, 8), np.float32)
+    m = rng.standard_normal((n, 3))
+    sd = np.abs(rng.standard_normal((n, 3))) * 0.3
+    S = np.linalg.norm(m, axis=1) + np.linalg.norm(sd, axis=1)
+    feat[:, 0:3] = m / S[:, None]
+    feat[:, 3:6] = sd / S[:, None]
+    feat[:, 6] = np.log1p(rng.exponential(1.0, n))
+    feat[:, 7] = S * rng.uniform(0.5, 2.0, n)
+    mean = np.zeros((n, 4), np.float32)
+    mean[:, :3] = m
+    target = np.zeros((n, 4), np.float32)
+    target[:, :3] = m + feat[:, 7:8] * 0.2 * rng.standard_normal((n, 3))
+    return feat, mean, target
+
+
+@pytest.mark.parametrize("bf16,tol", [(False, 1e-5), (True, 3e-2)])
+@pytest.mark.parametrize("n", [1, 63, 1000, 70001])
+def test_mlp_forward_matches_oracle(torch_cuda, bf16, tol, n):
+    from mog_solver_amd.closure import ClosureMLP
+
+    torch = torch_cuda
+    feat, mean, _ = random_inputs(n, seed=n)
+    params = C.init_params(seed=7, out_scale=1.0)
+    mlp = ClosureMLP(params, bf16=bf16)
+    got = mlp.forward(dev(torch, feat), dev(torch, mean)).cpu().numpy().astype(np.float64)
+    want = C.predict(params, feat, mean)
+    S = feat[:, 7:8].astype(np.float64)
+    y_err = (got[:, :3] - want) / S
+    y_ref = (want - mean[:, :3]) / S
+    rel = math.sqrt(np.mean(np.sum(y_err**2, 1)) / np.mean(np.sum(y_ref**2, 1)))
+    assert rel < tol, rel
+    assert np.all(got[:, 3] == 0)
+
+
+@pytest.mark.parametrize("bf16,tol", [(False, 1e-4), (True, 5e-2)])
+@pytest.mark.parametrize("n", [1, 100, 5000, 40000])
+def test_mlp_gradient_matches_oracle(torch_cuda, bf16, tol, n):
+    from mog_solver_amd.closure import ClosureMLP
+
+    torch = torch_cuda
+    feat, mean, target = random_inputs(n, seed=100 + n)
+    params = C.init_params(seed=11, out_scale=1.0)
+    wnorm = 1.0 / float(np.sum(feat[:, 7].astype(np.float64) ** 2))
+    mlp = ClosureMLP(params, bf16=bf16)
+    loss, grad = mlp.loss_and_grad(dev(torch, feat), dev(torch, mean), dev(torch, target), wnorm=wnorm)
+    grad = grad.cpu().numpy().astype(np.float64)
+    want_loss, want_grad = C.loss_and_grad(params, feat, mean, target, wnorm)
+    assert abs(loss - want_loss) / want_loss < tol
+    rel = np.linalg.norm(grad - want_grad) / np.linalg.norm(want_grad)
+    assert rel < tol, rel
+    # every parameter block carries gradient
+    at = 0
+    for name, shape in C.SHAPES:
+        size = int(np.prod(shape))
+        blk = want_grad[at:at + size]
+        assert np.linalg.norm(grad[at:at + size] - blk) <= max(tol * 10, 0.2) * np.linalg.norm(blk) + 1e-12, name
+        at += size
+
+
+def test_adam_matches_formula(torch_cuda):
+    from mog_solver_amd.closure import ClosureMLP
+
+    torch = torch_cuda
+    rng = np.random.default_rng(0)
+    params = rng.standard_normal(C.N_PARAMS).astype(np.float32)
+    mlp = ClosureMLP(params)
+    p = params.astype(np.float64)
+    m1 = np.zeros_like(p)
+    m2 = np.zeros_like(p)
+    lib = mlp.lib
+    from mog_solver_amd.solver import _ptr, _stream_ptr
+
+    for step in range(1, 4):
+        g = rng.standard_normal(C.N_PARAMS).astype(np.float32)
+        gd = dev(torch, g)
+        rc = lib.mog_mlp_adam(_ptr(mlp.params), _ptr(gd), _ptr(mlp.m1), _ptr(mlp.m2), 1e-2, 0.9, 0.999, 1e-8,
+                              step, _stream_ptr())
+        assert rc == 0
+        m1 = 0.9 * m1 + 0.1 * g
+        m2 = 0.999 * m2 + 0.001 * g.astype(np.float64) ** 2
+        p -= 1e-2 * (m1 / (1 - 0.9**step)) / (np.sqrt(m2 / (1 - 0.999**step)) + 1e-8)
+    np.testing.assert_allclose(mlp.params.cpu().numpy(), p, rtol=1e-5, atol=1e-6)
+
+
+def numpy_direct(pos, targets, W, G_eff, eps):
+    p = pos.astype(np.float64)
+    out = np.zeros((len(targets), 3))
+    for i, t in enumerate(targets):
+        d = p[t, :3] - p[:, :3]
+        d -= W * np.floor((d + W / 2) / W)
+        r2 = np.sum(d * d, 1) + eps * eps
+        r2[t] = np.inf
+        out[i] = G_eff * np.sum(p[:, 3:4] * d / (r2 * np.sqrt(r2))[:, None], 0)
+    return out
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.5])
+def test_direct_sum_matches_numpy(torch_cuda, eps):
+    from mog_solver_amd.closure import direct_sum
+
+    torch = torch_cuda
+    pos, _ = clustered(3000, seed=9)
+    pos[:, 3] = np.random.default_rng(1).uniform(0.5, 2.0, 3000).astype(np.float32)
+    targets = np.array([0, 1, 17, 2999, 1500], np.int32)
+    got = direct_sum(dev(torch, pos), targets, 100.0, 3.0, eps).cpu().numpy()
+    want = numpy_direct(pos, targets, 100.0, 3.0, eps)
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12 * np.abs(want).max())
+
+
+def test_direct_sum_invalid_target_is_nan(torch_cuda):
+    from mog_solver_amd.closure import direct_sum
+
+    torch = torch_cuda
+    pos, _ = clustered(500)
+    got = direct_sum(dev(torch, pos), np.array([3, 500, -1], np.int32), 100.0, 1.0, 0.1).cpu().numpy()
+    assert np.all(np.isfinite(got[0])) and np.all(np.isnan(got[1:]))
+
+
+def test_mog_force_approximates_newton(torch_cuda):
+    """Far from two compact clusters the MOG avgForce is the Newtonian
+    gradient G_eff * sum m d / r^3 (same sign, coupling G h^3 / 2): light
+    probe particles 4-8 cells away from both clusters."""
+    from mog_solver_amd.closure import direct_sum
+
+    torch = torch_cuda
+    k, g, W, G = 32, 32, 100.0, 100.0
+    h = W / g
+    rng = np.random.default_rng(4)
+    centres = np.array([[40.0, 50.0, 50.0], [62.0, 47.0, 52.0]])
+
+    def ball(n, c, r):
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        return c + d * r * rng.uniform(0, 1, (n, 1)) ** (1 / 3)
+
+    xyz = np.concatenate([ball(20000, centres[0], 1.5), ball(10000, centres[1], 1.5), rng.uniform(0, W, (4000, 3))])
+    pos = np.zeros((xyz.shape[0], 4), np.float32)
+    pos[:, :3] = xyz
+    pos[:, 3] = 1.0
+    pos[30000:, 3] = 1e-3
+    dist = np.min(np.linalg.norm(xyz[30000:, None, :] - centres[None], axis=2), axis=1)
+    probes = (30000 + np.nonzero((dist > 4 * h) & (dist < 8 * h))[0]).astype(np.int32)
+    assert probes.size > 100
+    s, _, _ = solver(k, g, W, G, compat="physical", seed=1)
+    p = dev(torch, pos)
+    a = s.forces(p).cpu().numpy()[probes, :3].astype(np.float64)
+    newton = direct_sum(p, probes, W, workloads.effective_G(G, W, g), 0.0).cpu().numpy()
+    cos = np.sum(a * newton, 1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(newton, axis=1))
+    ratio = np.linalg.norm(a) / np.linalg.norm(newton)
+    print(f"probes {probes.size}: median cos {np.median(cos):.4f}, min {cos.min():.4f}, norm ratio {ratio:.4f}")
+    assert np.median(cos) > 0.98
+    assert 0.8 < ratio < 1.2
+
+
+def test_training_reduces_error_and_beats_plain(torch_cuda):
+    """Trained on snapshots of a k=64 run, the k=8 closure predicts the k=64
+    force on held-out later snapshots better than the k=8 mean does."""
+    from mog_solver_amd import closure as CL
+
+    torch = torch_cuda
+    g, W, G = 16, 100.0, 100.0
+    pos, vel = clustered(32768, seed=21)
+    full, _, _ = solver(64, g, W, G, seed=1)
+    small, _, _ = solver(8, g, W, G, seed=2)
+    full.set_integrator(dt=0.02, damping=1.0)
+    p, v = dev(torch, pos), dev(torch, vel)
+    rho0 = CL.default_rho0(32768, 1.0, g)
+    train_set = CL.make_snapshots(full, small, p, v, 12, every=2, rho0=rho0)
+    test_set = CL.make_snapshots(full, small, p, v, 4, every=3, rho0=rho0)
+    mlp = CL.ClosureMLP(seed=0, bf16=True)
+    plain0, closure0 = CL.evaluate(mlp, test_set)
+    hist = CL.train(mlp, train_set, epochs=40, lr=3e-3, lr_final=3e-4)
+    plain, closure = CL.evaluate(mlp, test_set)
+    print(f"plain {plain:.4f} closure before {closure0:.4f} after {closure:.4f}; loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    assert hist[-1] < 0.7 * hist[0]
+    assert closure < 0.85 * plain
+
+
+def test_closure_solver_audit_and_fallback(torch_cuda):
+    from mog_solver_amd import closure as CL
+
+    torch = torch_cuda
+    g = 16
+    pos, vel = clustered(8192, seed=8)
+    full, _, _ = solver(32, g, seed=1)
+    small, _, _ = solver(4, g, seed=2)
+    mlp = CL.ClosureMLP(seed=0)
+    rho0 = CL.default_rho0(8192, 1.0, g)
+    p, v = dev(torch, pos), dev(torch, vel)
+    never = CL.ClosureSolver(full, small, mlp, rho0, audit_every=5, threshold=1e9, fallback_steps=3)
+    modes = [never.step(p, v) for _ in range(12)]
+    assert modes == ["closure"] * 12
+    assert [a[0] for a in never.audits] == [0, 5, 10] and not any(a[2] for a in never.audits)
+    always = CL.ClosureSolver(full, small, mlp, rho0, audit_every=4, threshold=0.0, fallback_steps=3)
+    modes = [always.step(p, v) for _ in range(9)]
+    # every audit fails: audit at step 0, 3 fallback steps, audit at 4, ...
+    assert modes == ["full"] * 9
+    assert [a[0] for a in always.audits] == [0, 4, 8] and all(a[2] for a in always.audits)
+    assert bool(torch.isfinite(p).all()) and bool(torch.isfinite(v).all())

@@ -77,3 +77,20 @@ def test_two_galaxies_reference_seeded():
     r = np.linalg.norm(pos[:2000, :3] - [30, 50, 50], axis=1)
     speed = np.linalg.norm(vel[:2000, :3] - [0.3, 0.1, 0.0], axis=1)
     assert np.allclose(speed, 0.3 * np.sqrt(r / 100.0 + 0.01), rtol=1e-4)
+
+
+def test_snapshot_roundtrip(tmp_path):
+    from mog_solver_amd.snapshot import load_snapshot, save_snapshot
+    from mog_solver_amd.transforms import compute_transforms
+
+    pos, vel = workloads.uniform_cube(100, 10.0, seed=2)
+    off, rot = compute_transforms(4, 8, 10.0)
+    cfg = {"n_grids": 4, "grid_size": 8, "world_size": 10.0, "G": 1.0, "dt": "1e-5", "record": True}
+    save_snapshot(tmp_path / "a.npz", pos, vel, rot, off, 17, cfg)
+    s = load_snapshot(tmp_path / "a.npz")
+    assert np.array_equal(s["pos"], pos) and np.array_equal(s["vel"], vel)
+    assert np.array_equal(s["rot"], rot) and np.array_equal(s["off"], off)
+    assert s["step"] == 17 and s["config"] == cfg
+    np.savez(tmp_path / "bad.npz", pos=pos)
+    with pytest.raises(Exception):
+        load_snapshot(tmp_path / "bad.npz")
