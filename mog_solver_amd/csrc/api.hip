@@ -9,7 +9,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <complex>
 #include <string>
+#include <vector>
 
 #include "../../include/mog.h"
 #include "mog_common.h"
@@ -32,6 +34,7 @@ struct mog_ctx {
   float2* d_tw;
   float* d_gtab;
   float* d_kd;
+  float2* d_conv;  // real-space kernel spectrum (solver convolution, compat physical)
   float* d_Wf;  // MFMA DFT tables (g >= 16): forward, inverse
   float* d_Wi;
   DepositTune tune;
@@ -197,6 +200,66 @@ void build_tables(const mog_config* c, float2* tw, float* gtab, float* kd) {
       }
 }
 
+// `solver: convolution` as a direct periodic real-space convolution with the
+// Green's-gradient kernel of main.py:250-278: K_d(r) = r_d / (4 pi |r|^3) at
+// the minimum-image cell offsets (index i -> i if i <= g/2 else i - g, the
+// offset in units of voxel = W/g), K(0) = 0, values rounded to float32 as
+// the reference stores them.  With rho the mass per cell, the force is
+// F_d = 2 pi G voxel^3 (rho (*) K_d), the same coupling as the spectral
+// solve (grad phi with del^2 phi = 2 pi G rho), so a single mass m gives
+// G voxel^3 / 2 * m r / |r|^3 at every other cell.  The circular convolution
+// is applied through the step's own transforms: the spectrum
+// T_d = 2 pi G voxel^3 / (1000 g^3) * DFT(K_d) (FP64 separable DFT of the
+// float32 kernel) multiplies the milli-mass spectrum, and the unnormalised
+// inverse transforms give F exactly as they give the FFT solver's force.
+// out[0..g^3) = T_y + i T_z, out[g^3..2 g^3) = T_x.
+void build_conv_tables(const mog_config* c, float2* out) {
+  const int g = c->grid_size;
+  const size_t g3 = (size_t)g * g * g;
+  const double voxel = c->world_size / (double)g;
+  std::vector<std::complex<double>> K[3];
+  for (int d = 0; d < 3; ++d) K[d].assign(g3, 0.0);
+  auto sh = [g](int i) { return i <= g / 2 ? i : i - g; };
+  for (int iz = 0; iz < g; ++iz)
+    for (int iy = 0; iy < g; ++iy)
+      for (int ix = 0; ix < g; ++ix) {
+        const double rx = sh(ix) * voxel, ry = sh(iy) * voxel, rz = sh(iz) * voxel;
+        const double r2 = rx * rx + ry * ry + rz * rz;
+        const double r = sqrt(r2);
+        if (r > 1e-10) {
+          const double f = 1.0 / (4.0 * M_PI * r2 * r);
+          const size_t i = ((size_t)iz * g + iy) * g + ix;
+          K[0][i] = (double)(float)(rx * f);
+          K[1][i] = (double)(float)(ry * f);
+          K[2][i] = (double)(float)(rz * f);
+        }
+      }
+  // forward DFT (sign -) along x, y, z
+  std::vector<std::complex<double>> w(g), line(g);
+  for (int j = 0; j < g; ++j) w[j] = std::polar(1.0, -2.0 * M_PI * (double)j / (double)g);
+  const size_t stride[3] = {1, (size_t)g, (size_t)g * g};
+  for (int d = 0; d < 3; ++d)
+    for (int axis = 0; axis < 3; ++axis) {
+      const size_t st = stride[axis];
+      for (size_t base = 0; base < g3; ++base) {
+        if ((base / st) % g != 0) continue;  // one line per start index
+        for (int kk = 0; kk < g; ++kk) {
+          std::complex<double> acc = 0.0;
+          for (int j = 0; j < g; ++j) acc += K[d][base + j * st] * w[(size_t)j * kk % g];
+          line[kk] = acc;
+        }
+        for (int kk = 0; kk < g; ++kk) K[d][base + kk * st] = line[kk];
+      }
+    }
+  const double scale = 2.0 * M_PI * c->G * voxel * voxel * voxel / (1000.0 * (double)g3);
+  const std::complex<double> I(0.0, 1.0);
+  for (size_t i = 0; i < g3; ++i) {
+    const std::complex<double> yz = scale * (K[1][i] + I * K[2][i]), x = scale * K[0][i];
+    out[i] = make_float2((float)yz.real(), (float)yz.imag());
+    out[g3 + i] = make_float2((float)x.real(), (float)x.imag());
+  }
+}
+
 Params make_params(const mog_config* c) {
   Params P;
   P.k = c->n_grids;
@@ -233,6 +296,7 @@ void release(mog_ctx* c) {
   (void)hipFree(c->d_tw);
   (void)hipFree(c->d_gtab);
   (void)hipFree(c->d_kd);
+  (void)hipFree(c->d_conv);
   (void)hipFree(c->d_order);
   (void)hipFree(c->d_idx);
   (void)hipFree(c->d_keys_a);
@@ -352,9 +416,10 @@ hipError_t solve_dispatch(mog_ctx* ctx, const long long* grids, float4* force, i
   const Params& P = ctx->P;
   if (P.g >= 16 && (ctx->spectral_mfma || ctx->cfg.precision != MOG_PRECISION_FP32))
     return launch_solve_mfma(P, ctx->cfg.precision, grids, ctx->d_S1, ctx->d_S2, force, ctx->d_Wf,
-                             ctx->d_Wi, ctx->d_gtab, ctx->d_kd, ctx->need_x, ctx->d_rot, world, s);
+                             ctx->d_Wi, ctx->d_gtab, ctx->d_kd, ctx->d_conv, ctx->need_x,
+                             ctx->d_rot, world, s);
   return launch_solve(P, grids, ctx->d_S1, ctx->d_S2, force, ctx->d_tw, ctx->d_gtab, ctx->d_kd,
-                      ctx->need_x, ctx->d_rot, world, s);
+                      ctx->d_conv, ctx->need_x, ctx->d_rot, world, s);
 }
 
 hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const int* order,
@@ -444,6 +509,16 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
   if (e == hipSuccess) e = hipMemcpy(c->d_Wf, Wf, 4 * g * g * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_Wi, Wi, 4 * g * g * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(c->d_grids, 0, cells * sizeof(long long));
+  if (e == hipSuccess && cfg->solver == MOG_SOLVER_CONVOLUTION &&
+      cfg->compat == MOG_COMPAT_PHYSICAL) {
+    // the reference runs the FFT solver for either key (main.py:671-672);
+    // the real-space convolution is the physical mode's meaning of the key
+    std::vector<float2> conv(2 * g * g * g);
+    build_conv_tables(cfg, conv.data());
+    if (dalloc(&c->d_conv, 2 * g * g * g) != hipSuccess) e = hipErrorOutOfMemory;
+    if (e == hipSuccess)
+      e = hipMemcpy(c->d_conv, conv.data(), conv.size() * sizeof(float2), hipMemcpyHostToDevice);
+  }
   free(Wf);
   free(Wi);
   free(tw);

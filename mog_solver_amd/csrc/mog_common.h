@@ -98,16 +98,19 @@ hipError_t launch_permute(const float4* pos, const int* order, long long n, floa
 hipError_t launch_gather_cached(const Params& P, float4* pos, float4* vel, long long n,
                                 const int* order, const unsigned short* cells,
                                 const float4* force, hipStream_t s);
+// conv != NULL: multiply by the spectrum of a real-space kernel instead of
+// the Green's function x i*k (conv[0..g^3) = T_y + i T_z, conv[g^3..) = T_x).
 hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
                         float4* force, const float2* tw, const float* gtab, const float* kd,
-                        int need_x, const float* rot, int world, hipStream_t s);
+                        const float2* conv, int need_x, const float* rot, int world,
+                        hipStream_t s);
 // MFMA spectral solve (g >= 16): precision 0 FP32, 1 BF16X3, 2 FP16.
 // Wf / Wi: forward / inverse DFT tables, each [Re W | -Im W] then
 // [Im W | Re W] as g rows of 2g floats.
 hipError_t launch_solve_mfma(const Params& P, int precision, const long long* grids, float2* S1,
                              float2* S2, float4* force, const float* Wf, const float* Wi,
-                             const float* gtab, const float* kd, int need_x, const float* rot,
-                             int world, hipStream_t s);
+                             const float* gtab, const float* kd, const float2* conv, int need_x,
+                             const float* rot, int world, hipStream_t s);
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
                                 const float* rot, const float* off, const float* frot,
                                 const float4* force, int world, float4* avg_out, int mode,

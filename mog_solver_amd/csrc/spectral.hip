@@ -77,8 +77,9 @@ __global__ __launch_bounds__(256) void spec_z_green_kernel(float2* __restrict__ 
                                                            float2* __restrict__ S2,
                                                            const float2* __restrict__ tw,
                                                            const float* __restrict__ gtab,
-                                                           const float* __restrict__ kd, int g,
-                                                           int lg, int need_x) {
+                                                           const float* __restrict__ kd,
+                                                           const float2* __restrict__ conv,
+                                                           int g, int lg, int need_x) {
   extern __shared__ float2 sm[];
   float2* twl = sm;
   float2* L = sm + g;           // [z][kx]
@@ -103,7 +104,15 @@ __global__ __launch_bounds__(256) void spec_z_green_kernel(float2* __restrict__ 
       acc.x += v.x;
       acc.y += v.y;
     }
-    const float gf = gtab[(kz * g + ky) * g + kx];
+    const int fi = (kz * g + ky) * g + kx;
+    if (conv) {
+      // real-space kernel spectrum: Z = rho * (T_y + i T_z), X = rho * T_x
+      const float2 t = conv[fi], u = conv[g * gg + fi];
+      Z[o] = make_float2(fmaf(acc.x, t.x, -(acc.y * t.y)), fmaf(acc.x, t.y, acc.y * t.x));
+      X[o] = make_float2(fmaf(acc.x, u.x, -(acc.y * u.y)), fmaf(acc.x, u.y, acc.y * u.x));
+      continue;
+    }
+    const float gf = gtab[fi];
     const float pr = acc.x * gf, pi = acc.y * gf;
     // Z = phi * (-kz' + i ky')
     const float kzv = kd[kz];
@@ -205,7 +214,8 @@ __global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restri
 
 hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
                         float4* force, const float2* tw, const float* gtab, const float* kd,
-                        int need_x, const float* rot, int world, hipStream_t s) {
+                        const float2* conv, int need_x, const float* rot, int world,
+                        hipStream_t s) {
   const int g = P.g, lg = P.lg, gg = g * g;
   const unsigned slabs = (unsigned)(P.k * g);
   const size_t ldsA = (size_t)(g + (gg + 1) / 2 + gg) * sizeof(float2);
@@ -223,8 +233,8 @@ hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, flo
     return e;
   hipLaunchKernelGGL(spec_fwd_xy_kernel, dim3(slabs), dim3(256), ldsA, s, grids, S1, tw, g, lg);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(spec_z_green_kernel, dim3(slabs), dim3(256), ldsB, s, S1, S2, tw, gtab, kd, g,
-                     lg, need_x);
+  hipLaunchKernelGGL(spec_z_green_kernel, dim3(slabs), dim3(256), ldsB, s, S1, S2, tw, gtab, kd,
+                     conv, g, lg, need_x);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(spec_inv_xy_kernel, dim3(slabs), dim3(256), ldsC, s, S1, S2, force, tw, g,
                      lg, need_x, rot, world);

@@ -195,8 +195,9 @@ __global__ __launch_bounds__(256) void mfma_z_green_kernel(float2* __restrict__ 
                                                            const float* __restrict__ Wf,
                                                            const float* __restrict__ Wi,
                                                            const float* __restrict__ gtab,
-                                                           const float* __restrict__ kd, int g,
-                                                           int lg, int need_x) {
+                                                           const float* __restrict__ kd,
+                                                           const float2* __restrict__ conv,
+                                                           int g, int lg, int need_x) {
   extern __shared__ float sm[];
   const int ld = g + 1, plane = g * ld;
   float* L = sm;              // [z][kx]
@@ -223,7 +224,18 @@ __global__ __launch_bounds__(256) void mfma_z_green_kernel(float2* __restrict__ 
   const float kyv = kd[ky];
   for (int i = threadIdx.x; i < gg; i += blockDim.x) {
     const int kz = i >> lg, kx = i & m;
-    const float gf = gtab[(kz * g + ky) * g + kx];
+    const int fi = (kz * g + ky) * g + kx;
+    if (conv) {
+      // real-space kernel spectrum: Z = rho * (T_y + i T_z), X = rho * T_x
+      const float ar = R[kz * ld + kx], ai = R[kz * ld + kx + plane];
+      const float2 t = conv[fi], u = conv[g * gg + fi];
+      Z[kz * ld + kx] = fmaf(ar, t.x, -(ai * t.y));
+      Z[kz * ld + kx + plane] = fmaf(ar, t.y, ai * t.x);
+      X[kz * ld + kx] = fmaf(ar, u.x, -(ai * u.y));
+      X[kz * ld + kx + plane] = fmaf(ar, u.y, ai * u.x);
+      continue;
+    }
+    const float gf = gtab[fi];
     const float pr = R[kz * ld + kx] * gf, pi = R[kz * ld + kx + plane] * gf;
     const float kzv = kd[kz];
     Z[kz * ld + kx] = fmaf(-pr, kzv, -(pi * kyv));
@@ -325,8 +337,8 @@ __global__ __launch_bounds__(256) void mfma_inv_xy_kernel(const float2* __restri
 template <int PREC>
 static hipError_t launch_mfma_prec(const Params& P, const long long* grids, float2* S1,
                                    float2* S2, float4* force, const float* Wf, const float* Wi,
-                                   const float* gtab, const float* kd, int need_x,
-                                   const float* rot, int world, hipStream_t s) {
+                                   const float* gtab, const float* kd, const float2* conv,
+                                   int need_x, const float* rot, int world, hipStream_t s) {
   const int g = P.g, lg = P.lg;
   const unsigned slabs = (unsigned)(P.k * g);
   const int threads = g == 16 ? 64 : 256;
@@ -347,7 +359,7 @@ static hipError_t launch_mfma_prec(const Params& P, const long long* grids, floa
                      g, lg);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(mfma_z_green_kernel<PREC>, dim3(slabs), dim3(threads), ldsB, s, S1, S2, Wf, Wi,
-                     gtab, kd, g, lg, need_x);
+                     gtab, kd, conv, g, lg, need_x);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(mfma_inv_xy_kernel<PREC>, dim3(slabs), dim3(threads), ldsC, s, S1, S2, force,
                      Wi, g, lg, need_x, rot, world);
@@ -356,17 +368,17 @@ static hipError_t launch_mfma_prec(const Params& P, const long long* grids, floa
 
 hipError_t launch_solve_mfma(const Params& P, int precision, const long long* grids, float2* S1,
                              float2* S2, float4* force, const float* Wf, const float* Wi,
-                             const float* gtab, const float* kd, int need_x, const float* rot,
-                             int world, hipStream_t s) {
+                             const float* gtab, const float* kd, const float2* conv, int need_x,
+                             const float* rot, int world, hipStream_t s) {
   switch (precision) {
     case PREC_BF16X3:
-      return launch_mfma_prec<PREC_BF16X3>(P, grids, S1, S2, force, Wf, Wi, gtab, kd, need_x, rot,
+      return launch_mfma_prec<PREC_BF16X3>(P, grids, S1, S2, force, Wf, Wi, gtab, kd, conv, need_x, rot,
                                            world, s);
     case PREC_FP16:
-      return launch_mfma_prec<PREC_FP16>(P, grids, S1, S2, force, Wf, Wi, gtab, kd, need_x, rot,
+      return launch_mfma_prec<PREC_FP16>(P, grids, S1, S2, force, Wf, Wi, gtab, kd, conv, need_x, rot,
                                          world, s);
     default:
-      return launch_mfma_prec<PREC_FP32>(P, grids, S1, S2, force, Wf, Wi, gtab, kd, need_x, rot,
+      return launch_mfma_prec<PREC_FP32>(P, grids, S1, S2, force, Wf, Wi, gtab, kd, conv, need_x, rot,
                                          world, s);
   }
 }

@@ -36,6 +36,10 @@ listed per function below and exercised in ``tests/test_oracle.py``:
   (zero net momentum), the buffer-aliasing behaviour of the reference.
 * ``gather``/``update`` -- identity reductions to single-grid PM, zero-force
   closed form, rotated uniform field.
+* ``solve_convolution`` -- brute-force pair sum at small g, the lone-mass
+  closed form ``G voxel^3 / 2 * m r / |r|^3``, zero net momentum for a
+  density confined to less than half the box, and agreement with the FFT
+  solver's force at separations of several cells.
 """
 from __future__ import annotations
 
@@ -371,7 +375,7 @@ def fft3_radix2(x, direction, flush=False):
     return x
 
 
-def solve(grids, g, W, G, compat="reference", fft="numpy", flush=False):
+def solve(grids, g, W, G, compat="reference", fft="numpy", flush=False, solver="fft"):
     """Force grids (k,g,g,g,4) float64 = (Fx, Fy, Fz, |F|) from milli-mass.
 
     * ``rho = grid / 1000`` (complex.glsl:23)
@@ -385,7 +389,14 @@ def solve(grids, g, W, G, compat="reference", fft="numpy", flush=False):
 
     ``fft="numpy"`` uses ``numpy.fft`` (FP64); ``fft="radix2"`` restates the
     reference's stage structure, optionally with its sub-1e-6 flush.
+
+    ``solver="convolution"``: the reference runs the FFT solver whatever the
+    key says (``step`` calls ``step_fft``, main.py:671-672), so with
+    ``compat="reference"`` the result is the FFT path's; with
+    ``compat="physical"`` it is ``solve_convolution``.
     """
+    if solver == "convolution" and compat == "physical":
+        return solve_convolution(grids, g, W, G)
     grids = np.asarray(grids)
     k = grids.shape[0]
     rho = grids.astype(np.float64) / 1000.0
@@ -408,6 +419,60 @@ def solve(grids, g, W, G, compat="reference", fft="numpy", flush=False):
         out[..., d] = real
     if compat == "reference" and aliases_x_with_z(g):
         out[..., 0] = out[..., 2]
+    out[..., 3] = np.sqrt(out[..., 0] ** 2 + out[..., 1] ** 2 + out[..., 2] ** 2)
+    return out
+
+
+def convolution_kernel32(g, W):
+    """Green's-gradient kernel of ``init_greens_kernel`` (main.py:250-278):
+    ``K[iz, iy, ix, d] = r_d / (4 pi |r|^3)`` with the minimum-image offsets
+    ``i if i <= g/2 else i - g`` in units of ``voxel = W / g``, zero at the
+    origin, float32 values (the reference's ``"f4"`` array)."""
+    voxel = W / g
+    s = np.arange(g)
+    s = np.where(s <= g // 2, s, s - g).astype(np.float64)
+    dz, dy, dx = np.meshgrid(s, s, s, indexing="ij")
+    rx, ry, rz = dx * voxel, dy * voxel, dz * voxel
+    r2 = rx * rx + ry * ry + rz * rz
+    r = np.sqrt(r2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        factor = np.where(r > 1e-10, 1.0 / (4.0 * np.pi * r2 * r), 0.0)
+    K = np.zeros((g, g, g, 3), dtype=np.float32)
+    K[..., 0] = rx * factor
+    K[..., 1] = ry * factor
+    K[..., 2] = rz * factor
+    return K
+
+
+def solve_convolution(grids, g, W, G, method="fft"):
+    """``solver: convolution`` as a direct periodic real-space convolution of
+    the density (mass per cell, ``grid / 1000``) with ``convolution_kernel32``:
+    ``F_d(c) = 2 pi G voxel^3 sum_c' rho(c') K_d(c - c')`` -- the coupling of
+    the spectral solve (grad phi with del^2 phi = 2 pi G rho), so a lone mass
+    m gives ``G voxel^3 / 2 * m * r / |r|^3`` at every other cell.
+
+    ``method="fft"`` evaluates the circular convolution with FP64 numpy.fft;
+    ``method="direct"`` sums over all cell pairs (O(g^6), for small g).
+    """
+    grids = np.asarray(grids)
+    k = grids.shape[0]
+    rho = grids.astype(np.float64) / 1000.0
+    K = convolution_kernel32(g, W).astype(np.float64)
+    scale = 2.0 * np.pi * G * (W / g) ** 3
+    out = np.zeros((k, g, g, g, 4), dtype=np.float64)
+    if method == "fft":
+        rho_hat = np.fft.fftn(rho, axes=(1, 2, 3))
+        for d in range(3):
+            k_hat = np.fft.fftn(K[..., d])
+            out[..., d] = scale * np.fft.ifftn(rho_hat * k_hat[None], axes=(1, 2, 3)).real
+    else:
+        idx = np.arange(g)
+        for i in range(k):
+            for z, y, x in np.ndindex(g, g, g):
+                dz = (z - idx)[:, None, None] % g
+                dy = (y - idx)[None, :, None] % g
+                dx = (x - idx)[None, None, :] % g
+                out[i, z, y, x, :3] = scale * np.einsum("zyx,zyxd->d", rho[i], K[dz, dy, dx])
     out[..., 3] = np.sqrt(out[..., 0] ** 2 + out[..., 1] ** 2 + out[..., 2] ** 2)
     return out
 
@@ -478,7 +543,8 @@ def update32(pos, vel, avg_force, dt, damping, world_size, timescale=1.0):
 # ---------------------------------------------------------------------------
 
 
-def forces(pos, rot, off, n_grids, grid_size, world_size, G, compat="reference", fft="numpy", flush=False):
+def forces(pos, rot, off, n_grids, grid_size, world_size, G, compat="reference", fft="numpy", flush=False,
+           solver="fft"):
     """FP64 ``avgForce`` (n,3) of the whole MOG pipeline, processed grid by
     grid (each grid's solve depends only on its own density), O(n) memory.
 
@@ -499,7 +565,7 @@ def forces(pos, rot, off, n_grids, grid_size, world_size, G, compat="reference",
         else:
             gi = _bincount_i64(idx, milli, g**3)
         grids[i] = gi.reshape(g, g, g)
-        F = solve(grids[i : i + 1], g, world_size, G, compat=compat, fft=fft, flush=flush)[0]
+        F = solve(grids[i : i + 1], g, world_size, G, compat=compat, fft=fft, flush=flush, solver=solver)[0]
         acc += F.reshape(-1, 4)[idx, :3] @ Ms[i].T
     return acc / P.k, grids
 

@@ -17,11 +17,12 @@ FORCE_TOL = 1e-5
 GRID_TOL = 2e-6
 
 
-def make_solver(k, g, W=100.0, G=100.0, dt=1e-5, damping=0.999, compat="reference", seed=0, precision="fp32"):
+def make_solver(k, g, W=100.0, G=100.0, dt=1e-5, damping=0.999, compat="reference", seed=0, precision="fp32",
+                solver="fft"):
     from mog_solver_amd.solver import MogSolver
 
     cfg = SolverConfig.from_dict(
-        dict(n_grids=k, grid_size=g, world_size=W, G=G, dt=dt, damping=damping, solver="fft", compat=compat,
+        dict(n_grids=k, grid_size=g, world_size=W, G=G, dt=dt, damping=damping, solver=solver, compat=compat,
              precision=precision)
     )
     rot, off = O.se3_transforms(k, g, W, seed)
@@ -114,6 +115,55 @@ def test_solve_matches_fp64(torch_cuda, g, compat):
     assert mag < 1e-5
     if compat == "reference" and O.aliases_x_with_z(g):
         assert np.array_equal(got[..., 0], got[..., 2])
+
+
+def clustered_grids(k, g, W, seed, n=40000):
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = np.mod(rng.normal(50, 12, (n, 3)), W)
+    pos[:, 3] = 1.0
+    return pos
+
+
+# solver: convolution.  compat physical: the real-space Green's-gradient
+# convolution (main.py:250-278 kernel) against the FP64 oracle, on the VALU
+# (g < 16) and matrix-core (g >= 16) transforms; compat reference: the key
+# runs the FFT solver, bit-identically.
+@pytest.mark.parametrize("g,k,mfma", [(8, 4, False), (16, 3, True), (16, 3, False), (32, 2, True), (64, 1, True)])
+def test_convolution_solve_matches_fp64(torch_cuda, g, k, mfma):
+    torch = torch_cuda
+    W, G = 100.0, 100.0
+    pos = clustered_grids(k, g, W, seed=g + k)
+    s, rot, off = make_solver(k, g, W, G, compat="physical", solver="convolution")
+    s.set_tuning(spectral_mfma=int(mfma))
+    grids = O.deposit(pos, rot, off, O.Params.make(k, g, W))
+    got = s.solve(dev(torch, grids)).cpu().numpy().astype(np.float64)
+    want = O.solve(grids, g, W, G, compat="physical", solver="convolution")
+    for i in range(k):
+        err = O.rel_rms(got[i, ..., :3].reshape(-1, 3), want[i, ..., :3].reshape(-1, 3))
+        assert err < GRID_TOL, f"grid {i}: rel rms {err}"
+    fft = O.solve(grids, g, W, G, compat="physical")
+    assert O.rel_rms(want[0, ..., :3].reshape(-1, 3), fft[0, ..., :3].reshape(-1, 3)) > 10 * GRID_TOL
+
+
+def test_convolution_key_under_reference_compat_is_fft(torch_cuda):
+    torch = torch_cuda
+    k, g, W = 4, 16, 100.0
+    pos = dev(torch, clustered_grids(k, g, W, seed=3, n=50000))
+    a, _, _ = make_solver(k, g, W, solver="convolution")
+    b, _, _ = make_solver(k, g, W, solver="fft")
+    fa, fb = a.forces(pos).cpu().numpy(), b.forces(pos).cpu().numpy()
+    assert np.array_equal(fa, fb)
+
+
+def test_convolution_forces_match_oracle(torch_cuda):
+    torch = torch_cuda
+    k, g, W, G = 6, 16, 100.0, 100.0
+    pos = clustered_grids(k, g, W, seed=8, n=30000)
+    s, rot, off = make_solver(k, g, W, G, compat="physical", solver="convolution")
+    got = s.forces(dev(torch, pos)).cpu().numpy()
+    want, _ = O.forces(pos, rot, off, k, g, W, G, compat="physical", solver="convolution")
+    assert O.rel_rms(got, want) < FORCE_TOL
 
 
 # Split-precision matrix-core DFT: per-grid force error against FP64 and the
