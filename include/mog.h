@@ -200,6 +200,39 @@ int mog_set_tuning(mog_ctx* ctx, const mog_tuning* t);
 const int64_t* mog_grids_ptr(mog_ctx* ctx);
 const float* mog_force_ptr(mog_ctx* ctx);
 
+/* ---- Batched ensemble (no counterpart in the reference) -----------------
+ * n_sims independent simulations stepped together: one persistent launch
+ * per phase (deposit, three solve passes, gather+update) over all of them,
+ * optionally replayed as one hipGraph.  Simulation s has its own config
+ * (grid_size <= 32, fp32; any n_grids, compat, solver, integrator) and
+ * particle count n[s]; its particles are rows [off_s, off_s + n[s]) of the
+ * packed pos/vel arrays, off_s = n[0] + ... + n[s-1].  Stepping the batch
+ * gives the same state, bit for bit, as stepping each simulation alone with
+ * the FP32 vector spectral solve.  Errors are reported through
+ * mog_last_error(NULL). */
+typedef struct mog_batch mog_batch;
+
+int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, mog_batch** out);
+void mog_batch_destroy(mog_batch* batch);
+
+/* Transforms of simulation `sim` (host arrays, copied), as mog_set_transforms. */
+int mog_batch_set_transforms(mog_batch* batch, int32_t sim, const float* rot_k12,
+                             const float* off_k4);
+
+/* One step of every simulation (async on `stream`). */
+int mog_batch_step(mog_batch* batch, float* pos_n4, float* vel_n4, void* stream);
+
+/* nsteps steps; use_graph != 0 captures the step once per (pos, vel) and
+ * replays it. */
+int mog_batch_run(mog_batch* batch, float* pos_n4, float* vel_n4, int32_t nsteps,
+                  int32_t use_graph, void* stream);
+
+/* Deposit task shape: particles per task (256 .. 2^24) and threads per
+ * workgroup (256, 512, 1024).  Rebuilds the task list, drops the graph. */
+int mog_batch_set_tuning(mog_batch* batch, int64_t deposit_chunk, int32_t deposit_threads);
+int mog_batch_get_tuning(const mog_batch* batch, int64_t* deposit_chunk, int32_t* deposit_threads,
+                         int32_t* n_tasks);
+
 /* Message of the last error of `ctx` (or of the last failed mog_create when
  * ctx is NULL).  Never NULL. */
 const char* mog_last_error(const mog_ctx* ctx);

@@ -48,6 +48,13 @@ EXPORTED = (
     "mog_set_tuning",
     "mog_grids_ptr",
     "mog_force_ptr",
+    "mog_batch_create",
+    "mog_batch_destroy",
+    "mog_batch_set_transforms",
+    "mog_batch_step",
+    "mog_batch_run",
+    "mog_batch_set_tuning",
+    "mog_batch_get_tuning",
     "mog_last_error",
 )
 
@@ -165,6 +172,20 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     lib.mog_grids_ptr.argtypes = [vp]
     lib.mog_force_ptr.restype = vp
     lib.mog_force_ptr.argtypes = [vp]
+    lib.mog_batch_create.restype = ctypes.c_int
+    lib.mog_batch_create.argtypes = [ctypes.POINTER(MogConfig), vp, i32, ctypes.POINTER(vp)]
+    lib.mog_batch_destroy.restype = None
+    lib.mog_batch_destroy.argtypes = [vp]
+    lib.mog_batch_set_transforms.restype = ctypes.c_int
+    lib.mog_batch_set_transforms.argtypes = [vp, i32, vp, vp]
+    lib.mog_batch_step.restype = ctypes.c_int
+    lib.mog_batch_step.argtypes = [vp, vp, vp, vp]
+    lib.mog_batch_run.restype = ctypes.c_int
+    lib.mog_batch_run.argtypes = [vp, vp, vp, i32, i32, vp]
+    lib.mog_batch_set_tuning.restype = ctypes.c_int
+    lib.mog_batch_set_tuning.argtypes = [vp, i64, i32]
+    lib.mog_batch_get_tuning.restype = ctypes.c_int
+    lib.mog_batch_get_tuning.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.mog_last_error.restype = ctypes.c_char_p
     lib.mog_last_error.argtypes = [vp]
     if path is None:

@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <complex>
 #include <string>
 #include <vector>
@@ -455,6 +456,155 @@ hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const i
   if (e != hipSuccess) return e;
   return launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
                               ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s);
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched ensemble
+// ---------------------------------------------------------------------------
+}  // namespace
+
+struct mog_batch {
+  int nsims;
+  std::vector<mog_config> cfg;
+  std::vector<BatchSim> sims;  // host copy of the device descriptor table
+  std::vector<char> have_transforms;
+  long long ntotal, cells_total, slabs_total, gather_blocks;
+  int grids_total, gmax;
+  BatchSim* d_sims;
+  float* d_rot;
+  float* d_off;
+  float* d_frot;
+  long long* d_grids;
+  float2* d_S1;
+  float2* d_S2;
+  float4* d_force;
+  float2* d_tw;
+  float* d_gtab;
+  float* d_kd;
+  float2* d_conv;
+  DepositTask* d_tasks;
+  int ntasks;
+  unsigned long long* d_counter;
+  // visiting order of the gather (Morton order inside each simulation),
+  // rebuilt every `resort_every` steps outside any captured graph
+  int* d_order;
+  int* d_idx;
+  unsigned long long* d_keys_a;
+  unsigned long long* d_keys_b;
+  void* d_sort_tmp;
+  size_t sort_tmp_bytes;
+  const void* order_pos;
+  int steps_since_order;
+  int resort_every;
+  long long deposit_chunk;
+  int deposit_threads;
+  hipStream_t cap_stream;
+  hipGraphExec_t gexec;
+  const void* g_pos;
+  const void* g_vel;
+};
+
+namespace {
+
+void batch_release(mog_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  void* ptrs[] = {b->d_sims, b->d_rot,  b->d_off, b->d_frot, b->d_grids, b->d_S1,
+                  b->d_S2,   b->d_force, b->d_tw, b->d_gtab, b->d_kd,    b->d_conv,
+                  b->d_tasks, b->d_counter, b->d_order, b->d_idx, b->d_keys_a, b->d_keys_b,
+                  b->d_sort_tmp};
+  for (void* q : ptrs) (void)hipFree(q);
+}
+
+void batch_drop_graph(mog_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+}
+
+// Deposit tasks: every simulation is cut into particle chunks of
+// `chunk` x groups of grids whose uint32 histograms fit kBatchLdsCells;
+// ordered by cost (pairs + histogram cells), largest first.
+hipError_t batch_build_tasks(mog_batch* b) {
+  std::vector<DepositTask> tasks;
+  std::vector<double> cost;
+  for (int si = 0; si < b->nsims; ++si) {
+    const BatchSim& sm = b->sims[si];
+    if (sm.n == 0) continue;
+    const int cells = sm.P.cells;
+    const int per = std::max(1, std::min(sm.P.k, kBatchLdsCells / cells));
+    for (long long p0 = 0; p0 < sm.n; p0 += b->deposit_chunk)
+      for (int g0 = 0; g0 < sm.P.k; g0 += per) {
+        DepositTask t;
+        t.sim = si;
+        t.grid0 = g0;
+        t.ng = std::min(per, sm.P.k - g0);
+        t.pad = 0;
+        t.p0 = p0;
+        t.p1 = std::min(sm.n, p0 + b->deposit_chunk);
+        tasks.push_back(t);
+        cost.push_back((double)(t.p1 - t.p0) * t.ng + 0.25 * (double)t.ng * cells);
+      }
+  }
+  std::vector<size_t> idx(tasks.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t c) { return cost[a] > cost[c]; });
+  std::vector<DepositTask> sorted(tasks.size());
+  for (size_t i = 0; i < idx.size(); ++i) sorted[i] = tasks[idx[i]];
+  (void)hipFree(b->d_tasks);
+  b->d_tasks = nullptr;
+  b->ntasks = (int)sorted.size();
+  hipError_t e = dalloc(&b->d_tasks, sorted.size());
+  if (e == hipSuccess && !sorted.empty())
+    e = hipMemcpy(b->d_tasks, sorted.data(), sorted.size() * sizeof(DepositTask),
+                  hipMemcpyHostToDevice);
+  return e;
+}
+
+// Rebuild the visiting order when it is stale for `pos` (direct launches,
+// never inside a capture).
+hipError_t batch_refresh_order(mog_batch* b, const float* pos, hipStream_t s) {
+  if (!b->d_order || b->resort_every <= 0) return hipSuccess;
+  if (b->order_pos != pos || b->steps_since_order >= b->resort_every) {
+    hipError_t e = build_batch_order((const float4*)pos, b->ntotal, b->gather_blocks, b->d_sims,
+                                     b->nsims, b->d_keys_a, b->d_keys_b, b->d_idx, b->d_order,
+                                     b->d_sort_tmp, b->sort_tmp_bytes, s);
+    if (e != hipSuccess) return e;
+    b->order_pos = pos;
+    b->steps_since_order = 0;
+  }
+  b->steps_since_order++;
+  return hipSuccess;
+}
+
+const int* batch_order(const mog_batch* b) {
+  return (b->d_order && b->resort_every > 0) ? b->d_order : nullptr;
+}
+
+hipError_t batch_enqueue(mog_batch* b, float* pos, float* vel, hipStream_t s) {
+  hipError_t e = launch_fill_zero(b->d_grids, b->cells_total, s);
+  if (e == hipSuccess) e = launch_fill_zero((long long*)b->d_counter, 1, s);
+  if (e == hipSuccess)
+    e = launch_batch_deposit((const float4*)pos, b->d_sims, b->d_tasks, b->ntasks, b->d_rot,
+                             b->d_off, b->d_frot, (unsigned long long*)b->d_grids, b->d_counter,
+                             b->deposit_threads, s);
+  if (e == hipSuccess)
+    e = launch_batch_solve(b->d_sims, b->nsims, b->slabs_total, b->gmax, b->d_grids, b->d_S1,
+                           b->d_S2, b->d_force, b->d_tw, b->d_gtab, b->d_kd, b->d_conv, b->d_rot, s);
+  if (e == hipSuccess)
+    e = launch_batch_gather((float4*)pos, (float4*)vel, b->gather_blocks, b->d_sims, b->nsims,
+                            b->d_rot, b->d_off, b->d_frot, b->d_force, batch_order(b), s);
+  return e;
+}
+
+int batch_check(mog_batch* b, const float* pos, const float* vel) {
+  if (!b) return fail(nullptr, MOG_EINVAL, "batch is NULL");
+  if (b->ntotal > 0 && (!pos || !vel || !aligned16(pos) || !aligned16(vel)))
+    return fail(nullptr, MOG_EINVAL, "pos/vel must be 16-byte aligned device pointers");
+  for (int i = 0; i < b->nsims; ++i)
+    if (!b->have_transforms[i])
+      return fail(nullptr, MOG_ENOTRANSFORMS, "mog_batch_set_transforms not called for sim %d", i);
+  return MOG_OK;
 }
 
 }  // namespace
@@ -905,6 +1055,245 @@ int mog_set_tuning(mog_ctx* ctx, const mog_tuning* t) {
 const int64_t* mog_grids_ptr(mog_ctx* ctx) { return ctx ? (const int64_t*)ctx->d_grids : nullptr; }
 
 const float* mog_force_ptr(mog_ctx* ctx) { return ctx ? (const float*)ctx->d_force : nullptr; }
+
+int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, mog_batch** out) {
+  if (!out) return fail(nullptr, MOG_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!cfgs || !n) return fail(nullptr, MOG_EINVAL, "cfgs/n is NULL");
+  if (n_sims < 1 || n_sims > 65536) return fail(nullptr, MOG_EINVAL, "n_sims must be in [1, 65536]");
+  mog_batch* b = new mog_batch();
+  b->nsims = n_sims;
+  b->cfg.assign(cfgs, cfgs + n_sims);
+  b->sims.resize(n_sims);
+  b->have_transforms.assign(n_sims, 0);
+  long long p_off = 0, cell_off = 0, slab_off = 0, conv_off = 0, blk_off = 0;
+  int grid_off = 0, tab_off = 0, line_off = 0;
+  b->gmax = 0;
+  for (int i = 0; i < n_sims; ++i) {
+    const mog_config* c = &cfgs[i];
+    int rc = validate(c);
+    if (rc != MOG_OK) {
+      delete b;
+      return fail(nullptr, rc, "sim %d: %s", i, g_create_error.c_str());
+    }
+    if (c->grid_size > 32 || c->precision != MOG_PRECISION_FP32) {
+      delete b;
+      return fail(nullptr, MOG_EUNSUPPORTED, "sim %d: the batch runs grid_size <= 32 in fp32", i);
+    }
+    if (n[i] < 0) {
+      delete b;
+      return fail(nullptr, MOG_EINVAL, "sim %d: n must be >= 0", i);
+    }
+    BatchSim& sm = b->sims[i];
+    sm.P = make_params(c);
+    sm.p_off = p_off;
+    sm.n = n[i];
+    sm.cell_off = cell_off;
+    sm.slab_off = slab_off;
+    sm.grid_off = grid_off;
+    sm.tab_off = tab_off;
+    sm.line_off = line_off;
+    sm.need_x = !(c->compat == MOG_COMPAT_REFERENCE && aliases_x_with_z(c->grid_size));
+    const bool conv = c->solver == MOG_SOLVER_CONVOLUTION && c->compat == MOG_COMPAT_PHYSICAL;
+    sm.conv_off = conv ? conv_off : -1;
+    sm.blk_off = blk_off;
+    blk_off += (n[i] + kBatchGatherBlock - 1) / kBatchGatherBlock;
+    const long long g = c->grid_size, cells = (long long)c->n_grids * g * g * g;
+    p_off += n[i];
+    cell_off += cells;
+    slab_off += (long long)c->n_grids * g;
+    grid_off += c->n_grids;
+    tab_off += (int)(g * g * g);
+    line_off += (int)g;
+    if (conv) conv_off += 2 * g * g * g;
+    b->gmax = std::max(b->gmax, (int)g);
+  }
+  if (p_off > 2147483647LL || cell_off > 2147483647LL) {
+    delete b;
+    return fail(nullptr, MOG_EINVAL, "total particles and cells must stay below 2^31");
+  }
+  b->ntotal = p_off;
+  b->cells_total = cell_off;
+  b->slabs_total = slab_off;
+  b->gather_blocks = blk_off;
+  b->grids_total = grid_off;
+  b->deposit_chunk = 16384;
+  b->deposit_threads = 1024;
+  hipError_t e = hipSuccess;
+  auto A = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  A(dalloc(&b->d_sims, n_sims));
+  A(dalloc(&b->d_rot, (size_t)grid_off * 12));
+  A(dalloc(&b->d_off, (size_t)grid_off * 4));
+  A(dalloc(&b->d_frot, (size_t)grid_off * 12));
+  A(dalloc(&b->d_grids, (size_t)cell_off));
+  A(dalloc(&b->d_S1, (size_t)cell_off));
+  A(dalloc(&b->d_S2, (size_t)cell_off));
+  A(dalloc(&b->d_force, (size_t)cell_off));
+  A(dalloc(&b->d_tw, (size_t)line_off));
+  A(dalloc(&b->d_gtab, (size_t)tab_off));
+  A(dalloc(&b->d_kd, (size_t)line_off));
+  A(dalloc(&b->d_conv, (size_t)(conv_off > 0 ? conv_off : 1)));
+  A(dalloc(&b->d_counter, 1));
+  b->resort_every = 16;
+  size_t tmp = 0;
+  A(batch_order_temp_bytes(p_off, n_sims, &tmp));
+  b->sort_tmp_bytes = tmp;
+  A(dalloc(&b->d_order, (size_t)p_off));
+  A(dalloc(&b->d_idx, (size_t)p_off));
+  A(dalloc(&b->d_keys_a, (size_t)p_off));
+  A(dalloc(&b->d_keys_b, (size_t)p_off));
+  A(hipMalloc(&b->d_sort_tmp, tmp > 0 ? tmp : 16));
+  if (e != hipSuccess) {
+    batch_release(b);
+    delete b;
+    return fail(nullptr, MOG_ENOMEM, "device allocation failed for the batch (%lld cells)", cell_off);
+  }
+  std::vector<float2> tw(line_off), conv(conv_off > 0 ? conv_off : 1);
+  std::vector<float> gtab(tab_off), kd(line_off);
+  for (int i = 0; i < n_sims; ++i) {
+    const BatchSim& sm = b->sims[i];
+    build_tables(&cfgs[i], tw.data() + sm.line_off, gtab.data() + sm.tab_off, kd.data() + sm.line_off);
+    if (sm.conv_off >= 0) build_conv_tables(&cfgs[i], conv.data() + sm.conv_off);
+  }
+  A(hipMemcpy(b->d_tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+  A(hipMemcpy(b->d_gtab, gtab.data(), gtab.size() * sizeof(float), hipMemcpyHostToDevice));
+  A(hipMemcpy(b->d_kd, kd.data(), kd.size() * sizeof(float), hipMemcpyHostToDevice));
+  A(hipMemcpy(b->d_conv, conv.data(), conv.size() * sizeof(float2), hipMemcpyHostToDevice));
+  A(hipMemcpy(b->d_sims, b->sims.data(), b->sims.size() * sizeof(BatchSim), hipMemcpyHostToDevice));
+  A(batch_build_tasks(b));
+  if (e != hipSuccess) {
+    batch_release(b);
+    delete b;
+    return fail(nullptr, MOG_EHIP, "batch tables: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return MOG_OK;
+}
+
+void mog_batch_destroy(mog_batch* b) {
+  if (!b) return;
+  batch_release(b);
+  delete b;
+}
+
+int mog_batch_set_transforms(mog_batch* b, int32_t sim, const float* rot_k12, const float* off_k4) {
+  if (!b) return fail(nullptr, MOG_EINVAL, "batch is NULL");
+  if (sim < 0 || sim >= b->nsims) return fail(nullptr, MOG_EINVAL, "sim %d out of range", sim);
+  if (!rot_k12 || !off_k4) return fail(nullptr, MOG_EINVAL, "rot/off is NULL");
+  BatchSim& sm = b->sims[sim];
+  const int k = sm.P.k;
+  double rmax = 0.0, omax = 0.0;
+  bool finite = true;
+  for (int i = 0; i < k * 12; ++i) {
+    finite = finite && isfinite(rot_k12[i]);
+    rmax = fmax(rmax, fabs((double)rot_k12[i]));
+  }
+  for (int i = 0; i < k; ++i)
+    for (int m = 0; m < 3; ++m) {
+      finite = finite && isfinite(off_k4[4 * i + m]);
+      omax = fmax(omax, fabs((double)off_k4[4 * i + m]));
+    }
+  if (!finite) return fail(nullptr, MOG_EINVAL, "rotations/offsets must be finite");
+  std::vector<float> frot((size_t)k * 12);
+  for (int i = 0; i < k; ++i)
+    make_fast_row(rot_k12 + 12 * i, off_k4 + 4 * i, sm.P.half, sm.P.h, frot.data() + 12 * i);
+  const double margin = fast_margin(b->cfg[sim].world_size, sm.P.g, rmax, omax);
+  sm.P.lim = margin < 0.05 ? (float)(0.5 - margin) : -1.0f;
+  const size_t g0 = (size_t)sm.grid_off;
+  hipError_t e = hipMemcpy(b->d_rot + 12 * g0, rot_k12, (size_t)k * 12 * sizeof(float),
+                           hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_off + 4 * g0, off_k4, (size_t)k * 4 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_frot + 12 * g0, frot.data(), frot.size() * sizeof(float),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_sims + sim, &sm, sizeof(BatchSim), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(nullptr, MOG_EHIP, "mog_batch_set_transforms: %s", hipGetErrorString(e));
+  b->have_transforms[sim] = 1;
+  return MOG_OK;
+}
+
+int mog_batch_step(mog_batch* b, float* pos, float* vel, void* stream) {
+  int rc = batch_check(b, pos, vel);
+  if (rc != MOG_OK) return rc;
+  if (b->ntotal == 0) return MOG_OK;
+  hipError_t e = batch_refresh_order(b, pos, (hipStream_t)stream);
+  if (e == hipSuccess) e = batch_enqueue(b, pos, vel, (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : fail(nullptr, MOG_EHIP, "mog_batch_step: %s", hipGetErrorString(e));
+}
+
+int mog_batch_run(mog_batch* b, float* pos, float* vel, int32_t nsteps, int32_t use_graph,
+                  void* stream) {
+  int rc = batch_check(b, pos, vel);
+  if (rc != MOG_OK) return rc;
+  if (nsteps < 0) return fail(nullptr, MOG_EINVAL, "nsteps must be >= 0");
+  if (b->ntotal == 0 || nsteps == 0) return MOG_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  if (!use_graph) {
+    for (int i = 0; i < nsteps; ++i)
+      if ((e = batch_refresh_order(b, pos, s)) != hipSuccess ||
+          (e = batch_enqueue(b, pos, vel, s)) != hipSuccess)
+        return fail(nullptr, MOG_EHIP, "mog_batch_run: %s", hipGetErrorString(e));
+    return MOG_OK;
+  }
+  if (!b->gexec || b->g_pos != pos || b->g_vel != vel) {
+    batch_drop_graph(b);
+    if (!b->cap_stream &&
+        (e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking)) != hipSuccess)
+      return fail(nullptr, MOG_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    if ((e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return fail(nullptr, MOG_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    const hipError_t ee = batch_enqueue(b, pos, vel, b->cap_stream);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(b->cap_stream, &graph);
+    if (ee != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(nullptr, MOG_EHIP, "capture: %s", hipGetErrorString(ee));
+    }
+    if (e != hipSuccess) return fail(nullptr, MOG_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      b->gexec = nullptr;
+      return fail(nullptr, MOG_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    b->g_pos = pos;
+    b->g_vel = vel;
+  }
+  for (int i = 0; i < nsteps; ++i) {
+    if ((e = batch_refresh_order(b, pos, s)) != hipSuccess)
+      return fail(nullptr, MOG_EHIP, "mog_batch_run: %s", hipGetErrorString(e));
+    if ((e = hipGraphLaunch(b->gexec, s)) != hipSuccess)
+      return fail(nullptr, MOG_EHIP, "hipGraphLaunch: %s", hipGetErrorString(e));
+  }
+  return MOG_OK;
+}
+
+int mog_batch_set_tuning(mog_batch* b, int64_t deposit_chunk, int32_t deposit_threads) {
+  if (!b) return fail(nullptr, MOG_EINVAL, "batch is NULL");
+  if (deposit_chunk < 256 || deposit_chunk > (1LL << 24))
+    return fail(nullptr, MOG_EINVAL, "deposit_chunk must be in [256, 2^24]");
+  if (deposit_threads != 256 && deposit_threads != 512 && deposit_threads != 1024)
+    return fail(nullptr, MOG_EINVAL, "deposit_threads must be 256, 512 or 1024");
+  b->deposit_chunk = deposit_chunk;
+  b->deposit_threads = deposit_threads;
+  batch_drop_graph(b);
+  hipError_t e = batch_build_tasks(b);
+  return e == hipSuccess ? MOG_OK : fail(nullptr, MOG_EHIP, "mog_batch_set_tuning: %s", hipGetErrorString(e));
+}
+
+int mog_batch_get_tuning(const mog_batch* b, int64_t* deposit_chunk, int32_t* deposit_threads,
+                         int32_t* n_tasks) {
+  if (!b) return fail(nullptr, MOG_EINVAL, "batch is NULL");
+  if (deposit_chunk) *deposit_chunk = b->deposit_chunk;
+  if (deposit_threads) *deposit_threads = b->deposit_threads;
+  if (n_tasks) *n_tasks = b->ntasks;
+  return MOG_OK;
+}
 
 const char* mog_last_error(const mog_ctx* ctx) {
   if (!ctx) return g_create_error.c_str();

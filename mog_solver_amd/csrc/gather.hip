@@ -11,23 +11,6 @@
 
 namespace mog {
 
-__device__ __forceinline__ void integrate(float4& p, float4& v, float ax, float ay, float az,
-                                          const Params& P) {
-  // vel -= avgForce * deltaTime; vel *= damping; pos += vel * deltaTime;
-  // pos = mod(pos, worldSize)  (update.glsl:83-86), each op rounded.
-  float vx = v.x - ax * P.dtp;
-  float vy = v.y - ay * P.dtp;
-  float vz = v.z - az * P.dtp;
-  vx = vx * P.damping;
-  vy = vy * P.damping;
-  vz = vz * P.damping;
-  float x = p.x + vx * P.dtp;
-  float y = p.y + vy * P.dtp;
-  float z = p.z + vz * P.dtp;
-  p = make_float4(glsl_mod_f(x, P.W), glsl_mod_f(y, P.W), glsl_mod_f(z, P.W), p.w);
-  v = make_float4(vx, vy, vz, 0.0f);
-}
-
 // t += M F for a grid-frame force F of grid R: world_m = sum_j R[4m+j] F_j
 // (transpose(rot) * gridForce, update.glsl:77).
 __device__ __forceinline__ void accumulate_rotated(const float* __restrict__ R, const float4 F,

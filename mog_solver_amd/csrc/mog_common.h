@@ -66,6 +66,24 @@ __device__ __forceinline__ unsigned long long milli_of(float m) {
   return mm > 0.0f ? (unsigned long long)mm : 0ull;
 }
 
+// The integrator of update.glsl:83-86, each operation rounded in order:
+// vel -= avgForce * deltaTime; vel *= damping; pos += vel * deltaTime;
+// pos = mod(pos, worldSize); stores (pos, m) and (vel, 0).
+__device__ __forceinline__ void integrate(float4& p, float4& v, float ax, float ay, float az,
+                                          const Params& P) {
+  float vx = v.x - ax * P.dtp;
+  float vy = v.y - ay * P.dtp;
+  float vz = v.z - az * P.dtp;
+  vx = vx * P.damping;
+  vy = vy * P.damping;
+  vz = vz * P.damping;
+  const float x = p.x + vx * P.dtp;
+  const float y = p.y + vy * P.dtp;
+  const float z = p.z + vz * P.dtp;
+  p = make_float4(glsl_mod_f(x, P.W), glsl_mod_f(y, P.W), glsl_mod_f(z, P.W), p.w);
+  v = make_float4(vx, vy, vz, 0.0f);
+}
+
 }  // namespace mog
 
 // Host-side launchers (defined in the kernel translation units).
@@ -142,4 +160,73 @@ hipError_t launch_adam(float* prm, const float* grad, float* m1, float* m2, floa
 hipError_t launch_update(const Params& P, float4* pos, float4* vel, const float4* avg,
                          long long n, hipStream_t s);
 
+
+// ---------------------------------------------------------------------------
+// Batched ensemble: many independent simulations in packed arrays
+// ---------------------------------------------------------------------------
+// One simulation of the ensemble and where its data live.  Particles of sim
+// s are rows [p_off, p_off + n) of the packed pos/vel; its k grids are cells
+// [cell_off, cell_off + k g^3) of the packed grids / S1 / S2 / force; its
+// transforms are rows [grid_off, grid_off + k) of rot / off / frot; its
+// solve slabs (k g per pass) are [slab_off, slab_off + k g) of the pass's
+// flat slab range.
+struct BatchSim {
+  Params P;
+  long long p_off, n;
+  long long cell_off, slab_off;
+  long long conv_off;  // float2 offset of its convolution spectrum; < 0: Green's function
+  long long blk_off;   // first gather block (ceil(n / kBatchGatherBlock) blocks per sim)
+  int grid_off;
+  int tab_off;         // gtab offset (g^3 floats)
+  int line_off;        // tw / kd offset (g entries)
+  int need_x;
+};
+
+// A deposit work item: particles [p0, p1) of `sim` into its grids
+// [grid0, grid0 + ng), privatised in LDS.
+struct DepositTask {
+  int sim, grid0, ng, pad;
+  long long p0, p1;
+};
+
+// Largest LDS histogram of a batched deposit task (uint32 counters).
+constexpr int kBatchLdsCells = 32768;
+// Particles per gather block; a block never straddles two simulations.
+constexpr int kBatchGatherBlock = 256;
+
+// The simulation owning gather block `blk` (sims ordered by blk_off); the
+// result is uniform across a workgroup that passes a uniform blk.
+__device__ __forceinline__ int sim_of_block(const BatchSim* __restrict__ sims, int nsims,
+                                            long long blk) {
+  int lo = 0, hi = nsims - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sims[mid].blk_off <= blk)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Visiting order of the batch gather: particles of each simulation sorted
+// by Morton key inside the simulation's own rows (one radix sort of
+// (sim << 30 | key) over all particles).
+hipError_t batch_order_temp_bytes(long long n, int nsims, size_t* bytes);
+hipError_t build_batch_order(const float4* pos, long long n, long long blocks,
+                             const BatchSim* sims, int nsims, unsigned long long* keys_a,
+                             unsigned long long* keys_b, int* idx_a, int* order, void* tmp,
+                             size_t tmp_bytes, hipStream_t s);
+
+hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const DepositTask* tasks,
+                                int ntasks, const float* rot, const float* off, const float* frot,
+                                unsigned long long* grids, unsigned long long* counter,
+                                int threads, hipStream_t s);
+hipError_t launch_batch_solve(const BatchSim* sims, int nsims, long long slabs, int gmax,
+                              const long long* grids, float2* S1, float2* S2, float4* force,
+                              const float2* tw, const float* gtab, const float* kd,
+                              const float2* conv, const float* rot, hipStream_t s);
+hipError_t launch_batch_gather(float4* pos, float4* vel, long long blocks, const BatchSim* sims,
+                               int nsims, const float* rot, const float* off, const float* frot,
+                               const float4* force, const int* order, hipStream_t s);
 }  // namespace mog

@@ -65,4 +65,52 @@ hipError_t build_order(const float4* pos, long long n, float W, unsigned* keys_a
   return rocprim::radix_sort_pairs(tmp, bytes, keys_a, keys_b, idx_a, order, (unsigned)n, 0, 30, s);
 }
 
+// Batch keys: (sim << 30) | Morton key in the simulation's own box, over
+// the gather blocks (simulation uniform per workgroup).
+__global__ __launch_bounds__(kBatchGatherBlock) void batch_morton_keys_kernel(
+    const float4* __restrict__ pos, long long blocks, const BatchSim* __restrict__ sims,
+    int nsims, unsigned long long* __restrict__ keys, int* __restrict__ idx) {
+  for (long long blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
+    const int si = sim_of_block(sims, nsims, blk);
+    const BatchSim& b = sims[si];
+    const long long j = (blk - b.blk_off) * kBatchGatherBlock + threadIdx.x;
+    if (j >= b.n) continue;
+    const long long i = b.p_off + j;
+    const float4 p = pos[i];
+    const float inv_w = 1.0f / b.P.W;
+    const unsigned m = spread3(quant10(p.x, inv_w)) | (spread3(quant10(p.y, inv_w)) << 1) |
+                       (spread3(quant10(p.z, inv_w)) << 2);
+    keys[i] = ((unsigned long long)si << 30) | m;
+    idx[i] = (int)i;
+  }
+}
+
+static int key_bits(int nsims) {
+  int b = 0;
+  while ((1 << b) < nsims) ++b;
+  return 30 + b;
+}
+
+hipError_t batch_order_temp_bytes(long long n, int nsims, size_t* bytes) {
+  *bytes = 0;
+  return rocprim::radix_sort_pairs((void*)nullptr, *bytes, (const unsigned long long*)nullptr,
+                                   (unsigned long long*)nullptr, (const int*)nullptr, (int*)nullptr,
+                                   (unsigned)n, 0, key_bits(nsims), (hipStream_t)0);
+}
+
+hipError_t build_batch_order(const float4* pos, long long n, long long blocks,
+                             const BatchSim* sims, int nsims, unsigned long long* keys_a,
+                             unsigned long long* keys_b, int* idx_a, int* order, void* tmp,
+                             size_t tmp_bytes, hipStream_t s) {
+  if (n <= 0 || blocks <= 0) return hipSuccess;
+  const long long grid = blocks < 16384 ? blocks : 16384;
+  hipLaunchKernelGGL(batch_morton_keys_kernel, dim3((unsigned)grid), dim3(kBatchGatherBlock), 0, s,
+                     pos, blocks, sims, nsims, keys_a, idx_a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  size_t bytes = tmp_bytes;
+  return rocprim::radix_sort_pairs(tmp, bytes, keys_a, keys_b, idx_a, order, (unsigned)n, 0,
+                                   key_bits(nsims), s);
+}
+
 }  // namespace mog

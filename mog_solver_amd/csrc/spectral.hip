@@ -32,17 +32,19 @@ __device__ __forceinline__ float2 cmul_tw(float2 a, float2 w) {
   return make_float2(fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.y, w.x, a.x * w.y));
 }
 
-// Pass A: one workgroup per (grid, z) slab.
-__global__ __launch_bounds__(256) void spec_fwd_xy_kernel(const long long* __restrict__ mass,
-                                                          float2* __restrict__ S,
-                                                          const float2* __restrict__ tw, int g,
-                                                          int lg) {
+// Pass A on slab `slab` = (grid, z) of one simulation's arrays.  The pass
+// bodies are device functions so that the batched ensemble's persistent
+// kernels run exactly the same arithmetic as the single-simulation launch.
+__device__ __forceinline__ void spec_fwd_xy_slab(const long long* __restrict__ mass,
+                                                 float2* __restrict__ S,
+                                                 const float2* __restrict__ tw, int g, int lg,
+                                                 long long slab) {
   extern __shared__ float2 sm[];
   float2* twl = sm;
   float* re = (float*)(sm + g);          // g*g reals
   float2* B = sm + g + (g * g + 1) / 2;  // g*g complex
   const int gg = g * g, m = g - 1;
-  const long long base = (long long)blockIdx.x * gg;
+  const long long base = slab * gg;
   for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < gg; i += blockDim.x) re[i] = (float)mass[base + i];
   __syncthreads();
@@ -72,22 +74,29 @@ __global__ __launch_bounds__(256) void spec_fwd_xy_kernel(const long long* __res
   }
 }
 
-// Pass B: one workgroup per (grid, ky): columns along z for every kx.
-__global__ __launch_bounds__(256) void spec_z_green_kernel(float2* __restrict__ S,
-                                                           float2* __restrict__ S2,
-                                                           const float2* __restrict__ tw,
-                                                           const float* __restrict__ gtab,
-                                                           const float* __restrict__ kd,
-                                                           const float2* __restrict__ conv,
-                                                           int g, int lg, int need_x) {
+// Pass A: one workgroup per (grid, z) slab.
+__global__ __launch_bounds__(256) void spec_fwd_xy_kernel(const long long* __restrict__ mass,
+                                                          float2* __restrict__ S,
+                                                          const float2* __restrict__ tw, int g,
+                                                          int lg) {
+  spec_fwd_xy_slab(mass, S, tw, g, lg, blockIdx.x);
+}
+
+// Pass B on slab (grid, ky): columns along z for every kx.
+__device__ __forceinline__ void spec_z_green_slab(float2* __restrict__ S, float2* __restrict__ S2,
+                                                  const float2* __restrict__ tw,
+                                                  const float* __restrict__ gtab,
+                                                  const float* __restrict__ kd,
+                                                  const float2* __restrict__ conv, int g, int lg,
+                                                  int need_x, long long slab) {
   extern __shared__ float2 sm[];
   float2* twl = sm;
   float2* L = sm + g;           // [z][kx]
   float2* Z = L + g * g;        // [kz][kx]  (y + i z spectrum)
   float2* X = Z + g * g;        // [kz][kx]  (x spectrum)
   const int gg = g * g, m = g - 1;
-  const int grid = blockIdx.x >> lg;
-  const int ky = blockIdx.x & m;
+  const int grid = (int)(slab >> lg);
+  const int ky = (int)(slab & m);
   const long long gbase = (long long)grid * g * gg;
   for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < gg; i += blockDim.x) {
@@ -143,16 +152,26 @@ __global__ __launch_bounds__(256) void spec_z_green_kernel(float2* __restrict__ 
   }
 }
 
-// Pass C: one workgroup per (grid, z): inverse y, inverse x, pack float4.
+// Pass B: one workgroup per (grid, ky).
+__global__ __launch_bounds__(256) void spec_z_green_kernel(float2* __restrict__ S,
+                                                           float2* __restrict__ S2,
+                                                           const float2* __restrict__ tw,
+                                                           const float* __restrict__ gtab,
+                                                           const float* __restrict__ kd,
+                                                           const float2* __restrict__ conv,
+                                                           int g, int lg, int need_x) {
+  spec_z_green_slab(S, S2, tw, gtab, kd, conv, g, lg, need_x, blockIdx.x);
+}
+
+// Pass C on slab (grid, z): inverse y, inverse x, pack float4.
 // world != 0 stores M F (the grid's force rotated to the world frame,
 // world_m = sum_j R[4m+j] F_j) for the step's gather; |F| is unchanged.
-__global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restrict__ S,
-                                                          const float2* __restrict__ S2,
-                                                          float4* __restrict__ force,
-                                                          const float2* __restrict__ tw, int g,
-                                                          int lg, int need_x,
-                                                          const float* __restrict__ rot,
-                                                          int world) {
+__device__ __forceinline__ void spec_inv_xy_slab(const float2* __restrict__ S,
+                                                 const float2* __restrict__ S2,
+                                                 float4* __restrict__ force,
+                                                 const float2* __restrict__ tw, int g, int lg,
+                                                 int need_x, const float* __restrict__ rot,
+                                                 int world, long long slab) {
   extern __shared__ float2 sm[];
   float2* twl = sm;
   float2* Z = sm + g;       // [ky][kx]
@@ -160,7 +179,7 @@ __global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restri
   float2* TZ = X + g * g;   // [y][kx]
   float2* TX = TZ + g * g;  // [y][kx]
   const int gg = g * g, m = g - 1;
-  const long long base = (long long)blockIdx.x * gg;
+  const long long base = slab * gg;
   for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < gg; i += blockDim.x) {
     Z[i] = S[base + i];
@@ -201,7 +220,7 @@ __global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restri
     if (!need_x) fx = fz;
     const float mag = sqrtf(fmaf(fx, fx, fmaf(fy, fy, fz * fz)));
     if (world) {
-      const float* R = rot + 12 * (blockIdx.x >> lg);
+      const float* R = rot + 12 * (slab >> lg);
       const float wx = fmaf(R[0], fx, fmaf(R[1], fy, R[2] * fz));
       const float wy = fmaf(R[4], fx, fmaf(R[5], fy, R[6] * fz));
       const float wz = fmaf(R[8], fx, fmaf(R[9], fy, R[10] * fz));
@@ -210,6 +229,116 @@ __global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restri
       force[base + o] = make_float4(fx, fy, fz, mag);
     }
   }
+}
+
+// Pass C: one workgroup per (grid, z).
+__global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restrict__ S,
+                                                          const float2* __restrict__ S2,
+                                                          float4* __restrict__ force,
+                                                          const float2* __restrict__ tw, int g,
+                                                          int lg, int need_x,
+                                                          const float* __restrict__ rot,
+                                                          int world) {
+  spec_inv_xy_slab(S, S2, force, tw, g, lg, need_x, rot, world, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched ensemble: the three passes as persistent kernels over the flat
+// slab range of all simulations (sims ordered by slab_off).  Each workgroup
+// strides over slabs; a slab runs the same body as the single-simulation
+// launch, with the simulation's own g, tables and buffer offsets.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int sim_of_slab(const BatchSim* __restrict__ sims, int nsims,
+                                           long long t) {
+  int lo = 0, hi = nsims - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sims[mid].slab_off <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void batch_fwd_xy_kernel(const BatchSim* __restrict__ sims,
+                                                           int nsims, long long slabs,
+                                                           const long long* __restrict__ mass,
+                                                           float2* __restrict__ S,
+                                                           const float2* __restrict__ tw) {
+  for (long long t = blockIdx.x; t < slabs; t += gridDim.x) {
+    const int si = sim_of_slab(sims, nsims, t);
+    const BatchSim& b = sims[si];
+    __syncthreads();  // the previous slab is done with LDS
+    spec_fwd_xy_slab(mass + b.cell_off, S + b.cell_off, tw + b.line_off, b.P.g, b.P.lg,
+                     t - b.slab_off);
+  }
+}
+
+__global__ __launch_bounds__(256) void batch_z_green_kernel(const BatchSim* __restrict__ sims,
+                                                            int nsims, long long slabs,
+                                                            float2* __restrict__ S,
+                                                            float2* __restrict__ S2,
+                                                            const float2* __restrict__ tw,
+                                                            const float* __restrict__ gtab,
+                                                            const float* __restrict__ kd,
+                                                            const float2* __restrict__ conv) {
+  for (long long t = blockIdx.x; t < slabs; t += gridDim.x) {
+    const int si = sim_of_slab(sims, nsims, t);
+    const BatchSim& b = sims[si];
+    __syncthreads();
+    spec_z_green_slab(S + b.cell_off, S2 + b.cell_off, tw + b.line_off, gtab + b.tab_off,
+                      kd + b.line_off, b.conv_off >= 0 ? conv + b.conv_off : nullptr, b.P.g,
+                      b.P.lg, b.need_x, t - b.slab_off);
+  }
+}
+
+__global__ __launch_bounds__(256) void batch_inv_xy_kernel(const BatchSim* __restrict__ sims,
+                                                           int nsims, long long slabs,
+                                                           const float2* __restrict__ S,
+                                                           const float2* __restrict__ S2,
+                                                           float4* __restrict__ force,
+                                                           const float2* __restrict__ tw,
+                                                           const float* __restrict__ rot) {
+  for (long long t = blockIdx.x; t < slabs; t += gridDim.x) {
+    const int si = sim_of_slab(sims, nsims, t);
+    const BatchSim& b = sims[si];
+    __syncthreads();
+    spec_inv_xy_slab(S + b.cell_off, S2 + b.cell_off, force + b.cell_off, tw + b.line_off, b.P.g,
+                     b.P.lg, b.need_x, rot + 12 * b.grid_off, 1, t - b.slab_off);
+  }
+}
+
+hipError_t launch_batch_solve(const BatchSim* sims, int nsims, long long slabs, int gmax,
+                              const long long* grids, float2* S1, float2* S2, float4* force,
+                              const float2* tw, const float* gtab, const float* kd,
+                              const float2* conv, const float* rot, hipStream_t s) {
+  if (slabs <= 0) return hipSuccess;
+  const int gg = gmax * gmax;
+  const size_t ldsA = (size_t)(gmax + (gg + 1) / 2 + gg) * sizeof(float2);
+  const size_t ldsB = (size_t)(gmax + 3 * gg) * sizeof(float2);
+  const size_t ldsC = (size_t)(gmax + 4 * gg) * sizeof(float2);
+  hipError_t e;
+  if ((e = hipFuncSetAttribute((const void*)batch_fwd_xy_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)) != hipSuccess)
+    return e;
+  if ((e = hipFuncSetAttribute((const void*)batch_z_green_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)) != hipSuccess)
+    return e;
+  if ((e = hipFuncSetAttribute((const void*)batch_inv_xy_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)) != hipSuccess)
+    return e;
+  // persistent: a few workgroups per CU (256 CUs), never more than the slabs
+  const unsigned wg = (unsigned)(slabs < 2048 ? slabs : 2048);
+  hipLaunchKernelGGL(batch_fwd_xy_kernel, dim3(wg), dim3(256), ldsA, s, sims, nsims, slabs, grids,
+                     S1, tw);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(batch_z_green_kernel, dim3(wg), dim3(256), ldsB, s, sims, nsims, slabs, S1,
+                     S2, tw, gtab, kd, conv);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(batch_inv_xy_kernel, dim3(wg), dim3(256), ldsC, s, sims, nsims, slabs, S1, S2,
+                     force, tw, rot);
+  return hipGetLastError();
 }
 
 hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, float2* S2,

@@ -54,6 +54,23 @@ def init_perlin(pos, vel, world_size, noise_scale=4.0, density_contrast=2.0, see
     _abi.check(lib, None, rc)
 
 
+def mog_config(cfg: SolverConfig) -> _abi.MogConfig:
+    """The C ``mog_config`` of a coerced solver config."""
+    return _abi.MogConfig(
+        n_grids=cfg.n_grids,
+        grid_size=cfg.grid_size,
+        world_size=cfg.world_size,
+        G=cfg.G,
+        dt=cfg.dt,
+        damping=cfg.damping,
+        timescale=cfg.timescale,
+        compat=_abi.COMPAT[cfg.compat],
+        precision=_abi.PRECISION[cfg.precision],
+        solver=_abi.SOLVER[cfg.solver],
+        reserved=0,
+    )
+
+
 def _check_particles(t, name):
     torch = _torch()
     if not isinstance(t, torch.Tensor) or not t.is_cuda:
@@ -68,19 +85,7 @@ class MogSolver:
     def __init__(self, cfg: SolverConfig, rot=None, off=None, seed: int = 0):
         self.lib = _abi.load()
         self.cfg = cfg
-        c = _abi.MogConfig(
-            n_grids=cfg.n_grids,
-            grid_size=cfg.grid_size,
-            world_size=cfg.world_size,
-            G=cfg.G,
-            dt=cfg.dt,
-            damping=cfg.damping,
-            timescale=cfg.timescale,
-            compat=_abi.COMPAT[cfg.compat],
-            precision=_abi.PRECISION[cfg.precision],
-            solver=_abi.SOLVER[cfg.solver],
-            reserved=0,
-        )
+        c = mog_config(cfg)
         h = ctypes.c_void_p()
         rc = self.lib.mog_create(ctypes.byref(c), ctypes.byref(h))
         _abi.check(self.lib, None, rc)

@@ -12,6 +12,9 @@ periodic cube ``[0, W)``.
   the MOG coupling ``G_eff = G * h^3 / 2`` (density is mass per cell and the
   Poisson factor is -2*pi*G, greens.glsl:40, complex.glsl:23).
 * ``plummer_spheres`` -- Plummer spheres at uniform-random centres, v = 0.
+* ``plummer_ensemble`` -- 64 uneven simulations (n log-uniform in
+  [1e4, 5e5], n_grids in {32, 64, 128, 240}, grid_size in {16, 32}), each a
+  Plummer sphere of scale radius 5, for the batched ensemble.
 * ``two_galaxies_reference`` -- the reference's key-T state, seeded.
 The Perlin-fBm sampler of the reference's key R/1/2/3 runs on the device
 (``solver.init_perlin``).
@@ -98,6 +101,37 @@ def plummer_spheres(n=4_000_000, n_spheres=64, scale=1.5, world_size=100.0, mass
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     xyz = centres[which] + r[:, None] * d
     return _pack(xyz, np.zeros((n, 3)), mass, world_size)
+
+
+def plummer_sphere(n, scale=5.0, world_size=100.0, mass=1.0, rng=None, centre=None):
+    """One Plummer sphere (radius a / sqrt(u^(-2/3) - 1), isotropic
+    directions) about ``centre`` (default the box centre), v = 0."""
+    rng = np.random.default_rng() if rng is None else rng
+    c = np.full(3, world_size / 2.0) if centre is None else np.asarray(centre, dtype=np.float64)
+    u = rng.uniform(1e-6, 1 - 1e-6, n)
+    r = np.minimum(scale / np.sqrt(u ** (-2.0 / 3.0) - 1.0), world_size / 2)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return _pack(c + r[:, None] * d, np.zeros((n, 3)), mass, world_size)
+
+
+def plummer_ensemble(n_sims=64, n_min=1e4, n_max=5e5, grids=(32, 64, 128, 240), sizes=(16, 32),
+                     scale=5.0, world_size=100.0, G=100.0, dt=1e-5, damping=0.999, seed=4):
+    """Uneven independent simulations: n = 10^U(log10 n_min, log10 n_max),
+    n_grids uniform over ``grids``, grid_size uniform over ``sizes``, each a
+    Plummer sphere of scale radius ``scale``.  Returns a list of
+    ``(config dict, pos, vel)``."""
+    rng = np.random.default_rng(seed)
+    ns = np.round(10.0 ** rng.uniform(np.log10(n_min), np.log10(n_max), n_sims)).astype(np.int64)
+    ks = rng.choice(np.asarray(grids), n_sims)
+    gs = rng.choice(np.asarray(sizes), n_sims)
+    out = []
+    for i in range(n_sims):
+        cfg = dict(n_grids=int(ks[i]), grid_size=int(gs[i]), world_size=world_size, G=G, dt=dt,
+                   damping=damping, solver="fft", num_particles=int(ns[i]))
+        pos, vel = plummer_sphere(int(ns[i]), scale, world_size, rng=rng)
+        out.append((cfg, pos, vel))
+    return out
 
 
 def _spinning_blob(rs, count, world_size, centre, bulk, spin_axis):

@@ -94,3 +94,22 @@ def test_snapshot_roundtrip(tmp_path):
     np.savez(tmp_path / "bad.npz", pos=pos)
     with pytest.raises(Exception):
         load_snapshot(tmp_path / "bad.npz")
+
+
+def test_plummer_ensemble_shapes_and_ranges():
+    from mog_solver_amd import workloads as w
+
+    ens = w.plummer_ensemble(seed=4)
+    assert len(ens) == 64
+    ns = np.array([c["num_particles"] for c, _, _ in ens])
+    assert ns.min() >= 1e4 and ns.max() <= 5e5
+    assert {c["n_grids"] for c, _, _ in ens} <= {32, 64, 128, 240}
+    assert {c["grid_size"] for c, _, _ in ens} == {16, 32}
+    for c, p, v in ens[:5]:
+        assert p.shape == (c["num_particles"], 4) and v.shape == p.shape
+        assert p.dtype == np.float32 and np.all(p[:, :3] >= 0) and np.all(p[:, :3] < 100)
+        r = np.linalg.norm(p[:, :3].astype(np.float64) - 50.0, axis=1)
+        # Plummer: half-mass radius a / sqrt(2^(2/3) - 1) ~ 1.305 a
+        assert abs(np.median(r) / (5.0 * 1.305) - 1) < 0.1
+    again = w.plummer_ensemble(seed=4)
+    assert all(np.array_equal(a[1], b[1]) for a, b in zip(ens[:3], again[:3]))
