@@ -9,9 +9,10 @@ degrees), k = 240 grids of g = 16, W = 100, G = 100, dt = 1e-5, damping
 HBM; K steps are replayed from a hipGraph between two synchronisations.
 
 Printed (one JSON line): steps/s (value), particle-grid interactions/s,
-the relative RMS force error of the GPU against the FP64 oracle on a grid
-sample, per-phase HIP-event times, the roofline of the dominant kernel and
-the CPU baseline (the NumPy FP64 oracle timed on this host on a sample).
+the relative RMS force error of the GPU against the FP64 oracle (all k
+grids, initial state), per-phase HIP-event times, the roofline of the
+dominant kernel and the CPU baseline: the NumPy FP64 oracle timed on this
+host, on one core (a sample of grids) and as a process pool over all grids.
 """
 from __future__ import annotations
 
@@ -30,6 +31,70 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "steps/s & particle-grid ops/s at n=1e6,k=240,g=16 at fixed force error"
 HBM_PEAK_GBS = 8000.0
+
+# HBM-side bytes per launch, from two rocprofv3 --pmc runs of this script
+# (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the dispatches)
+# with --copy-calibration 256 --no-autotune on the two-disks workload: the
+# 256 MiB copy reads as FETCH 128 MiB and WRITE 256 MiB, so FETCH_SIZE is
+# scaled by 2.0 and WRITE_SIZE by 1.0.  Figures and commands:
+# profiles/pmc_traffic_two_disks_n1e6_k240_g16.md.  Used only when the run
+# matches (workload, n, k, g, precision, grids per deposit workgroup).
+PMC_TRAFFIC_KIB = {
+    ("two_disks", 1_000_000, 240, 16, "fp32", 2): {
+        "deposit": (909_808.0, 61_287.0),
+        "gather_update": (158_654.0, 64_610.0),
+    },
+}
+FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
+
+
+def _oracle_grid_sum(job):
+    """Sum over a range of grids of the FP64 oracle's per-grid world force
+    (a worker of the CPU baseline's process pool)."""
+    pos_h, rot, off, g, W, G, compat = job
+    from oracle import mog_oracle as O
+
+    avg, _ = O.forces(pos_h, rot, off, rot.shape[0], g, W, G, compat=compat)
+    return avg * rot.shape[0]
+
+
+def _ready(_):
+    return os.getpid()
+
+
+def start_cpu_pool(workers):
+    """The CPU baseline's worker processes, started before this process
+    touches the GPU (workers import NumPy and the oracle only)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+
+    # one thread per worker process: the pool is the parallelism (the
+    # children inherit the environment at start-up)
+    keys = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")
+    saved = {key: os.environ.get(key) for key in keys}
+    for key in keys:
+        os.environ[key] = "1"
+    try:
+        ex = cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn"))
+        list(ex.map(_ready, range(workers)))
+    finally:
+        for key, v in saved.items():
+            if v is None:
+                os.environ.pop(key, None)
+            else:
+                os.environ[key] = v
+    return ex
+
+
+def cpu_pool_forces(ex, workers, pos_h, rot, off, g, W, G, compat):
+    """FP64 oracle avgForce over all grids, grids sharded across the pool's
+    processes; returns (forces, wall seconds)."""
+    k = rot.shape[0]
+    bounds = np.linspace(0, k, workers + 1).astype(int)
+    jobs = [(pos_h, rot[a:b], off[a:b], g, W, G, compat) for a, b in zip(bounds[:-1], bounds[1:]) if b > a]
+    t0 = time.perf_counter()
+    total = sum(ex.map(_oracle_grid_sum, jobs))
+    return total / k, time.perf_counter() - t0
 
 
 def parse():
@@ -52,6 +117,9 @@ def parse():
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tuning instead of autotuning first")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy a MIB-MiB device buffer three times (the reference "
+                         "dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -61,12 +129,25 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    pool = workers = None
+    if not args.no_cpu_baseline:
+        # at most 16 processes: the host CPU share of one GPU
+        workers = max(1, min(16, os.cpu_count() or 1))
+        pool = start_cpu_pool(workers)
     import torch
 
     from mog_solver_amd.config import SolverConfig
     from mog_solver_amd.solver import MogSolver
     from mog_solver_amd import workloads
     from mog_solver_amd.transforms import compute_transforms
+
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration * 262144, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     W, G, dt, damping = 100.0, 100.0, 1e-5, 0.999
     n, k, g = args.n, args.k, args.g
@@ -99,6 +180,7 @@ def main():
     sample = MogSolver(sample_cfg, rot=rot[:kk], off=off[:kk])
     gpu_avg = sample.forces(pos).cpu().numpy()[:, :3].astype(np.float64)
     sample.close()
+    gpu_full = solver.forces(pos).cpu().numpy()[:, :3].astype(np.float64)
 
     tuned = None
     if not args.no_autotune:  # outside the timed region, on copies of the state
@@ -142,6 +224,12 @@ def main():
     }
     dom = max(("deposit", "gather_update", "solve"), key=lambda key: phases[key])
     achieved = alg_bytes[dom] / (phases[dom] / 1e3) / 1e9
+    tuning = solver.get_tuning()
+    measured = PMC_TRAFFIC_KIB.get((args.workload, n, k, g, args.precision, tuning["deposit_grids_per_wg"]), {})
+    traffic = None
+    if dom in measured:
+        fetch_kib, write_kib = measured[dom]
+        traffic = round((fetch_kib * FETCH_FACTOR + write_kib * WRITE_FACTOR) * 1024.0)
     roofline = {
         "bound": "hbm",
         "kernel": dom,
@@ -149,32 +237,44 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": traffic,
         "alg_bytes_per_launch": alg_bytes[dom],
         "avg_launch_ms": round(phases[dom], 4),
     }
 
     cpu = None
-    force_err = None
+    force_err = force_err_sample = None
     if not args.no_cpu_baseline:
         from oracle import mog_oracle as O
 
+        # (i) one core: the oracle on the first kk grids, scaled to k grids
         t1 = time.perf_counter()
         want, _ = O.forces(pos_h, rot[:kk], off[:kk], kk, g, W, G, compat=args.compat)
         cpu_s = time.perf_counter() - t1
-        force_err = O.rel_rms(gpu_avg, want)
-        est_step = cpu_s * k / kk
+        force_err_sample = O.rel_rms(gpu_avg, want)
+        # (ii) the process pool sharding all k grids over the host's cores;
+        # also the force error over all k grids
+        want_full, pool_s = cpu_pool_forces(pool, workers, pos_h, rot, off, g, W, G, args.compat)
+        pool.shutdown()
+        force_err = O.rel_rms(gpu_full, want_full)
         cpu = {
-            "value": round(1.0 / est_step, 6),
+            "value": round(1.0 / pool_s, 6),
             "unit": "steps/s",
-            "cores": 1,
+            "cores": workers,
             "kind": "port",
-            "sample": f"NumPy FP64 oracle force step of all {n} particles on {kk} of {k} grids "
-                      f"({cpu_s:.2f} s), scaled by k/{kk} to one {k}-grid step",
-            "interactions_per_s": round(n * kk / cpu_s, 1),
+            "sample": f"NumPy FP64 oracle force step (deposit, FFT solve, gather) of all {n} particles "
+                      f"on all {k} grids, grids sharded over {workers} processes: {pool_s:.2f} s wall",
+            "interactions_per_s": round(n * k / pool_s, 1),
+            "single_core": {
+                "value": round(kk / (cpu_s * k), 6),
+                "unit": "steps/s",
+                "cores": 1,
+                "sample": f"{kk} of {k} grids in {cpu_s:.2f} s, scaled by k/{kk}",
+            },
             "host_cpu_count": os.cpu_count(),
         }
-        print(f"[bench] cpu baseline {cpu_s:.2f} s for {kk} grids; force rel rms {force_err:.3e}",
+        print(f"[bench] cpu baseline: 1 core {cpu_s:.2f} s for {kk} grids; {workers} processes "
+              f"{pool_s:.2f} s for {k} grids; force rel rms {force_err:.3e} (all grids)",
               file=sys.stderr, flush=True)
 
     out = {
@@ -199,8 +299,9 @@ def main():
         },
         "interactions_per_s": round(n * k * steps_per_s, 1),
         "force_rel_rms_vs_fp64": None if force_err is None else float(f"{force_err:.3e}"),
+        "force_rel_rms_vs_fp64_first_grids": None if force_err_sample is None else float(f"{force_err_sample:.3e}"),
         "force_sample_grids": kk,
-        "tuning": solver.get_tuning(),
+        "tuning": tuning,
         "autotuned": tuned is not None,
         "phases_ms": {key: round(v, 4) for key, v in phases.items()},
         "roofline": roofline,

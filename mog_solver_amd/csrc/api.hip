@@ -332,8 +332,10 @@ DepositTune default_tune(const Params& P) {
   const size_t grid_bytes = (size_t)P.cells * 4;
   const int per = (int)((32 * 1024) / grid_bytes);
   t.grids_per_wg = per < 1 ? 1 : (per > 8 ? 8 : per);
+  // measured optima on MI355X: g=16 (16 KB grids) 512 threads, 2 grids and
+  // 4096 workgroups; g=32 (128 KB) 1024 threads, 1 grid, 1024 workgroups
   t.threads = grid_bytes > 64 * 1024 ? 1024 : 512;
-  t.wg_target = grid_bytes > 64 * 1024 ? 1024 : 2048;
+  t.wg_target = grid_bytes > 64 * 1024 ? 1024 : 4096;
   return t;
 }
 
@@ -469,8 +471,15 @@ struct mog_batch {
   std::vector<mog_config> cfg;
   std::vector<BatchSim> sims;  // host copy of the device descriptor table
   std::vector<char> have_transforms;
-  long long ntotal, cells_total, slabs_total, gather_blocks;
-  int grids_total, gmax;
+  long long ntotal, cells_total, gather_blocks;
+  int grids_total;
+  // solve classes by grid size (g <= 16, g = 32): simulations, slabs, LDS
+  struct SolveClass {
+    int start, count, gmax;
+    long long slabs;
+  } cls[2];
+  int* d_cls_sim;
+  long long* d_cls_off;
   BatchSim* d_sims;
   float* d_rot;
   float* d_off;
@@ -513,7 +522,7 @@ void batch_release(mog_batch* b) {
   void* ptrs[] = {b->d_sims, b->d_rot,  b->d_off, b->d_frot, b->d_grids, b->d_S1,
                   b->d_S2,   b->d_force, b->d_tw, b->d_gtab, b->d_kd,    b->d_conv,
                   b->d_tasks, b->d_counter, b->d_order, b->d_idx, b->d_keys_a, b->d_keys_b,
-                  b->d_sort_tmp};
+                  b->d_sort_tmp, b->d_cls_sim, b->d_cls_off};
   for (void* q : ptrs) (void)hipFree(q);
 }
 
@@ -588,9 +597,12 @@ hipError_t batch_enqueue(mog_batch* b, float* pos, float* vel, hipStream_t s) {
     e = launch_batch_deposit((const float4*)pos, b->d_sims, b->d_tasks, b->ntasks, b->d_rot,
                              b->d_off, b->d_frot, (unsigned long long*)b->d_grids, b->d_counter,
                              b->deposit_threads, s);
-  if (e == hipSuccess)
-    e = launch_batch_solve(b->d_sims, b->nsims, b->slabs_total, b->gmax, b->d_grids, b->d_S1,
-                           b->d_S2, b->d_force, b->d_tw, b->d_gtab, b->d_kd, b->d_conv, b->d_rot, s);
+  for (int c = 0; c < 2 && e == hipSuccess; ++c) {
+    const mog_batch::SolveClass& sc = b->cls[c];
+    e = launch_batch_solve(b->d_sims, b->d_cls_sim + sc.start, b->d_cls_off + sc.start, sc.count,
+                           sc.slabs, sc.gmax, b->d_grids, b->d_S1, b->d_S2, b->d_force, b->d_tw,
+                           b->d_gtab, b->d_kd, b->d_conv, b->d_rot, s);
+  }
   if (e == hipSuccess)
     e = launch_batch_gather((float4*)pos, (float4*)vel, b->gather_blocks, b->d_sims, b->nsims,
                             b->d_rot, b->d_off, b->d_frot, b->d_force, batch_order(b), s);
@@ -626,7 +638,10 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
   c->resort_every = 16;
   c->sorted_deposit = false;
   c->cell_cache = false;
-  c->spectral_mfma = true;
+  // the FP32 vector transforms beat the FP32 matrix-core ones at g = 16 and
+  // 32 (the FP32 MFMA rate equals the packed vector rate); the split
+  // precisions always use the matrix cores
+  c->spectral_mfma = false;
 
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) {
@@ -1066,9 +1081,11 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
   b->cfg.assign(cfgs, cfgs + n_sims);
   b->sims.resize(n_sims);
   b->have_transforms.assign(n_sims, 0);
-  long long p_off = 0, cell_off = 0, slab_off = 0, conv_off = 0, blk_off = 0;
+  long long p_off = 0, cell_off = 0, conv_off = 0, blk_off = 0, cls_slabs[2] = {0, 0};
   int grid_off = 0, tab_off = 0, line_off = 0;
-  b->gmax = 0;
+  std::vector<int> cls_sim[2];
+  std::vector<long long> cls_off[2];
+  int cls_gmax[2] = {0, 0};
   for (int i = 0; i < n_sims; ++i) {
     const mog_config* c = &cfgs[i];
     int rc = validate(c);
@@ -1088,8 +1105,12 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
     sm.P = make_params(c);
     sm.p_off = p_off;
     sm.n = n[i];
+    const int ci = c->grid_size <= 16 ? 0 : 1;
     sm.cell_off = cell_off;
-    sm.slab_off = slab_off;
+    sm.slab_off = cls_slabs[ci];
+    cls_sim[ci].push_back(i);
+    cls_off[ci].push_back(cls_slabs[ci]);
+    cls_gmax[ci] = std::max(cls_gmax[ci], c->grid_size);
     sm.grid_off = grid_off;
     sm.tab_off = tab_off;
     sm.line_off = line_off;
@@ -1101,12 +1122,11 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
     const long long g = c->grid_size, cells = (long long)c->n_grids * g * g * g;
     p_off += n[i];
     cell_off += cells;
-    slab_off += (long long)c->n_grids * g;
+    cls_slabs[ci] += (long long)c->n_grids * g;
     grid_off += c->n_grids;
     tab_off += (int)(g * g * g);
     line_off += (int)g;
     if (conv) conv_off += 2 * g * g * g;
-    b->gmax = std::max(b->gmax, (int)g);
   }
   if (p_off > 2147483647LL || cell_off > 2147483647LL) {
     delete b;
@@ -1114,8 +1134,17 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
   }
   b->ntotal = p_off;
   b->cells_total = cell_off;
-  b->slabs_total = slab_off;
   b->gather_blocks = blk_off;
+  std::vector<int> all_sim;
+  std::vector<long long> all_off;
+  for (int ci = 0; ci < 2; ++ci) {
+    b->cls[ci].start = (int)all_sim.size();
+    b->cls[ci].count = (int)cls_sim[ci].size();
+    b->cls[ci].gmax = cls_gmax[ci];
+    b->cls[ci].slabs = cls_slabs[ci];
+    all_sim.insert(all_sim.end(), cls_sim[ci].begin(), cls_sim[ci].end());
+    all_off.insert(all_off.end(), cls_off[ci].begin(), cls_off[ci].end());
+  }
   b->grids_total = grid_off;
   b->deposit_chunk = 16384;
   b->deposit_threads = 1024;
@@ -1136,6 +1165,8 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
   A(dalloc(&b->d_kd, (size_t)line_off));
   A(dalloc(&b->d_conv, (size_t)(conv_off > 0 ? conv_off : 1)));
   A(dalloc(&b->d_counter, 1));
+  A(dalloc(&b->d_cls_sim, (size_t)n_sims));
+  A(dalloc(&b->d_cls_off, (size_t)n_sims));
   b->resort_every = 16;
   size_t tmp = 0;
   A(batch_order_temp_bytes(p_off, n_sims, &tmp));
@@ -1162,6 +1193,9 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
   A(hipMemcpy(b->d_kd, kd.data(), kd.size() * sizeof(float), hipMemcpyHostToDevice));
   A(hipMemcpy(b->d_conv, conv.data(), conv.size() * sizeof(float2), hipMemcpyHostToDevice));
   A(hipMemcpy(b->d_sims, b->sims.data(), b->sims.size() * sizeof(BatchSim), hipMemcpyHostToDevice));
+  A(hipMemcpy(b->d_cls_sim, all_sim.data(), all_sim.size() * sizeof(int), hipMemcpyHostToDevice));
+  A(hipMemcpy(b->d_cls_off, all_off.data(), all_off.size() * sizeof(long long),
+              hipMemcpyHostToDevice));
   A(batch_build_tasks(b));
   if (e != hipSuccess) {
     batch_release(b);

@@ -168,8 +168,8 @@ hipError_t launch_update(const Params& P, float4* pos, float4* vel, const float4
 // s are rows [p_off, p_off + n) of the packed pos/vel; its k grids are cells
 // [cell_off, cell_off + k g^3) of the packed grids / S1 / S2 / force; its
 // transforms are rows [grid_off, grid_off + k) of rot / off / frot; its
-// solve slabs (k g per pass) are [slab_off, slab_off + k g) of the pass's
-// flat slab range.
+// solve slabs (k g per pass) are [slab_off, slab_off + k g) of its
+// grid-size class's flat slab range.
 struct BatchSim {
   Params P;
   long long p_off, n;
@@ -222,10 +222,14 @@ hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const D
                                 int ntasks, const float* rot, const float* off, const float* frot,
                                 unsigned long long* grids, unsigned long long* counter,
                                 int threads, hipStream_t s);
-hipError_t launch_batch_solve(const BatchSim* sims, int nsims, long long slabs, int gmax,
-                              const long long* grids, float2* S1, float2* S2, float4* force,
-                              const float2* tw, const float* gtab, const float* kd,
-                              const float2* conv, const float* rot, hipStream_t s);
+// Solve passes over one grid-size class: simulations cls_sim[0..ncls) whose
+// slabs start at cls_off[j] in the class's range of `slabs` slabs; LDS is
+// sized for the class's largest grid `gmax`.
+hipError_t launch_batch_solve(const BatchSim* sims, const int* cls_sim, const long long* cls_off,
+                              int ncls, long long slabs, int gmax, const long long* grids,
+                              float2* S1, float2* S2, float4* force, const float2* tw,
+                              const float* gtab, const float* kd, const float2* conv,
+                              const float* rot, hipStream_t s);
 hipError_t launch_batch_gather(float4* pos, float4* vel, long long blocks, const BatchSim* sims,
                                int nsims, const float* rot, const float* off, const float* frot,
                                const float4* force, const int* order, hipStream_t s);

@@ -243,17 +243,20 @@ __global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Batched ensemble: the three passes as persistent kernels over the flat
-// slab range of all simulations (sims ordered by slab_off).  Each workgroup
-// strides over slabs; a slab runs the same body as the single-simulation
-// launch, with the simulation's own g, tables and buffer offsets.
+// Batched ensemble: the three passes, one launch each per grid-size class
+// (g <= 16, g = 32), one workgroup per slab of any simulation of the class.
+// A class lists its simulations (cls_sim) and their first slab in the
+// class's flat slab range (cls_off, ascending); a slab runs the same body as
+// the single-simulation launch, with the simulation's own g, tables and
+// buffer offsets.  Splitting by class sizes LDS per class, so g = 16 slabs
+// keep the occupancy of the single-simulation launch.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int sim_of_slab(const BatchSim* __restrict__ sims, int nsims,
-                                           long long t) {
-  int lo = 0, hi = nsims - 1;
+__device__ __forceinline__ int class_slot(const long long* __restrict__ cls_off, int ncls,
+                                          long long t) {
+  int lo = 0, hi = ncls - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (sims[mid].slab_off <= t)
+    if (cls_off[mid] <= t)
       lo = mid;
     else
       hi = mid - 1;
@@ -262,58 +265,57 @@ __device__ __forceinline__ int sim_of_slab(const BatchSim* __restrict__ sims, in
 }
 
 __global__ __launch_bounds__(256) void batch_fwd_xy_kernel(const BatchSim* __restrict__ sims,
-                                                           int nsims, long long slabs,
+                                                           const int* __restrict__ cls_sim,
+                                                           const long long* __restrict__ cls_off,
+                                                           int ncls,
                                                            const long long* __restrict__ mass,
                                                            float2* __restrict__ S,
                                                            const float2* __restrict__ tw) {
-  for (long long t = blockIdx.x; t < slabs; t += gridDim.x) {
-    const int si = sim_of_slab(sims, nsims, t);
-    const BatchSim& b = sims[si];
-    __syncthreads();  // the previous slab is done with LDS
-    spec_fwd_xy_slab(mass + b.cell_off, S + b.cell_off, tw + b.line_off, b.P.g, b.P.lg,
-                     t - b.slab_off);
-  }
+  const long long t = blockIdx.x;
+  const int j = class_slot(cls_off, ncls, t);
+  const BatchSim& b = sims[cls_sim[j]];
+  spec_fwd_xy_slab(mass + b.cell_off, S + b.cell_off, tw + b.line_off, b.P.g, b.P.lg,
+                   t - cls_off[j]);
 }
 
 __global__ __launch_bounds__(256) void batch_z_green_kernel(const BatchSim* __restrict__ sims,
-                                                            int nsims, long long slabs,
-                                                            float2* __restrict__ S,
+                                                            const int* __restrict__ cls_sim,
+                                                            const long long* __restrict__ cls_off,
+                                                            int ncls, float2* __restrict__ S,
                                                             float2* __restrict__ S2,
                                                             const float2* __restrict__ tw,
                                                             const float* __restrict__ gtab,
                                                             const float* __restrict__ kd,
                                                             const float2* __restrict__ conv) {
-  for (long long t = blockIdx.x; t < slabs; t += gridDim.x) {
-    const int si = sim_of_slab(sims, nsims, t);
-    const BatchSim& b = sims[si];
-    __syncthreads();
-    spec_z_green_slab(S + b.cell_off, S2 + b.cell_off, tw + b.line_off, gtab + b.tab_off,
-                      kd + b.line_off, b.conv_off >= 0 ? conv + b.conv_off : nullptr, b.P.g,
-                      b.P.lg, b.need_x, t - b.slab_off);
-  }
+  const long long t = blockIdx.x;
+  const int j = class_slot(cls_off, ncls, t);
+  const BatchSim& b = sims[cls_sim[j]];
+  spec_z_green_slab(S + b.cell_off, S2 + b.cell_off, tw + b.line_off, gtab + b.tab_off,
+                    kd + b.line_off, b.conv_off >= 0 ? conv + b.conv_off : nullptr, b.P.g, b.P.lg,
+                    b.need_x, t - cls_off[j]);
 }
 
 __global__ __launch_bounds__(256) void batch_inv_xy_kernel(const BatchSim* __restrict__ sims,
-                                                           int nsims, long long slabs,
-                                                           const float2* __restrict__ S,
+                                                           const int* __restrict__ cls_sim,
+                                                           const long long* __restrict__ cls_off,
+                                                           int ncls, const float2* __restrict__ S,
                                                            const float2* __restrict__ S2,
                                                            float4* __restrict__ force,
                                                            const float2* __restrict__ tw,
                                                            const float* __restrict__ rot) {
-  for (long long t = blockIdx.x; t < slabs; t += gridDim.x) {
-    const int si = sim_of_slab(sims, nsims, t);
-    const BatchSim& b = sims[si];
-    __syncthreads();
-    spec_inv_xy_slab(S + b.cell_off, S2 + b.cell_off, force + b.cell_off, tw + b.line_off, b.P.g,
-                     b.P.lg, b.need_x, rot + 12 * b.grid_off, 1, t - b.slab_off);
-  }
+  const long long t = blockIdx.x;
+  const int j = class_slot(cls_off, ncls, t);
+  const BatchSim& b = sims[cls_sim[j]];
+  spec_inv_xy_slab(S + b.cell_off, S2 + b.cell_off, force + b.cell_off, tw + b.line_off, b.P.g,
+                   b.P.lg, b.need_x, rot + 12 * b.grid_off, 1, t - cls_off[j]);
 }
 
-hipError_t launch_batch_solve(const BatchSim* sims, int nsims, long long slabs, int gmax,
-                              const long long* grids, float2* S1, float2* S2, float4* force,
-                              const float2* tw, const float* gtab, const float* kd,
-                              const float2* conv, const float* rot, hipStream_t s) {
-  if (slabs <= 0) return hipSuccess;
+hipError_t launch_batch_solve(const BatchSim* sims, const int* cls_sim, const long long* cls_off,
+                              int ncls, long long slabs, int gmax, const long long* grids,
+                              float2* S1, float2* S2, float4* force, const float2* tw,
+                              const float* gtab, const float* kd, const float2* conv,
+                              const float* rot, hipStream_t s) {
+  if (slabs <= 0 || ncls <= 0) return hipSuccess;
   const int gg = gmax * gmax;
   const size_t ldsA = (size_t)(gmax + (gg + 1) / 2 + gg) * sizeof(float2);
   const size_t ldsB = (size_t)(gmax + 3 * gg) * sizeof(float2);
@@ -328,16 +330,15 @@ hipError_t launch_batch_solve(const BatchSim* sims, int nsims, long long slabs, 
   if ((e = hipFuncSetAttribute((const void*)batch_inv_xy_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)) != hipSuccess)
     return e;
-  // persistent: a few workgroups per CU (256 CUs), never more than the slabs
-  const unsigned wg = (unsigned)(slabs < 2048 ? slabs : 2048);
-  hipLaunchKernelGGL(batch_fwd_xy_kernel, dim3(wg), dim3(256), ldsA, s, sims, nsims, slabs, grids,
-                     S1, tw);
+  const dim3 grid((unsigned)slabs);
+  hipLaunchKernelGGL(batch_fwd_xy_kernel, grid, dim3(256), ldsA, s, sims, cls_sim, cls_off, ncls,
+                     grids, S1, tw);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(batch_z_green_kernel, dim3(wg), dim3(256), ldsB, s, sims, nsims, slabs, S1,
-                     S2, tw, gtab, kd, conv);
+  hipLaunchKernelGGL(batch_z_green_kernel, grid, dim3(256), ldsB, s, sims, cls_sim, cls_off, ncls,
+                     S1, S2, tw, gtab, kd, conv);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(batch_inv_xy_kernel, dim3(wg), dim3(256), ldsC, s, sims, nsims, slabs, S1, S2,
-                     force, tw, rot);
+  hipLaunchKernelGGL(batch_inv_xy_kernel, grid, dim3(256), ldsC, s, sims, cls_sim, cls_off, ncls,
+                     S1, S2, force, tw, rot);
   return hipGetLastError();
 }
 
