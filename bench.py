@@ -41,8 +41,8 @@ HBM_PEAK_GBS = 8000.0
 # matches (workload, n, k, g, precision, grids per deposit workgroup).
 PMC_TRAFFIC_KIB = {
     ("two_disks", 1_000_000, 240, 16, "fp32", 2): {
-        "deposit": (909_808.0, 61_287.0),
-        "gather_update": (158_654.0, 64_610.0),
+        "deposit": (909_256.0, 61_289.0),
+        "gather_update": (159_296.0, 64_615.0),
     },
 }
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
