@@ -233,6 +233,29 @@ int mog_batch_set_tuning(mog_batch* batch, int64_t deposit_chunk, int32_t deposi
 int mog_batch_get_tuning(const mog_batch* batch, int64_t* deposit_chunk, int32_t* deposit_threads,
                          int32_t* n_tasks);
 
+/* ---- 2D path (the DIM == 2 branches of update.glsl:19-34, pack.glsl:8;
+ * never compiled by the reference, main.py:237) -------------------------
+ * k planar grids of g x g cells over [0, W)^2.  rot_k12 / off_k4 use the 3D
+ * packed layout; the 2x2 block rot[4j+m] (j, m < 2) is the planar rotation
+ * (mat2(col0.xy, col1.xy)) and off[0..1] the offset.  Particles are (n,4)
+ * float32 = (x, y, unused, m).  Density is int64 milli-mass per cell, the
+ * Green's function -2 pi G / k^2 (DC removed), the force i k phi; the
+ * average world force is written as (Fx, Fy, 0, 0).  Errors are reported
+ * through mog_last_error(NULL). */
+typedef struct mog2d_ctx mog2d_ctx;
+
+int mog2d_create(int32_t n_grids, int32_t grid_size, double world_size, double G,
+                 const float* rot_k12, const float* off_k4, mog2d_ctx** out);
+void mog2d_destroy(mog2d_ctx* ctx);
+
+/* Average force at the particles (async on `stream`). */
+int mog2d_forces(mog2d_ctx* ctx, const float* pos_n4, int64_t n, float* avg_n4, void* stream);
+
+/* One step: force, then vel -= F dt; vel *= damping; pos += vel dt; pos mod W
+ * (update.glsl:83-86), z untouched when vel.z is 0. */
+int mog2d_step(mog2d_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, double dt, double damping,
+               void* stream);
+
 /* Message of the last error of `ctx` (or of the last failed mog_create when
  * ctx is NULL).  Never NULL. */
 const char* mog_last_error(const mog_ctx* ctx);

@@ -55,6 +55,10 @@ EXPORTED = (
     "mog_batch_run",
     "mog_batch_set_tuning",
     "mog_batch_get_tuning",
+    "mog2d_create",
+    "mog2d_destroy",
+    "mog2d_forces",
+    "mog2d_step",
     "mog_last_error",
 )
 
@@ -186,6 +190,14 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     lib.mog_batch_set_tuning.argtypes = [vp, i64, i32]
     lib.mog_batch_get_tuning.restype = ctypes.c_int
     lib.mog_batch_get_tuning.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    lib.mog2d_create.restype = ctypes.c_int
+    lib.mog2d_create.argtypes = [i32, i32, d, d, vp, vp, ctypes.POINTER(vp)]
+    lib.mog2d_destroy.restype = None
+    lib.mog2d_destroy.argtypes = [vp]
+    lib.mog2d_forces.restype = ctypes.c_int
+    lib.mog2d_forces.argtypes = [vp, vp, i64, vp, vp]
+    lib.mog2d_step.restype = ctypes.c_int
+    lib.mog2d_step.argtypes = [vp, vp, vp, i64, d, d, vp]
     lib.mog_last_error.restype = ctypes.c_char_p
     lib.mog_last_error.argtypes = [vp]
     if path is None:

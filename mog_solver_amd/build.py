@@ -24,7 +24,7 @@ LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libmog.so"
 ARCH = os.environ.get("MOG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "deposit.hip", "spectral.hip", "spectral_mfma.hip", "gather.hip", "order.hip", "init.hip", "closure.hip", "batch.hip"]
+SOURCES = ["api.hip", "deposit.hip", "spectral.hip", "spectral_mfma.hip", "gather.hip", "order.hip", "init.hip", "closure.hip", "batch.hip", "mog2d.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

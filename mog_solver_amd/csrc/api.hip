@@ -514,7 +514,26 @@ struct mog_batch {
   const void* g_vel;
 };
 
+struct mog2d_ctx {
+  int k, g, lg;
+  double W, G;
+  float* d_rot;
+  float* d_off;
+  float2* d_tw;
+  float* d_gtab;
+  float* d_kd;
+  long long* d_grids;
+  float2* d_force;
+  float4* d_avg;
+  long long avg_cap;
+};
+
 namespace {
+
+void mog2d_release(mog2d_ctx* c) {
+  void* ptrs[] = {c->d_rot, c->d_off, c->d_tw, c->d_gtab, c->d_kd, c->d_grids, c->d_force, c->d_avg};
+  for (void* q : ptrs) (void)hipFree(q);
+}
 
 void batch_release(mog_batch* b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -1327,6 +1346,121 @@ int mog_batch_get_tuning(const mog_batch* b, int64_t* deposit_chunk, int32_t* de
   if (deposit_threads) *deposit_threads = b->deposit_threads;
   if (n_tasks) *n_tasks = b->ntasks;
   return MOG_OK;
+}
+
+int mog2d_create(int32_t n_grids, int32_t grid_size, double world_size, double G,
+                 const float* rot_k12, const float* off_k4, mog2d_ctx** out) {
+  if (!out) return fail(nullptr, MOG_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (n_grids < 1 || n_grids > 65535) return fail(nullptr, MOG_EINVAL, "n_grids must be in [1, 65535]");
+  if (!is_pow2(grid_size) || grid_size < 2 || grid_size > 64)
+    return fail(nullptr, MOG_EINVAL, "grid_size must be a power of two in [2, 64]");
+  if (!(isfinite(world_size) && world_size > 0.0) || !isfinite(G))
+    return fail(nullptr, MOG_EINVAL, "world_size must be finite and > 0, G finite");
+  if (!rot_k12 || !off_k4) return fail(nullptr, MOG_EINVAL, "rot/off is NULL");
+  for (int i = 0; i < n_grids * 12; ++i)
+    if (!isfinite(rot_k12[i])) return fail(nullptr, MOG_EINVAL, "rotations must be finite");
+  for (int i = 0; i < n_grids * 4; ++i)
+    if (!isfinite(off_k4[i])) return fail(nullptr, MOG_EINVAL, "offsets must be finite");
+  mog2d_ctx* c = new mog2d_ctx();
+  c->k = n_grids;
+  c->g = grid_size;
+  c->lg = ilog2(grid_size);
+  c->W = world_size;
+  c->G = G;
+  const size_t g = (size_t)grid_size, cells = (size_t)n_grids * g * g;
+  hipError_t e = hipSuccess;
+  auto A = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  A(dalloc(&c->d_rot, (size_t)n_grids * 12));
+  A(dalloc(&c->d_off, (size_t)n_grids * 4));
+  A(dalloc(&c->d_tw, g));
+  A(dalloc(&c->d_gtab, g * g));
+  A(dalloc(&c->d_kd, g));
+  A(dalloc(&c->d_grids, cells));
+  A(dalloc(&c->d_force, cells));
+  if (e != hipSuccess) {
+    mog2d_release(c);
+    delete c;
+    return fail(nullptr, MOG_ENOMEM, "device allocation failed for the 2D grids");
+  }
+  // FP64-derived tables: twiddles, -2 pi G / k^2 / (1000 g^2) with the
+  // Nyquist-positive wave numbers (0 at DC), derivative factors with the
+  // Nyquist entry zeroed
+  std::vector<float2> tw(g);
+  std::vector<float> gtab(g * g), kd(g);
+  auto sh = [&](int i) { return i <= (int)g / 2 ? i : i - (int)g; };
+  for (size_t j = 0; j < g; ++j) {
+    const double th = 2.0 * M_PI * (double)j / (double)g;
+    tw[j] = make_float2((float)cos(th), (float)sin(th));
+    kd[j] = (j == g / 2 && g > 1) ? 0.0f : (float)((double)sh((int)j) * 2.0 * M_PI / world_size);
+  }
+  for (size_t iy = 0; iy < g; ++iy)
+    for (size_t ix = 0; ix < g; ++ix) {
+      const double kx = sh((int)ix) * 2.0 * M_PI / world_size, ky = sh((int)iy) * 2.0 * M_PI / world_size;
+      const double k2 = kx * kx + ky * ky;
+      gtab[iy * g + ix] = (ix | iy) ? (float)(-2.0 * M_PI * G / k2 / (1000.0 * (double)(g * g))) : 0.0f;
+    }
+  A(hipMemcpy(c->d_rot, rot_k12, (size_t)n_grids * 12 * sizeof(float), hipMemcpyHostToDevice));
+  A(hipMemcpy(c->d_off, off_k4, (size_t)n_grids * 4 * sizeof(float), hipMemcpyHostToDevice));
+  A(hipMemcpy(c->d_tw, tw.data(), g * sizeof(float2), hipMemcpyHostToDevice));
+  A(hipMemcpy(c->d_gtab, gtab.data(), g * g * sizeof(float), hipMemcpyHostToDevice));
+  A(hipMemcpy(c->d_kd, kd.data(), g * sizeof(float), hipMemcpyHostToDevice));
+  if (e != hipSuccess) {
+    mog2d_release(c);
+    delete c;
+    return fail(nullptr, MOG_EHIP, "2D tables: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return MOG_OK;
+}
+
+void mog2d_destroy(mog2d_ctx* ctx) {
+  if (!ctx) return;
+  mog2d_release(ctx);
+  delete ctx;
+}
+
+int mog2d_forces(mog2d_ctx* ctx, const float* pos_n4, int64_t n, float* avg_n4, void* stream) {
+  if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
+  if (n < 0 || n > (int64_t)2147483647) return fail(nullptr, MOG_EINVAL, "n must be in [0, 2^31)");
+  if (n > 0 && (!pos_n4 || !avg_n4 || !aligned16(pos_n4) || !aligned16(avg_n4)))
+    return fail(nullptr, MOG_EINVAL, "pos/avg must be 16-byte aligned device pointers");
+  hipError_t e = launch_mog2d_forces((const float4*)pos_n4, n, ctx->k, ctx->lg, (float)ctx->W,
+                                     ctx->d_rot, ctx->d_off, ctx->d_tw, ctx->d_gtab, ctx->d_kd,
+                                     ctx->d_grids, ctx->d_force, (float4*)avg_n4, (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : fail(nullptr, MOG_EHIP, "mog2d_forces: %s", hipGetErrorString(e));
+}
+
+int mog2d_step(mog2d_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, double dt, double damping,
+               void* stream) {
+  if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
+  if (n < 0 || n > (int64_t)2147483647) return fail(nullptr, MOG_EINVAL, "n must be in [0, 2^31)");
+  if (!isfinite(dt) || !isfinite(damping)) return fail(nullptr, MOG_EINVAL, "dt/damping must be finite");
+  if (n == 0) return MOG_OK;
+  if (!pos_n4 || !vel_n4 || !aligned16(pos_n4) || !aligned16(vel_n4))
+    return fail(nullptr, MOG_EINVAL, "pos/vel must be 16-byte aligned device pointers");
+  hipStream_t s = (hipStream_t)stream;
+  if (ctx->avg_cap < n) {
+    (void)hipFree(ctx->d_avg);
+    ctx->d_avg = nullptr;
+    ctx->avg_cap = 0;
+    if (dalloc(&ctx->d_avg, (size_t)n) != hipSuccess)
+      return fail(nullptr, MOG_ENOMEM, "2D force buffer for %lld particles", (long long)n);
+    ctx->avg_cap = n;
+  }
+  hipError_t e = launch_mog2d_forces((const float4*)pos_n4, n, ctx->k, ctx->lg, (float)ctx->W,
+                                     ctx->d_rot, ctx->d_off, ctx->d_tw, ctx->d_gtab, ctx->d_kd,
+                                     ctx->d_grids, ctx->d_force, ctx->d_avg, s);
+  if (e == hipSuccess) {
+    Params P{};
+    P.W = (float)ctx->W;
+    P.dtp = (float)dt;
+    P.damping = (float)damping;
+    e = launch_update(P, (float4*)pos_n4, (float4*)vel_n4, ctx->d_avg, n, s);
+  }
+  return e == hipSuccess ? MOG_OK : fail(nullptr, MOG_EHIP, "mog2d_step: %s", hipGetErrorString(e));
 }
 
 const char* mog_last_error(const mog_ctx* ctx) {

@@ -218,6 +218,14 @@ hipError_t build_batch_order(const float4* pos, long long n, long long blocks,
                              unsigned long long* keys_b, int* idx_a, int* order, void* tmp,
                              size_t tmp_bytes, hipStream_t s);
 
+// 2D MOG: clear, deposit, per-grid 2D spectral solve and gather of the
+// average world force (avg = (Fx, Fy, 0, 0)).  grids: k g^2 int64 scratch,
+// force: k g^2 float2 scratch; tw / gtab / kd: the 2D tables.
+hipError_t launch_mog2d_forces(const float4* pos, long long n, int k, int lg, float W,
+                               const float* rot, const float* off, const float2* tw,
+                               const float* gtab, const float* kd, long long* grids,
+                               float2* force, float4* avg, hipStream_t s);
+
 hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const DepositTask* tasks,
                                 int ntasks, const float* rot, const float* off, const float* frot,
                                 unsigned long long* grids, unsigned long long* counter,

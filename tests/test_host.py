@@ -68,7 +68,7 @@ def test_sampler_matches_oracle_bit_exact(k, g, W, seed):
 def header_functions():
     text = (ROOT / "include" / "mog.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(mog_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(mog(?:2d)?_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_header_symbol():
@@ -77,7 +77,7 @@ def test_library_exports_every_header_symbol():
     assert set(declared) == set(_abi.EXPORTED)
     out = subprocess.run(["nm", "-D", "--defined-only", str(_abi.LIB_PATH)], capture_output=True, text=True,
                          check=True).stdout
-    exported = set(re.findall(r"\bT (mog_\w+)", out))
+    exported = set(re.findall(r"\bT (mog(?:2d)?_\w+)", out))
     missing = set(declared) - exported
     assert not missing, missing
     lib = _abi.load()
