@@ -71,13 +71,24 @@ MOG_HD int cell_exact(float c0, float c1, float c2, const float* R, const float*
 // (the -0.5 is folded into F[4m+3]) and r = rint(w): floor(v) = r and the
 // distance of v to the nearest integer is 0.5 - |w - r|.  Flags combine with
 // bitwise & so that no branch is taken per axis.
+MOG_HD int cell_fast_a(float c0, float c1, float c2, const float* F, float a0, float a1, float a2,
+                       float lim, int lg, int* cell);
+
 MOG_HD int cell_fast(float c0, float c1, float c2, const float* F, float lim, int lg, int* cell) {
-  const float w0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, F[3])));
-  const float w1 = fmaf(F[6], c2, fmaf(F[5], c1, fmaf(F[4], c0, F[7])));
-  const float w2 = fmaf(F[10], c2, fmaf(F[9], c1, fmaf(F[8], c0, F[11])));
+  return cell_fast_a(c0, c1, c2, F, F[3], F[7], F[11], lim, lg, cell);
+}
+
+// cell_fast with the row's three addends F[3], F[7], F[11] passed as values
+// (a kernel can then keep them in vector registers).
+MOG_HD int cell_fast_a(float c0, float c1, float c2, const float* F, float a0, float a1, float a2,
+                       float lim, int lg, int* cell) {
+  const float w0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, a0)));
+  const float w1 = fmaf(F[6], c2, fmaf(F[5], c1, fmaf(F[4], c0, a1)));
+  const float w2 = fmaf(F[10], c2, fmaf(F[9], c1, fmaf(F[8], c0, a2)));
   const float r0 = rintf(w0), r1 = rintf(w1), r2 = rintf(w2);
-  const int ok = (int)(fabsf(w0 - r0) < lim) & (int)(fabsf(w1 - r1) < lim) &
-                 (int)(fabsf(w2 - r2) < lim);
+  // one max3 and one compare (callers admit only finite coordinates, so no
+  // NaN reaches fmaxf)
+  const int ok = (int)(fmaxf(fmaxf(fabsf(w0 - r0), fabsf(w1 - r1)), fabsf(w2 - r2)) < lim);
   const int gm = (1 << lg) - 1;
   int c = (int)r2;
   c = (c << lg) | ((int)r1 & gm);

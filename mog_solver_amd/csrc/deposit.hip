@@ -53,6 +53,8 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
     const int gr = min(grid0 + gi, P.k - 1);
 #pragma unroll
     for (int j = 0; j < 12; ++j) F[gi][j] = frot[12 * gr + j];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) F[gi][4 * m + 3] = to_vgpr(F[gi][4 * m + 3]);
   }
   __syncthreads();
 

@@ -26,13 +26,23 @@ __device__ __forceinline__ void accumulate_rotated(const float* __restrict__ R, 
 // loop is unrolled by U so that U force fetches are in flight per wave.
 // `order` (optional) is the visiting order: thread t handles particle
 // order[t], so that the lanes of a wave are neighbours in space.
-template <bool WORLD>
+//
+// LDS3: the fast rows' addends (3 per grid) are staged in LDS, so the FMA
+// chain reads them as vector operands (ds_read) instead of re-copying them
+// from scalar registers with three v_mov per pair; used when k * 12 bytes
+// fit the launch's LDS budget.
+template <bool WORLD, bool LDS3>
 __global__ __launch_bounds__(256) void gather_update_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, long long n,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
     const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode,
     const int* __restrict__ order) {
   constexpr int U = 4;
+  extern __shared__ float adds[];  // LDS3: adds[3 g + m] = frot[12 g + 4 m + 3]
+  if (LDS3) {
+    for (int i = threadIdx.x; i < 3 * P.k; i += blockDim.x) adds[i] = frot[12 * (i / 3) + 4 * (i % 3) + 3];
+    __syncthreads();
+  }
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const long long i = order ? (long long)order[t] : t;
@@ -48,7 +58,10 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int gr = grid + u;
-      const int cell = cell_index(c0, c1, c2, pfast, frot + 12 * gr, rot + 12 * gr, off + 4 * gr, P);
+      const int cell =
+          LDS3 ? cell_index_a(c0, c1, c2, pfast, frot + 12 * gr, adds[3 * gr], adds[3 * gr + 1],
+                              adds[3 * gr + 2], rot + 12 * gr, off + 4 * gr, P)
+               : cell_index(c0, c1, c2, pfast, frot + 12 * gr, rot + 12 * gr, off + 4 * gr, P);
       F[u] = force[(long long)gr * P.cells + cell];
     }
 #pragma unroll
@@ -153,12 +166,20 @@ hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long 
                                 const int* order, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const long long blocks = (n + 255) / 256;
-  if (world)
-    hipLaunchKernelGGL(gather_update_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, pos,
-                       vel, n, rot, off, frot, force, P, avg_out, mode, order);
+  const size_t lds = (size_t)P.k * 3 * sizeof(float);
+  const bool lds3 = lds <= 16 * 1024;  // k <= 1365: a few KB per workgroup
+  if (world && lds3)
+    hipLaunchKernelGGL((gather_update_kernel<true, true>), dim3((unsigned)blocks), dim3(256), lds, s,
+                       pos, vel, n, rot, off, frot, force, P, avg_out, mode, order);
+  else if (world)
+    hipLaunchKernelGGL((gather_update_kernel<true, false>), dim3((unsigned)blocks), dim3(256), 0, s,
+                       pos, vel, n, rot, off, frot, force, P, avg_out, mode, order);
+  else if (lds3)
+    hipLaunchKernelGGL((gather_update_kernel<false, true>), dim3((unsigned)blocks), dim3(256), lds,
+                       s, pos, vel, n, rot, off, frot, force, P, avg_out, mode, order);
   else
-    hipLaunchKernelGGL(gather_update_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, pos,
-                       vel, n, rot, off, frot, force, P, avg_out, mode, order);
+    hipLaunchKernelGGL((gather_update_kernel<false, false>), dim3((unsigned)blocks), dim3(256), 0,
+                       s, pos, vel, n, rot, off, frot, force, P, avg_out, mode, order);
   return hipGetLastError();
 }
 

@@ -41,6 +41,17 @@ static __device__ __attribute__((noinline)) int cell_exact_call(float c0, float 
 // Cell of the centred position (c0,c1,c2) in one grid: the fast path when
 // the particle is admitted (pfast) and v is clear of every cell face,
 // otherwise the exact float32 sequence of the reference (cellmath.h).
+// A wave-uniform value copied into a vector register.  A VOP3 instruction
+// reads at most one scalar operand on this target, so an FMA whose
+// multiplier and addend are both wave-uniform would otherwise re-copy the
+// addend with a v_mov on every use; a kernel that keeps a fast row for a
+// whole particle loop copies its three addends once.
+__device__ __forceinline__ float to_vgpr(float x) {
+  float y;
+  asm("v_mov_b32 %0, %1" : "=v"(y) : "s"(x));
+  return y;
+}
+
 // F points at the grid's fast row (registers or scalar memory), R and O at
 // its packed rotation row and offset in global memory.
 __device__ __forceinline__ int cell_index(float c0, float c1, float c2, int pfast, const float* F,
@@ -48,6 +59,17 @@ __device__ __forceinline__ int cell_index(float c0, float c1, float c2, int pfas
                                           const float* __restrict__ O, const Params& P) {
   int cell;
   const int ok = cell_fast(c0, c1, c2, F, P.lim, P.lg, &cell) & pfast;
+  if (!ok) cell = cell_exact_call(c0, c1, c2, R, O, P.W, P.half, P.h, P.lg);
+  return cell;
+}
+
+// cell_index with the fast row's addends given separately (see cell_fast_a).
+__device__ __forceinline__ int cell_index_a(float c0, float c1, float c2, int pfast,
+                                            const float* F, float a0, float a1, float a2,
+                                            const float* __restrict__ R,
+                                            const float* __restrict__ O, const Params& P) {
+  int cell;
+  const int ok = cell_fast_a(c0, c1, c2, F, a0, a1, a2, P.lim, P.lg, &cell) & pfast;
   if (!ok) cell = cell_exact_call(c0, c1, c2, R, O, P.W, P.half, P.h, P.lg);
   return cell;
 }
