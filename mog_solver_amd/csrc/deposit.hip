@@ -60,6 +60,8 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
 
   const long long p0 = (long long)blockIdx.x * chunk;
   const long long p1 = min(n, p0 + chunk);
+  const unsigned int lds_limit32 =
+      (unsigned int)(lds_limit < 0xFFFFFFFFull ? lds_limit : 0xFFFFFFFFull);
   // UP particles per thread per iteration: their loads are issued together.
   constexpr int UP = 2;
   const long long step = (long long)UP * blockDim.x;
@@ -72,14 +74,19 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
     }
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
-      const unsigned long long milli = milli_of(q[u].w);
-      if (milli == 0ull) continue;
+      // uint(m * 1000) (milli_of): 0 exactly when the float product is below
+      // 1 (or negative, or NaN); while it is below 2^32 a 32-bit truncation
+      // gives it, which spares the 64-bit conversion for every real mass
+      const float mm = q[u].w * 1000.0f;
+      if (!(mm >= 1.0f)) continue;
+      const bool fits = mm < 4294967296.0f;
+      const unsigned int m32 = (unsigned int)mm;
       const float c0 = q[u].x - P.half;
       const float c1 = q[u].y - P.half;
       const float c2 = q[u].z - P.half;
       const int pfast = particle_fast(c0, c1, c2, P.CB);
-      if (milli <= lds_limit) {
-        const unsigned int mv = (unsigned int)milli;
+      if (fits && m32 <= lds_limit32) {
+        const unsigned int mv = m32;
 #pragma unroll
         for (int gi = 0; gi < G; ++gi) {
           if (gi < ng) {
@@ -89,6 +96,7 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
           }
         }
       } else {
+        const unsigned long long milli = milli_of(q[u].w);
 #pragma unroll
         for (int gi = 0; gi < G; ++gi) {
           if (gi < ng) {
