@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define MOG_ABI_VERSION 1
+#define MOG_ABI_VERSION 2
 
 typedef enum mog_status {
   MOG_OK = 0,
@@ -152,27 +152,31 @@ int mog_init_perlin(float* pos_n4, float* vel_n4, int64_t n, double world_size,
 
 /* ---- Learned closure (no counterpart in the reference) ----------------
  * Features of the ensemble of `ctx` (typically few grids, e.g. k = 24) at
- * the particles: runs the context's clear, deposit and spectral solve, then
- * writes feat (n,8) = (m/S, s/S, log(1 + rho/rho0), S) and mean (n,4) =
- * (m, 0), with m the mean and s the across-grid standard deviation of the
- * world-frame grid forces, rho the mean mass of the particle's cells and
- * S = |m| + |s| a per-particle force scale. */
+ * the particles: runs the context's clear, deposit and spectral solve, a
+ * second solve with the Green's function divided by the NGP window
+ * sinc(k_x h/2) sinc(k_y h/2) sinc(k_z h/2), and writes
+ * feat (n,12) = (m/S, s/S, c/S, log(1 + rho/rho0), S, 0) and
+ * mean (n,4) = (c, 0), with m the mean and s the across-grid standard
+ * deviation of the world-frame NGP grid forces, c the mean trilinear
+ * interpolation of the deconvolved world-frame force fields, rho the mean
+ * mass of the particle's cells and S = |c| + |s| a per-particle force
+ * scale. */
 int mog_closure_features(mog_ctx* ctx, const float* pos_n4, int64_t n, double rho0,
-                         float* feat_n8, float* mean_n4, void* stream);
+                         float* feat_n12, float* mean_n4, void* stream);
 
-/* Number of fp32 parameters of the closure MLP (7 -> 64 -> 64 -> 64 -> 3,
+/* Number of fp32 parameters of the closure MLP (10 -> 64 -> 64 -> 64 -> 3,
  * SiLU; W1 b1 W2 b2 W3 b3 W4 b4 with W[in][out] row-major). */
 int mog_mlp_param_count(void);
 
 /* force (n,4) = mean + S * MLP(x) on the matrix cores (bf16 != 0: BF16
- * operands, else FP32; FP32 accumulation). */
-int mog_mlp_forward(const float* params, const float* feat_n8, const float* mean_n4, int64_t n,
+ * operands, else FP32; FP32 accumulation); x = feat[0..9], S = feat[10]. */
+int mog_mlp_forward(const float* params, const float* feat_n12, const float* mean_n4, int64_t n,
                     float* force_n4, int32_t bf16, void* stream);
 
 /* Fused forward + backward over n particles: adds dLoss/dparams to grad
  * and the loss to *loss (device pointers; the caller clears them), with
- * loss = wnorm * sum_p S_p^2 |MLP(x_p) - (target_p - m_p)/S_p|^2. */
-int mog_mlp_train_grad(const float* params, const float* feat_n8, const float* mean_n4,
+ * loss = wnorm * sum_p S_p^2 |MLP(x_p) - (target_p - mean_p)/S_p|^2. */
+int mog_mlp_train_grad(const float* params, const float* feat_n12, const float* mean_n4,
                        const float* target_n4, int64_t n, double wnorm, float* grad, float* loss,
                        int32_t bf16, void* stream);
 

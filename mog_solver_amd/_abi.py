@@ -11,6 +11,7 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libmog.so"
+ABI_VERSION = 2  # MOG_ABI_VERSION of include/mog.h this binding follows
 
 MOG_OK = 0
 MOG_EINVAL = -1
@@ -130,6 +131,9 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     i32 = ctypes.c_int32
     lib.mog_abi_version.restype = ctypes.c_int
     lib.mog_abi_version.argtypes = []
+    if lib.mog_abi_version() != ABI_VERSION:
+        raise ImportError(f"{p} has ABI version {lib.mog_abi_version()}, this binding needs {ABI_VERSION}; "
+                          "rebuild it")
     lib.mog_create.restype = ctypes.c_int
     lib.mog_create.argtypes = [ctypes.POINTER(MogConfig), ctypes.POINTER(vp)]
     lib.mog_destroy.restype = None

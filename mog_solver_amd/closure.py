@@ -37,7 +37,9 @@ def _torch():
     return torch
 
 
-K_IN, K_H, K_OUT = 7, 64, 3
+K_IN, K_H, K_OUT = 10, 64, 3
+N_FEAT = 12  # floats per particle: the K_IN inputs, the scale S, 0
+S_COL = 10
 LAYOUT = [("W1", (K_IN, K_H)), ("b1", (K_H,)), ("W2", (K_H, K_H)), ("b2", (K_H,)),
           ("W3", (K_H, K_H)), ("b3", (K_H,)), ("W4", (K_H, K_OUT)), ("b4", (K_OUT,))]
 
@@ -49,7 +51,7 @@ def rel_rms(a, b):
 
 
 class ClosureMLP:
-    """The closure network: 7 -> 64 -> 64 -> 64 -> 3, SiLU."""
+    """The closure network: 10 -> 64 -> 64 -> 64 -> 3, SiLU."""
 
     def __init__(self, params=None, seed=0, bf16=True, device=None):
         torch = _torch()
@@ -97,7 +99,7 @@ class ClosureMLP:
 
     @staticmethod
     def wnorm(feat):
-        s = feat[:, 7].double()
+        s = feat[:, S_COL].double()
         return 1.0 / max(float((s * s).sum()), 1e-300)
 
     def loss_and_grad(self, feat, mean, target, wnorm=None, stream=None):
@@ -124,17 +126,18 @@ class ClosureMLP:
 
 
 def closure_features(solver_small: MogSolver, pos, rho0, feat=None, mean=None, stream=None):
-    """(feat (n,8), mean (n,4)) of the cheap ensemble at ``pos`` (written into
-    ``feat``/``mean`` when given)."""
+    """(feat (n,12), mean (n,4)) of the cheap ensemble at ``pos`` (written into
+    ``feat``/``mean`` when given): feat = (m/S, s/S, c/S, log(1 + rho/rho0),
+    S, 0) and mean = (c, 0), c the deconvolved cloud-in-cell estimate."""
     torch = _torch()
     _check_particles(pos, "pos")
     n = pos.shape[0]
     if feat is None:
-        feat = torch.empty((n, 8), dtype=torch.float32, device=pos.device)
+        feat = torch.empty((n, N_FEAT), dtype=torch.float32, device=pos.device)
     if mean is None:
         mean = torch.empty((n, 4), dtype=torch.float32, device=pos.device)
-    if tuple(feat.shape) != (n, 8) or tuple(mean.shape) != (n, 4) or not (feat.is_contiguous() and mean.is_contiguous()):
-        raise ValueError("feat must be (n,8) and mean (n,4) contiguous float32")
+    if tuple(feat.shape) != (n, N_FEAT) or tuple(mean.shape) != (n, 4) or not (feat.is_contiguous() and mean.is_contiguous()):
+        raise ValueError("feat must be (n,12) and mean (n,4) contiguous float32")
     solver_small._check(solver_small.lib.mog_closure_features(solver_small._ctx, _ptr(pos), n, float(rho0),
                                                               _ptr(feat), _ptr(mean), _stream_ptr(stream)))
     return feat, mean
@@ -201,14 +204,27 @@ def train(mlp: ClosureMLP, snaps: Snapshots, epochs=20, lr=2e-3, lr_final=2e-4, 
     return history
 
 
+def ngp_mean(feat):
+    """The cheap ensemble's plain NGP mean force (n,3) from its features."""
+    return feat[:, 0:3] * feat[:, S_COL:S_COL + 1]
+
+
 def evaluate(mlp: ClosureMLP, snaps: Snapshots):
-    """Mean relative RMS error vs the full-k force of the cheap ensemble's mean
-    and of the closure."""
-    plain, closure = [], []
+    """Mean relative RMS error vs the full-k force of the cheap ensemble's
+    base estimate (the deconvolved cloud-in-cell mean) and of the closure."""
+    r = evaluate_all(mlp, snaps)
+    return r["cic"], r["closure"]
+
+
+def evaluate_all(mlp: ClosureMLP, snaps: Snapshots):
+    """Mean relative RMS error vs the full-k force of the cheap ensemble's
+    NGP mean, its deconvolved cloud-in-cell mean and the closure."""
+    ngp, cic, closure = [], [], []
     for f, m, t in zip(snaps.feat, snaps.mean, snaps.target):
-        plain.append(rel_rms(m, t))
+        ngp.append(rel_rms(ngp_mean(f), t))
+        cic.append(rel_rms(m, t))
         closure.append(rel_rms(mlp.forward(f, m), t))
-    return float(np.mean(plain)), float(np.mean(closure))
+    return {"ngp": float(np.mean(ngp)), "cic": float(np.mean(cic)), "closure": float(np.mean(closure))}
 
 
 class ClosureSolver:
@@ -240,7 +256,7 @@ class ClosureSolver:
         torch = _torch()
         n = pos.shape[0]
         if self._buf is None or self._buf[0].shape[0] != n:
-            self._buf = (torch.empty((n, 8), dtype=torch.float32, device=pos.device),
+            self._buf = (torch.empty((n, N_FEAT), dtype=torch.float32, device=pos.device),
                          torch.empty((n, 4), dtype=torch.float32, device=pos.device),
                          torch.empty((n, 4), dtype=torch.float32, device=pos.device))
         return self._buf

@@ -36,6 +36,10 @@ struct mog_ctx {
   float* d_gtab;
   float* d_kd;
   float2* d_conv;  // real-space kernel spectrum (solver convolution, compat physical)
+  // closure features: deconvolved Green's function and world-frame force
+  // grids (allocated by the first mog_closure_features call)
+  float* d_gtab_dc;
+  float4* d_force_dc;
   float* d_Wf;  // MFMA DFT tables (g >= 16): forward, inverse
   float* d_Wi;
   DepositTune tune;
@@ -298,6 +302,8 @@ void release(mog_ctx* c) {
   (void)hipFree(c->d_gtab);
   (void)hipFree(c->d_kd);
   (void)hipFree(c->d_conv);
+  (void)hipFree(c->d_gtab_dc);
+  (void)hipFree(c->d_force_dc);
   (void)hipFree(c->d_order);
   (void)hipFree(c->d_idx);
   (void)hipFree(c->d_keys_a);
@@ -974,17 +980,51 @@ int mog_closure_features(mog_ctx* ctx, const float* pos, int64_t n, double rho0,
   if (!(isfinite(rho0) && rho0 > 0.0)) return fail(ctx, MOG_EINVAL, "rho0 must be finite and > 0");
   hipStream_t s = (hipStream_t)stream;
   const Params& P = ctx->P;
+  if (!ctx->d_gtab_dc) {
+    // the spectral Green's table divided by the NGP window
+    // sinc(k_x h/2) sinc(k_y h/2) sinc(k_z h/2) (Nyquist-positive k)
+    const int g = P.g;
+    std::vector<float> gt((size_t)g * g * g), dc((size_t)g * g * g);
+    std::vector<float2> tw(g);
+    std::vector<float> kd(g);
+    build_tables(&ctx->cfg, tw.data(), gt.data(), kd.data());
+    auto win = [&](int i) {
+      const int sh = i <= g / 2 ? i : i - g;
+      const double x = 0.5 * (double)sh * 2.0 * M_PI / ctx->cfg.world_size * (double)P.h;
+      return sh == 0 ? 1.0 : sin(x) / x;
+    };
+    for (int iz = 0; iz < g; ++iz)
+      for (int iy = 0; iy < g; ++iy)
+        for (int ix = 0; ix < g; ++ix) {
+          const size_t at = ((size_t)iz * g + iy) * g + ix;
+          dc[at] = (float)((double)gt[at] / (win(ix) * win(iy) * win(iz)));
+        }
+    if (dalloc(&ctx->d_gtab_dc, dc.size()) != hipSuccess ||
+        dalloc(&ctx->d_force_dc, (size_t)P.k * P.cells) != hipSuccess) {
+      (void)hipFree(ctx->d_gtab_dc);
+      (void)hipFree(ctx->d_force_dc);
+      ctx->d_gtab_dc = nullptr;
+      ctx->d_force_dc = nullptr;
+      return fail(ctx, MOG_ENOMEM, "mog_closure_features: deconvolved grids");
+    }
+    hipError_t ec = hipMemcpy(ctx->d_gtab_dc, dc.data(), dc.size() * sizeof(float),
+                              hipMemcpyHostToDevice);
+    if (ec != hipSuccess) return hip_fail(ctx, ec, "mog_closure_features");
+  }
   hipError_t e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
   if (e == hipSuccess)
     e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
                        (unsigned long long*)ctx->d_grids, ctx->tune, s);
   if (e == hipSuccess) e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
+  if (e == hipSuccess)
+    e = launch_solve(P, ctx->d_grids, ctx->d_S1, ctx->d_S2, ctx->d_force_dc, ctx->d_tw,
+                     ctx->d_gtab_dc, ctx->d_kd, nullptr, ctx->need_x, ctx->d_rot, 1, s);
   const int* order = nullptr;
   if (e == hipSuccess) e = refresh_order(ctx, pos, n, s, &order);
   if (e == hipSuccess)
     e = launch_closure_features(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                                ctx->d_force, ctx->d_grids, (float)rho0, order, (float4*)feat,
-                                (float4*)mean, s);
+                                ctx->d_force, ctx->d_force_dc, ctx->d_grids, (float)rho0, order,
+                                (float4*)feat, (float4*)mean, s);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_closure_features");
 }
 

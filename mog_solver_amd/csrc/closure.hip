@@ -2,20 +2,29 @@
 // per-particle MLP that predicts the force of a larger ensemble.
 //
 // Features (per particle, from a context with few grids, e.g. k = 24):
-//   m   = mean over grids of the world-frame force M_i F_i(cell)      (3)
-//   s   = across-grid standard deviation of those forces, per axis     (3)
-//   rho = mean over grids of the mass of the particle's cell           (1)
-// Network input x = (m / S, s / S, log(1 + rho / rho0)), S = |m| + |s| + tiny
+//   m   = mean over grids of the world-frame force M_i F_i(cell), NGP    (3)
+//   s   = across-grid standard deviation of those forces, per axis      (3)
+//   c   = mean over grids of the trilinear (cloud-in-cell) interpolation
+//         of the deconvolved world-frame force field: each grid's force
+//         spectrum divided by the NGP window sinc(k_x h/2) sinc(k_y h/2)
+//         sinc(k_z h/2), sampled at the particle's continuous cell
+//         coordinate.  The average of NGP forces over many random offsets
+//         is the field smoothed by one cell; interpolating the deconvolved
+//         field estimates that average from few grids, so c is a much
+//         better base than m (both are inputs)                          (3)
+//   rho = mean over grids of the mass of the particle's cell            (1)
+// Network input x = (m/S, s/S, c/S, log(1 + rho/rho0)), S = |c| + |s| + tiny
 // (a per-particle force scale, so the network is scale-free), output y (3);
-// predicted force F = m + S y.  Layers 7 -> 64 -> 64 -> 64 -> 3 with SiLU
-// activations; parameters fp32, row-major W[in][out], each followed by its
-// bias: W1 b1 W2 b2 W3 b3 W4 b4 (9027 floats).
+// predicted force F = c + S y.  feat holds 12 floats per particle: x[0..9],
+// S, 0.  Layers 10 -> 64 -> 64 -> 64 -> 3 with SiLU activations;
+// parameters fp32, row-major W[in][out], each followed by its bias:
+// W1 b1 W2 b2 W3 b3 W4 b4 (9219 floats).
 //
 // GEMMs run on the matrix cores over tiles of 64 particles held in LDS:
 // BF16 operands (v_mfma_f32_16x16x32_bf16) or FP32 operands
 // (v_mfma_f32_16x16x4_f32), FP32 accumulation either way.  The training
 // kernel is fused: forward (pre-activations kept in LDS), loss
-// sum_p w_p |y_p - t_p|^2 with t = (F_target - m) / S and w = S^2 / mean S^2,
+// sum_p w_p |y_p - t_p|^2 with t = (F_target - c) / S and w = S^2 / mean S^2,
 // backward, and gradient accumulation in LDS, flushed once per workgroup.
 #include "mog_common.h"
 
@@ -24,7 +33,8 @@ namespace mog {
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-constexpr int kIn = 7, kH = 64, kOut = 3;
+constexpr int kIn = 10, kH = 64, kOut = 3;
+constexpr int kXS = 16;  // LDS row stride of a tile's inputs
 constexpr int oW1 = 0, ob1 = oW1 + kIn * kH, oW2 = ob1 + kH, ob2 = oW2 + kH * kH,
               oW3 = ob2 + kH, ob3 = oW3 + kH * kH, oW4 = ob3 + kH, ob4 = oW4 + kH * kOut,
               kParams = ob4 + kOut;
@@ -98,8 +108,9 @@ __device__ void wg_gemm(const Mat A, const Mat B, float* C, int ldc, int M, int 
 __global__ __launch_bounds__(256) void closure_features_kernel(
     const float4* __restrict__ pos, long long n, const float* __restrict__ rot,
     const float* __restrict__ off, const float* __restrict__ frot,
-    const float4* __restrict__ force, const long long* __restrict__ grids, Params P, float rho0,
-    const int* __restrict__ order, float4* __restrict__ feat, float4* __restrict__ mean_out) {
+    const float4* __restrict__ force, const float4* __restrict__ force_dc,
+    const long long* __restrict__ grids, Params P, float rho0, const int* __restrict__ order,
+    float4* __restrict__ feat, float4* __restrict__ mean_out) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   // visiting order (spatially sorted when given): neighbouring lanes read
@@ -108,27 +119,51 @@ __global__ __launch_bounds__(256) void closure_features_kernel(
   const float4 p = pos[i];
   const float c0 = p.x - P.half, c1 = p.y - P.half, c2 = p.z - P.half;
   const int pfast = particle_fast(c0, c1, c2, P.CB);
+  const int gm = P.g - 1;
   // Sums are taken about the first grid's force (shifted data), so the
   // variance does not cancel catastrophically when the spread is small.
   float bx = 0.f, by = 0.f, bz = 0.f;
   float sx = 0.f, sy = 0.f, sz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, sm = 0.f;
+  float cx = 0.f, cy = 0.f, cz = 0.f;
   for (int g = 0; g < P.k; ++g) {
-    const int cell = cell_index(c0, c1, c2, pfast, frot + 12 * g, rot + 12 * g, off + 4 * g, P);
-    const long long at = (long long)g * P.cells + cell;
-    const float4 F = force[at];
+    const float* F = frot + 12 * g;
+    const int cell = cell_index(c0, c1, c2, pfast, F, rot + 12 * g, off + 4 * g, P);
+    const long long base = (long long)g * P.cells;
+    const float4 f = force[base + cell];
     if (g == 0) {
-      bx = F.x;
-      by = F.y;
-      bz = F.z;
+      bx = f.x;
+      by = f.y;
+      bz = f.z;
     }
-    const float ex = F.x - bx, ey = F.y - by, ez = F.z - bz;
+    const float ex = f.x - bx, ey = f.y - by, ez = f.z - bz;
     sx += ex;
     sy += ey;
     sz += ez;
     qx = fmaf(ex, ex, qx);
     qy = fmaf(ey, ey, qy);
     qz = fmaf(ez, ez, qz);
-    sm += (float)grids[at] * 1e-3f;
+    sm += (float)grids[base + cell] * 1e-3f;
+    // continuous cell coordinate u = l/h - 1/2 (cell centres at integers)
+    // from the fast row, then trilinear weights over the 8 surrounding
+    // cell centres of the periodic grid
+    const float u0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, F[3])));
+    const float u1 = fmaf(F[6], c2, fmaf(F[5], c1, fmaf(F[4], c0, F[7])));
+    const float u2 = fmaf(F[10], c2, fmaf(F[9], c1, fmaf(F[8], c0, F[11])));
+    const float fl0 = floorf(u0), fl1 = floorf(u1), fl2 = floorf(u2);
+    const float w0 = u0 - fl0, w1 = u1 - fl1, w2 = u2 - fl2;
+    const int x0 = (int)fl0 & gm, y0 = (int)fl1 & gm, z0 = (int)fl2 & gm;
+    const int x1 = (x0 + 1) & gm, y1 = (y0 + 1) & gm, z1 = (z0 + 1) & gm;
+    const float4* D = force_dc + base;
+#pragma unroll
+    for (int corner = 0; corner < 8; ++corner) {
+      const int xx = corner & 1 ? x1 : x0, yy = corner & 2 ? y1 : y0, zz = corner & 4 ? z1 : z0;
+      const float w = (corner & 1 ? w0 : 1.f - w0) * (corner & 2 ? w1 : 1.f - w1) *
+                      (corner & 4 ? w2 : 1.f - w2);
+      const float4 d = D[(((zz << P.lg) | yy) << P.lg) | xx];
+      cx = fmaf(w, d.x, cx);
+      cy = fmaf(w, d.y, cy);
+      cz = fmaf(w, d.z, cz);
+    }
   }
   const float ik = 1.0f / (float)P.k;
   const float ux = sx * ik, uy = sy * ik, uz = sz * ik;
@@ -136,11 +171,15 @@ __global__ __launch_bounds__(256) void closure_features_kernel(
   const float vx = fmaxf(qx * ik - ux * ux, 0.f), vy = fmaxf(qy * ik - uy * uy, 0.f),
               vz = fmaxf(qz * ik - uz * uz, 0.f);
   const float dx = sqrtf(vx), dy = sqrtf(vy), dz = sqrtf(vz);
-  const float S = sqrtf(mx * mx + my * my + mz * mz) + sqrtf(vx + vy + vz) + 1e-30f;
+  cx *= ik;
+  cy *= ik;
+  cz *= ik;
+  const float S = sqrtf(cx * cx + cy * cy + cz * cz) + sqrtf(vx + vy + vz) + 1e-30f;
   const float iS = 1.0f / S;
-  feat[2 * i] = make_float4(mx * iS, my * iS, mz * iS, dx * iS);
-  feat[2 * i + 1] = make_float4(dy * iS, dz * iS, log1pf(sm * ik / rho0), S);
-  mean_out[i] = make_float4(mx, my, mz, 0.f);
+  feat[3 * i] = make_float4(mx * iS, my * iS, mz * iS, dx * iS);
+  feat[3 * i + 1] = make_float4(dy * iS, dz * iS, cx * iS, cy * iS);
+  feat[3 * i + 2] = make_float4(cz * iS, log1pf(sm * ik / rho0), S, 0.f);
+  mean_out[i] = make_float4(cx, cy, cz, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -151,26 +190,26 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restric
                                                           const float4* __restrict__ feat,
                                                           const float4* __restrict__ mean,
                                                           long long n, float4* __restrict__ out) {
-  __shared__ float X[TILE * 8];
+  __shared__ float X[TILE * kXS];
   __shared__ float H0[TILE * kH];
   __shared__ float H1[TILE * kH];
   __shared__ float Y[TILE * 16];
   const long long ntile = (n + TILE - 1) / TILE;
   for (long long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
     const long long base = tile * TILE;
-    for (int i = threadIdx.x; i < TILE * 8; i += blockDim.x) {
-      const long long p = base + i / 8;
-      const int f = i % 8;
+    for (int i = threadIdx.x; i < TILE * kXS; i += blockDim.x) {
+      const long long p = base + i / kXS;
+      const int f = i % kXS;
       float v = 0.f;
       if (p < n && f < kIn) {
-        const float4 a = feat[2 * p + f / 4];
+        const float4 a = feat[3 * p + f / 4];
         v = (&a.x)[f % 4];
       }
       X[i] = v;
     }
     __syncthreads();
     // layer 1: Z1 = X W1 + b1 (bias added after); H0 holds Z1
-    wg_gemm<BF16>(Mat{X, 8, 1, TILE, kIn, false}, Mat{prm + oW1, kH, 1, kIn, kH, false}, H0, kH,
+    wg_gemm<BF16>(Mat{X, kXS, 1, TILE, kIn, false}, Mat{prm + oW1, kH, 1, kIn, kH, false}, H0, kH,
                   TILE, kH, 32, TILE, kH, false);
     __syncthreads();
     for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) H0[i] += prm[ob1 + i % kH];
@@ -191,7 +230,7 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restric
     for (int r = threadIdx.x; r < TILE; r += blockDim.x) {
       const long long p = base + r;
       if (p < n) {
-        const float sc = feat[2 * p + 1].w;
+        const float sc = feat[3 * p + 2].z;
         const float4 m = mean[p];
         out[p] = make_float4(fmaf(sc, Y[r * 16 + 0] + prm[ob4 + 0], m.x),
                              fmaf(sc, Y[r * 16 + 1] + prm[ob4 + 1], m.y),
@@ -286,20 +325,34 @@ __global__ __launch_bounds__(256) void mlp_forward_reg_kernel(const float* __res
   for (long long gi = (long long)blockIdx.x * wpb + (threadIdx.x >> 6); gi < groups; gi += nwaves) {
     const long long p = gi * 16 + r;
     const bool live = p < n;
-    float4 fa = make_float4(0.f, 0.f, 0.f, 0.f), fb = fa;
+    // lane group 0 supplies inputs 0..7, group 1 inputs 8, 9 and the bias
+    // slot 10 (= 1), groups 2 and 3 zeros
+    float4 fa = make_float4(0.f, 0.f, 0.f, 0.f), fb = fa, fc = fa;
     if (live && q == 0) {
-      fa = feat[2 * p];
-      fb = feat[2 * p + 1];
+      fa = feat[3 * p];
+      fb = feat[3 * p + 1];
     }
+    if (live && q <= 1) fc = feat[3 * p + 2];
     v8bf x;
-    x[0] = (__bf16)fa.x;
-    x[1] = (__bf16)fa.y;
-    x[2] = (__bf16)fa.z;
-    x[3] = (__bf16)fa.w;
-    x[4] = (__bf16)fb.x;
-    x[5] = (__bf16)fb.y;
-    x[6] = (__bf16)fb.z;
-    x[7] = (__bf16)(q == 0 ? 1.f : 0.f);
+    if (q == 0) {
+      x[0] = (__bf16)fa.x;
+      x[1] = (__bf16)fa.y;
+      x[2] = (__bf16)fa.z;
+      x[3] = (__bf16)fa.w;
+      x[4] = (__bf16)fb.x;
+      x[5] = (__bf16)fb.y;
+      x[6] = (__bf16)fb.z;
+      x[7] = (__bf16)fb.w;
+    } else {
+      x[0] = (__bf16)fc.x;
+      x[1] = (__bf16)fc.y;
+      x[2] = (__bf16)(q == 1 ? 1.f : 0.f);
+      x[3] = (__bf16)0.f;
+      x[4] = (__bf16)0.f;
+      x[5] = (__bf16)0.f;
+      x[6] = (__bf16)0.f;
+      x[7] = (__bf16)0.f;
+    }
     v4f h[4], z[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -328,7 +381,7 @@ __global__ __launch_bounds__(256) void mlp_forward_reg_kernel(const float* __res
     }
     if (live && q == 0) {
       const float4 m = mean[p];
-      out[p] = make_float4(fmaf(fb.w, y[0], m.x), fmaf(fb.w, y[1], m.y), fmaf(fb.w, y[2], m.z), 0.f);
+      out[p] = make_float4(fmaf(fc.z, y[0], m.x), fmaf(fc.z, y[1], m.y), fmaf(fc.z, y[2], m.z), 0.f);
     }
   }
 }
@@ -349,8 +402,8 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const float* __restrict_
                                                         float* __restrict__ grad,
                                                         float* __restrict__ loss) {
   extern __shared__ float sm[];
-  float* X = sm;                  // TILE x 8
-  float* Z1 = X + TILE * 8;       // TILE x 64 (pre-activations)
+  float* X = sm;                  // TILE x kXS
+  float* Z1 = X + TILE * kXS;     // TILE x 64 (pre-activations)
   float* Z2 = Z1 + TILE * kH;
   float* Z3 = Z2 + TILE * kH;
   float* DY = Z3 + TILE * kH;     // TILE x 16
@@ -364,19 +417,19 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const float* __restrict_
   const long long ntile = (n + TILE - 1) / TILE;
   for (long long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
     const long long base = tile * TILE;
-    for (int i = threadIdx.x; i < TILE * 8; i += blockDim.x) {
-      const long long p = base + i / 8;
-      const int f = i % 8;
+    for (int i = threadIdx.x; i < TILE * kXS; i += blockDim.x) {
+      const long long p = base + i / kXS;
+      const int f = i % kXS;
       float v = 0.f;
       if (p < n && f < kIn) {
-        const float4 a = feat[2 * p + f / 4];
+        const float4 a = feat[3 * p + f / 4];
         v = (&a.x)[f % 4];
       }
       X[i] = v;
     }
     __syncthreads();
     // forward
-    wg_gemm<BF16>(Mat{X, 8, 1, TILE, kIn, false}, Mat{prm + oW1, kH, 1, kIn, kH, false}, Z1, kH,
+    wg_gemm<BF16>(Mat{X, kXS, 1, TILE, kIn, false}, Mat{prm + oW1, kH, 1, kIn, kH, false}, Z1, kH,
                   TILE, kH, 32, TILE, kH, false);
     __syncthreads();
     for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) Z1[i] += prm[ob1 + i % kH];
@@ -401,7 +454,7 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const float* __restrict_
       for (int c = 0; c < 16; ++c) {
         float d = 0.f;
         if (p < n && c < kOut) {
-          const float S = feat[2 * p + 1].w;
+          const float S = feat[3 * p + 2].z;
           const float4 m = mean[p], tg = target[p];
           const float t = ((&tg.x)[c] - (&m.x)[c]) / S;
           const float w = S * S * wnorm;
@@ -454,7 +507,7 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const float* __restrict_
     for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) D1[i] *= dsilu(Z1[i]);
     __syncthreads();
     // layer 1: dW1 += X^T D1, db1 += sum D1
-    wg_gemm<BF16>(Mat{X, 1, 8, kIn, TILE, false}, Mat{D1, kH, 1, TILE, kH, false}, G + oW1, kH, 16,
+    wg_gemm<BF16>(Mat{X, 1, kXS, kIn, TILE, false}, Mat{D1, kH, 1, TILE, kH, false}, G + oW1, kH, 16,
                   kH, TILE, kIn, kH, true);
     for (int i = threadIdx.x; i < kH; i += blockDim.x) {
       float s = 0.f;
@@ -541,11 +594,12 @@ hipError_t launch_direct_sum(const float4* pos, long long n, const int* targets,
 
 hipError_t launch_closure_features(const Params& P, const float4* pos, long long n,
                                    const float* rot, const float* off, const float* frot,
-                                   const float4* force, const long long* grids, float rho0,
-                                   const int* order, float4* feat, float4* mean, hipStream_t s) {
+                                   const float4* force, const float4* force_dc,
+                                   const long long* grids, float rho0, const int* order,
+                                   float4* feat, float4* mean, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(closure_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     pos, n, rot, off, frot, force, grids, P, rho0, order, feat, mean);
+                     pos, n, rot, off, frot, force, force_dc, grids, P, rho0, order, feat, mean);
   return hipGetLastError();
 }
 
@@ -576,7 +630,7 @@ hipError_t launch_mlp_train(const float* prm, const float4* feat, const float4* 
                             const float4* target, long long n, float wnorm, float* grad,
                             float* loss, bool bf16, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const size_t lds = (size_t)(TILE * 8 + 3 * TILE * kH + TILE * 16 + 2 * TILE * kH + kParams + 1) *
+  const size_t lds = (size_t)(TILE * kXS + 3 * TILE * kH + TILE * 16 + 2 * TILE * kH + kParams + 1) *
                      sizeof(float);
   const void* fn = bf16 ? (const void*)mlp_train_kernel<true> : (const void*)mlp_train_kernel<false>;
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);

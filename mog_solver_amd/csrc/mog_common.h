@@ -167,8 +167,9 @@ hipError_t launch_init_perlin(float4* pos, float4* vel, long long n, float W, fl
                               int sphere, hipStream_t s);
 hipError_t launch_closure_features(const Params& P, const float4* pos, long long n,
                                    const float* rot, const float* off, const float* frot,
-                                   const float4* force, const long long* grids, float rho0,
-                                   const int* order, float4* feat, float4* mean, hipStream_t s);
+                                   const float4* force, const float4* force_dc,
+                                   const long long* grids, float rho0, const int* order,
+                                   float4* feat, float4* mean, hipStream_t s);
 size_t mlp_param_count();
 hipError_t launch_direct_sum(const float4* pos, long long n, const int* targets, int nt, double W,
                              double G_eff, double eps2, double* out, hipStream_t s);

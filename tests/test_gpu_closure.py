@@ -49,30 +49,38 @@ def test_features_match_oracle(torch_cuda, compat):
     feat = feat.cpu().numpy().astype(np.float64)
     mean = mean.cpu().numpy().astype(np.float64)
     want_f, want_m = C.features(pos, rot, off, k, g, W, G, rho0, compat=compat)
+    # mean = cloud-in-cell mean of the deconvolved grids: float32 rounding of
+    # the interpolation weights and of the grid values
     scale = np.sqrt(np.mean(np.sum(want_m**2, 1)))
     assert np.sqrt(np.mean(np.sum((mean[:, :3] - want_m) ** 2, 1))) / scale < 1e-5
     assert np.all(mean[:, 3] == 0)
     # scale-free features: absolute error (values are O(1))
     assert np.max(np.abs(feat[:, 0:3] - want_f[:, 0:3])) < 1e-4
     assert np.max(np.abs(feat[:, 3:6] - want_f[:, 3:6])) < 2e-3
-    np.testing.assert_allclose(feat[:, 6], want_f[:, 6], rtol=1e-5, atol=1e-6)
-    np.testing.assert_allclose(feat[:, 7], want_f[:, 7], rtol=1e-4)
+    assert np.max(np.abs(feat[:, 6:9] - want_f[:, 6:9])) < 1e-4
+    np.testing.assert_allclose(feat[:, 9], want_f[:, 9], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(feat[:, C.S_COL], want_f[:, C.S_COL], rtol=1e-4)
+    assert np.all(feat[:, 11] == 0)
 
 
 def random_inputs(n, seed):
+    """Closure inputs with the feature layout (m/S, s/S, c/S, log1p(rho),
+    S, 0) and mean = c."""
     rng = np.random.default_rng(seed)
-    feat = np.zeros((n, 8), np.float32)
+    feat = np.zeros((n, 12), np.float32)
     m = rng.standard_normal((n, 3))
+    c = m + 0.1 * rng.standard_normal((n, 3))
     sd = np.abs(rng.standard_normal((n, 3))) * 0.3
-    S = np.linalg.norm(m, axis=1) + np.linalg.norm(sd, axis=1)
+    S = np.linalg.norm(c, axis=1) + np.linalg.norm(sd, axis=1)
     feat[:, 0:3] = m / S[:, None]
     feat[:, 3:6] = sd / S[:, None]
-    feat[:, 6] = np.log1p(rng.exponential(1.0, n))
-    feat[:, 7] = S * rng.uniform(0.5, 2.0, n)
+    feat[:, 6:9] = c / S[:, None]
+    feat[:, 9] = np.log1p(rng.exponential(1.0, n))
+    feat[:, C.S_COL] = S * rng.uniform(0.5, 2.0, n)
     mean = np.zeros((n, 4), np.float32)
-    mean[:, :3] = m
+    mean[:, :3] = c
     target = np.zeros((n, 4), np.float32)
-    target[:, :3] = m + feat[:, 7:8] * 0.2 * rng.standard_normal((n, 3))
+    target[:, :3] = c + feat[:, C.S_COL:C.S_COL + 1] * 0.2 * rng.standard_normal((n, 3))
     return feat, mean, target
 
 
@@ -87,7 +95,7 @@ def test_mlp_forward_matches_oracle(torch_cuda, bf16, tol, n):
     mlp = ClosureMLP(params, bf16=bf16)
     got = mlp.forward(dev(torch, feat), dev(torch, mean)).cpu().numpy().astype(np.float64)
     want = C.predict(params, feat, mean)
-    S = feat[:, 7:8].astype(np.float64)
+    S = feat[:, C.S_COL:C.S_COL + 1].astype(np.float64)
     y_err = (got[:, :3] - want) / S
     y_ref = (want - mean[:, :3]) / S
     rel = math.sqrt(np.mean(np.sum(y_err**2, 1)) / np.mean(np.sum(y_ref**2, 1)))
@@ -103,7 +111,7 @@ def test_mlp_gradient_matches_oracle(torch_cuda, bf16, tol, n):
     torch = torch_cuda
     feat, mean, target = random_inputs(n, seed=100 + n)
     params = C.init_params(seed=11, out_scale=1.0)
-    wnorm = 1.0 / float(np.sum(feat[:, 7].astype(np.float64) ** 2))
+    wnorm = 1.0 / float(np.sum(feat[:, C.S_COL].astype(np.float64) ** 2))
     mlp = ClosureMLP(params, bf16=bf16)
     loss, grad = mlp.loss_and_grad(dev(torch, feat), dev(torch, mean), dev(torch, target), wnorm=wnorm)
     grad = grad.cpu().numpy().astype(np.float64)

@@ -83,7 +83,8 @@ def test_library_exports_every_header_symbol():
     lib = _abi.load()
     for name in declared:
         assert hasattr(lib, name)
-    assert lib.mog_abi_version() == 1
+    version = re.search(r"#define MOG_ABI_VERSION (\d+)", (ROOT / "include" / "mog.h").read_text())
+    assert lib.mog_abi_version() == int(version.group(1)) == _abi.ABI_VERSION
 
 
 def test_config_struct_layout():
