@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void closure_features_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// MLP inference: F = m + S * MLP(x)
+// MLP inference: F = c + S * MLP(x) (c: the features' base estimate, "mean")
 // ---------------------------------------------------------------------------
 template <bool BF16>
 __global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restrict__ prm,
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restric
 // q is defined to be feature kperm(c, q, j) = 16(2c + j/4) + 4q + j%4 --
 // exactly the lane's own accumulator h[2c + j/4][j%4] -- and the weight
 // fragments are loaded in the same order.  No LDS round trip, no shuffles.
-// Layer 1's bias rides in the padded K slot 7 (input 1.0).
+// Layer 1's bias rides in the padded K slot 10 (input 1.0).
 __device__ __forceinline__ int kperm(int c, int q, int j) {
   return 16 * (2 * c + (j >> 2)) + 4 * q + (j & 3);
 }

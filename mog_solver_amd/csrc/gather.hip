@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
           LDS3 ? cell_index_a(c0, c1, c2, pfast, frot + 12 * gr, adds[3 * gr], adds[3 * gr + 1],
                               adds[3 * gr + 2], rot + 12 * gr, off + 4 * gr, P)
                : cell_index(c0, c1, c2, pfast, frot + 12 * gr, rot + 12 * gr, off + 4 * gr, P);
-      F[u] = force[(long long)gr * P.cells + cell];
+      F[u] = cell_load(force + (long long)gr * P.cells, cell);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
   for (; grid < P.k; ++grid) {
     const int cell =
         cell_index(c0, c1, c2, pfast, frot + 12 * grid, rot + 12 * grid, off + 4 * grid, P);
-    const float4 F = force[(long long)grid * P.cells + cell];
+    const float4 F = cell_load(force + (long long)grid * P.cells, cell);
     if (WORLD) {
       tx += F.x;
       ty += F.y;

@@ -74,6 +74,13 @@ __device__ __forceinline__ int cell_index_a(float c0, float c1, float c2, int pf
   return cell;
 }
 
+// Cell `cell` (0 <= cell < 2^28) of a grid of float4: a 32-bit byte offset
+// on a wave-uniform base, so the load takes the scalar-base addressing form.
+__device__ __forceinline__ float4 cell_load(const float4* __restrict__ grid, int cell) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(grid) +
+                                          ((unsigned)cell << 4));
+}
+
 // Position of the cached cell of (slot t, grid) in the uint16 cell cache:
 // tiles of 64 slots (one wave) x k grids, so a wave's deposit writes and its
 // gather reads whole contiguous rows.
