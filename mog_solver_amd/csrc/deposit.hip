@@ -32,20 +32,44 @@ hipError_t launch_fill_zero(long long* p, long long count, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Deferred (particle, grid) pairs per workgroup: pairs that the fast cell
+// test cannot decide (near a face, or a particle outside the fast bound).
+constexpr int kDeferCap = 512;
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ unsigned int lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned int)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
+}
+
 // LDS-privatised deposit; grid (chunks, ceil(k / G)).  Dynamic LDS holds
 // G * cells uint32 counters.  The G fast rows are wave-uniform and live in
 // scalar registers for the whole chunk.
+//
+// Every lane adds `decided ? milli : 0` to its fast cell (always in range);
+// the pairs the fast test cannot decide (about 1e-3 of them) are appended to
+// an LDS queue, and after the particle loop whole waves run the exact
+// float32 sequence for the queued pairs.  The loop itself then makes no
+// function call, so the next iteration's particle loads stay in flight
+// across it (a call would wait for them), and no wave runs the exact
+// sequence for a single lane.  A full queue falls back to deciding in place.
+// Particles whose milli-mass does not fit the 32-bit LDS partials take the
+// 64-bit global path (rare: huge masses).
 template <int G>
 __global__ __launch_bounds__(1024) void deposit_lds_kernel(
     const float4* __restrict__ pos, long long n, long long chunk,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
     Params P, unsigned long long lds_limit, unsigned long long* __restrict__ grids) {
   extern __shared__ unsigned int hist[];
+  __shared__ unsigned int defer_p[kDeferCap];  // particle offset within the chunk
+  __shared__ unsigned char defer_g[kDeferCap];  // grid within the group
+  __shared__ unsigned int defer_n;
   const int cells = P.cells;
   const int grid0 = blockIdx.y * G;
   const int ng = min(G, P.k - grid0);
   const int total = ng * cells;
   for (int i = threadIdx.x; i < total; i += blockDim.x) hist[i] = 0u;
+  if (threadIdx.x == 0) defer_n = 0u;
 
   float F[G][12];
 #pragma unroll
@@ -65,48 +89,118 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
   // UP particles per thread per iteration: their loads are issued together.
   constexpr int UP = 2;
   const long long step = (long long)UP * blockDim.x;
+  // the next iteration's particles are loaded while this one computes
+  float4 qn[UP];
+#pragma unroll
+  for (int u = 0; u < UP; ++u) {
+    const long long p = p0 + threadIdx.x + (long long)u * blockDim.x;
+    qn[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);  // mass 0 is skipped
+  }
   for (long long pb = p0 + threadIdx.x; pb < p1; pb += step) {
     float4 q[UP];
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
-      const long long p = pb + (long long)u * blockDim.x;
-      q[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);  // mass 0 is skipped
+      q[u] = qn[u];
+      const long long p = pb + step + (long long)u * blockDim.x;
+      qn[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    // all UP x G cells first, with no branch between them (independent
+    // work the scheduler can interleave), then the atomics, then one
+    // wave-uniform test for the rare paths
+    unsigned int m32[UP];
+    bool small[UP], big[UP];
+    float cc[UP][3];
+    int pf[UP];
+    int cell[UP][G];
+    bool ok[UP][G];
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
       // uint(m * 1000) (milli_of): 0 exactly when the float product is below
       // 1 (or negative, or NaN); while it is below 2^32 a 32-bit truncation
       // gives it, which spares the 64-bit conversion for every real mass
       const float mm = q[u].w * 1000.0f;
-      if (!(mm >= 1.0f)) continue;
-      const bool fits = mm < 4294967296.0f;
-      const unsigned int m32 = (unsigned int)mm;
-      const float c0 = q[u].x - P.half;
-      const float c1 = q[u].y - P.half;
-      const float c2 = q[u].z - P.half;
-      const int pfast = particle_fast(c0, c1, c2, P.CB);
-      if (fits && m32 <= lds_limit32) {
-        const unsigned int mv = m32;
+      const bool live = mm >= 1.0f;
+      m32[u] = live && mm < 4294967296.0f ? (unsigned int)mm : 0u;
+      small[u] = live && mm < 4294967296.0f && m32[u] <= lds_limit32;
+      big[u] = live && !small[u];
+      cc[u][0] = q[u].x - P.half;
+      cc[u][1] = q[u].y - P.half;
+      cc[u][2] = q[u].z - P.half;
+      pf[u] = particle_fast(cc[u][0], cc[u][1], cc[u][2], P.CB);
 #pragma unroll
-        for (int gi = 0; gi < G; ++gi) {
-          if (gi < ng) {
-            const int gr = grid0 + gi;
-            const int cell = cell_index(c0, c1, c2, pfast, F[gi], rot + 12 * gr, off + 4 * gr, P);
-            atomicAdd(&hist[gi * cells + cell], mv);
+      for (int gi = 0; gi < G; ++gi)
+        ok[u][gi] = cell_fast_a(cc[u][0], cc[u][1], cc[u][2], F[gi], F[gi][3], F[gi][7],
+                                F[gi][11], P.lim, P.lg, &cell[u][gi]) & pf[u];
+    }
+    // grids gi >= ng (last group) add 0 to allocated, unflushed counters
+    bool rare = false;
+#pragma unroll
+    for (int u = 0; u < UP; ++u) {
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi) {
+        const bool take = gi < ng && small[u];
+        // the addend is hidden from the optimiser: a known-zero add would be
+        // turned into an atomic load and a wait
+        unsigned int v = take && ok[u][gi] ? m32[u] : 0u;
+        asm volatile("" : "+v"(v));
+        atomicAdd(&hist[gi * cells + cell[u][gi]], v);
+        rare |= take && !ok[u][gi];
+      }
+      rare |= big[u];
+    }
+    if (__ballot(rare)) {  // wave-uniform, rare
+#pragma unroll
+      for (int u = 0; u < UP; ++u) {
+        const float c0 = cc[u][0], c1 = cc[u][1], c2 = cc[u][2];
+        const long long p = pb + (long long)u * blockDim.x;
+        if (big[u]) {  // huge mass: the 64-bit global grids
+          const unsigned long long milli = milli_of(q[u].w);
+#pragma unroll
+          for (int gi = 0; gi < G; ++gi) {
+            if (gi < ng) {
+              const int gr = grid0 + gi;
+              const int c = cell_index(c0, c1, c2, pf[u], F[gi], rot + 12 * gr, off + 4 * gr, P);
+              atomicAdd(&grids[(long long)gr * cells + c], milli);
+            }
           }
         }
-      } else {
-        const unsigned long long milli = milli_of(q[u].w);
 #pragma unroll
         for (int gi = 0; gi < G; ++gi) {
-          if (gi < ng) {
-            const int gr = grid0 + gi;
-            const int cell = cell_index(c0, c1, c2, pfast, F[gi], rot + 12 * gr, off + 4 * gr, P);
-            atomicAdd(&grids[(long long)gr * cells + cell], milli);
+          const bool defer = gi < ng && small[u] && !ok[u][gi];
+          const unsigned long long need = __ballot(defer);
+          if (need) {  // queue the undecided pairs: one reservation per wave
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned int base = 0u;
+            if (lane == leader) base = atomicAdd(&defer_n, (unsigned int)__popcll(need));
+            base = __builtin_amdgcn_readlane(base, leader);
+            if (defer) {
+              const unsigned int slot = base + lanes_below(need);
+              if (slot < (unsigned int)kDeferCap) {
+                defer_p[slot] = (unsigned int)(p - p0);
+                defer_g[slot] = (unsigned char)gi;
+              } else {
+                const int gr = grid0 + gi;
+                const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W,
+                                               P.half, P.h, P.lg);
+                atomicAdd(&hist[gi * cells + ex], m32[u]);
+              }
+            }
           }
         }
       }
     }
+  }
+  __syncthreads();
+  // the deferred pairs, one per lane
+  const unsigned int nd = min(defer_n, (unsigned int)kDeferCap);
+  for (unsigned int d = threadIdx.x; d < nd; d += blockDim.x) {
+    const int gi = defer_g[d];
+    const int gr = grid0 + gi;
+    const float4 q = pos[p0 + defer_p[d]];
+    const int cell = cell_exact_call(q.x - P.half, q.y - P.half, q.z - P.half, rot + 12 * gr,
+                                     off + 4 * gr, P.W, P.half, P.h, P.lg);
+    atomicAdd(&hist[gi * cells + cell], (unsigned int)(q.w * 1000.0f));
   }
   __syncthreads();
   unsigned long long* dst = grids + (long long)grid0 * cells;
@@ -254,7 +348,8 @@ static hipError_t launch_lds(const Params& P, const float4* pos, long long n, lo
   const void* fn = !sorted ? (const void*)deposit_lds_kernel<G>
                    : cells_out ? (const void*)deposit_sorted_kernel<G, true>
                                : (const void*)deposit_sorted_kernel<G, false>;
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+  // the exact dynamic size: the kernels also hold static LDS
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   const dim3 grid((unsigned)chunks, (unsigned)groups);
   if (!sorted)

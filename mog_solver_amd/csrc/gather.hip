@@ -54,15 +54,36 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
   float tx = 0.f, ty = 0.f, tz = 0.f;
   int grid = 0;
   for (; grid + U <= P.k; grid += U) {
-    float4 F[U];
+    // the U fast cells (always in range) and their force fetches first, then
+    // one wave-uniform test: only a wave with an undecided pair re-fetches
+    // those pairs from the exact cells (the exact sequence is a call, whose
+    // entry waits for every load in flight)
+    int cell[U];
+    bool ok[U];
+    bool rare = false;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int gr = grid + u;
-      const int cell =
-          LDS3 ? cell_index_a(c0, c1, c2, pfast, frot + 12 * gr, adds[3 * gr], adds[3 * gr + 1],
-                              adds[3 * gr + 2], rot + 12 * gr, off + 4 * gr, P)
-               : cell_index(c0, c1, c2, pfast, frot + 12 * gr, rot + 12 * gr, off + 4 * gr, P);
-      F[u] = cell_load(force + (long long)gr * P.cells, cell);
+      const float* Fr = frot + 12 * gr;
+      ok[u] = (LDS3 ? cell_fast_a(c0, c1, c2, Fr, adds[3 * gr], adds[3 * gr + 1],
+                                  adds[3 * gr + 2], P.lim, P.lg, &cell[u])
+                    : cell_fast(c0, c1, c2, Fr, P.lim, P.lg, &cell[u])) &
+              pfast;
+      rare |= !ok[u];
+    }
+    float4 F[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) F[u] = cell_load(force + (long long)(grid + u) * P.cells, cell[u]);
+    if (__ballot(rare)) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!ok[u]) {
+          const int gr = grid + u;
+          const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half,
+                                         P.h, P.lg);
+          F[u] = cell_load(force + (long long)gr * P.cells, ex);
+        }
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
