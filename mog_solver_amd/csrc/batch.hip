@@ -19,6 +19,72 @@
 
 namespace mog {
 
+// Deferred (particle, grid) pairs per task: pairs that the fast cell test
+// cannot decide, settled by whole waves at the end of the task.
+constexpr int kBatchDeferCap = 512;
+
+// One group of GU grids (gi0 .. gi0+GU-1, all < ng) of one particle: the GU
+// fast cells with no branch between them, their atomics, then one
+// wave-uniform test for the rare paths (undecided pairs go to the task's
+// queue; huge masses to the 64-bit global grids).
+template <int GU>
+__device__ __forceinline__ void deposit_group(
+    int gi0, int gbase, int cells, const Params& P, float c0, float c1, float c2,
+    int pfast, bool small, bool big, unsigned int m32, float mass, unsigned int pofs,
+    const float* __restrict__ frot, const float* __restrict__ rot, const float* __restrict__ off,
+    unsigned int* hist, unsigned long long* __restrict__ dst, unsigned int* defer_p,
+    unsigned short* defer_g, unsigned int* defer_n) {
+  int cell[GU];
+  bool ok[GU];
+  bool rare = big;
+#pragma unroll
+  for (int u = 0; u < GU; ++u)
+    ok[u] = cell_fast(c0, c1, c2, frot + 12 * (gbase + gi0 + u), P.lim, P.lg, &cell[u]) & pfast;
+#pragma unroll
+  for (int u = 0; u < GU; ++u) {
+    // hidden from the optimiser: a known-zero add would become an atomic
+    // load and a wait
+    unsigned int v = small && ok[u] ? m32 : 0u;
+    asm volatile("" : "+v"(v));
+    atomicAdd(&hist[(gi0 + u) * cells + cell[u]], v);
+    rare |= small && !ok[u];
+  }
+  if (!__ballot(rare)) return;  // wave-uniform
+#pragma unroll
+  for (int u = 0; u < GU; ++u) {
+    const int gi = gi0 + u;
+    const int gr = gbase + gi;
+    if (big) {
+      const int c = ok[u] ? cell[u]
+                          : cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half,
+                                            P.h, P.lg);
+      atomicAdd(&dst[(long long)gi * cells + c], milli_of(mass));
+    }
+    const bool defer = small && !ok[u];
+    const unsigned long long need = __ballot(defer);
+    if (need) {  // queue: one reservation per wave
+      const int lane = threadIdx.x & 63;
+      const int leader = __ffsll((long long)need) - 1;
+      unsigned int base = 0u;
+      if (lane == leader) base = atomicAdd(defer_n, (unsigned int)__popcll(need));
+      base = __builtin_amdgcn_readlane(base, leader);
+      if (defer) {
+        const unsigned int slot =
+            base + __builtin_amdgcn_mbcnt_hi((unsigned int)(need >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((unsigned int)need, 0u));
+        if (slot < (unsigned int)kBatchDeferCap) {
+          defer_p[slot] = pofs;
+          defer_g[slot] = (unsigned short)gi;
+        } else {
+          const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half,
+                                         P.h, P.lg);
+          atomicAdd(&hist[gi * cells + ex], m32);
+        }
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(1024) void batch_deposit_kernel(
     const float4* __restrict__ pos, const BatchSim* __restrict__ sims,
     const DepositTask* __restrict__ tasks, int ntasks, const float* __restrict__ rot,
@@ -26,8 +92,14 @@ __global__ __launch_bounds__(1024) void batch_deposit_kernel(
     unsigned long long* __restrict__ grids, unsigned long long* __restrict__ counter) {
   extern __shared__ unsigned int hist[];
   __shared__ int cur;
+  __shared__ unsigned int defer_p[kBatchDeferCap];  // particle offset within the task
+  __shared__ unsigned short defer_g[kBatchDeferCap];  // grid within the task
+  __shared__ unsigned int defer_n;
   for (;;) {
-    if (threadIdx.x == 0) cur = (int)atomicAdd(counter, 1ull);
+    if (threadIdx.x == 0) {
+      cur = (int)atomicAdd(counter, 1ull);
+      defer_n = 0u;
+    }
     __syncthreads();
     const int ti = cur;
     if (ti >= ntasks) break;  // uniform across the workgroup
@@ -35,34 +107,57 @@ __global__ __launch_bounds__(1024) void batch_deposit_kernel(
     const BatchSim& b = sims[task.sim];
     const Params P = b.P;
     const int cells = P.cells;
-    const int total = task.ng * cells;
+    const int ng = task.ng;
+    const int total = ng * cells;
     for (int i = threadIdx.x; i < total; i += blockDim.x) hist[i] = 0u;
     __syncthreads();
     const long long p0 = b.p_off + task.p0, p1 = b.p_off + task.p1;
     const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)(task.p1 - task.p0);
+    const unsigned int lds_limit32 =
+        (unsigned int)(lds_limit < 0xFFFFFFFFull ? lds_limit : 0xFFFFFFFFull);
     const int gbase = b.grid_off + task.grid0;
     unsigned long long* dst = grids + b.cell_off + (long long)task.grid0 * cells;
+    // the next particle is loaded while this one is deposited
+    float4 qn = p0 + threadIdx.x < p1 ? pos[p0 + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
     for (long long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-      const float4 q = pos[p];
-      const unsigned long long milli = milli_of(q.w);
-      if (milli == 0ull) continue;
+      const float4 q = qn;
+      qn = p + blockDim.x < p1 ? pos[p + blockDim.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+      // milli-mass as in the single-simulation deposit: 32-bit LDS partials
+      // while they cannot overflow, the 64-bit global grids otherwise
+      const float mm = q.w * 1000.0f;
+      const bool live = mm >= 1.0f;
+      const unsigned int m32 = live && mm < 4294967296.0f ? (unsigned int)mm : 0u;
+      const bool small = live && mm < 4294967296.0f && m32 <= lds_limit32;
+      const bool big = live && !small;
       const float c0 = q.x - P.half, c1 = q.y - P.half, c2 = q.z - P.half;
       const int pfast = particle_fast(c0, c1, c2, P.CB);
-      for (int gi = 0; gi < task.ng; ++gi) {
-        const int gr = gbase + gi;
-        const int cell = cell_index(c0, c1, c2, pfast, frot + 12 * gr, rot + 12 * gr, off + 4 * gr, P);
-        if (milli <= lds_limit)
-          atomicAdd(&hist[gi * cells + cell], (unsigned int)milli);
-        else
-          atomicAdd(&dst[(long long)gi * cells + cell], milli);
-      }
+      int gi0 = 0;
+      for (; gi0 + 4 <= ng; gi0 += 4)
+        deposit_group<4>(gi0, gbase, cells, P, c0, c1, c2, pfast, small, big, m32, q.w,
+                         (unsigned int)(p - p0), frot, rot, off, hist, dst, defer_p, defer_g,
+                         &defer_n);
+      for (; gi0 < ng; ++gi0)
+        deposit_group<1>(gi0, gbase, cells, P, c0, c1, c2, pfast, small, big, m32, q.w,
+                         (unsigned int)(p - p0), frot, rot, off, hist, dst, defer_p, defer_g,
+                         &defer_n);
+    }
+    __syncthreads();
+    // the deferred pairs, one per lane
+    const unsigned int nd = min(defer_n, (unsigned int)kBatchDeferCap);
+    for (unsigned int d = threadIdx.x; d < nd; d += blockDim.x) {
+      const int gi = defer_g[d];
+      const int gr = gbase + gi;
+      const float4 q = pos[p0 + defer_p[d]];
+      const int cell = cell_exact_call(q.x - P.half, q.y - P.half, q.z - P.half, rot + 12 * gr,
+                                       off + 4 * gr, P.W, P.half, P.h, P.lg);
+      atomicAdd(&hist[gi * cells + cell], (unsigned int)(q.w * 1000.0f));
     }
     __syncthreads();
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
       const unsigned int v = hist[i];
       if (v) atomicAdd(&dst[i], (unsigned long long)v);
     }
-    __syncthreads();  // LDS and `cur` are reused by the next task
+    __syncthreads();  // LDS, `cur` and the queue are reused by the next task
   }
 }
 
@@ -108,9 +203,43 @@ __global__ __launch_bounds__(kBatchGatherBlock) void batch_gather_kernel(
     const float c0 = p.x - P.half, c1 = p.y - P.half, c2 = p.z - P.half;
     const int pfast = particle_fast(c0, c1, c2, P.CB);
     float tx = 0.f, ty = 0.f, tz = 0.f;
-    for (int gr = 0; gr < P.k; ++gr) {
+    // groups of U grids: the fast cells and their fetches first, then one
+    // wave-uniform test re-fetches undecided pairs from the exact cells; the
+    // forces are summed in grid order, as in the single-simulation gather
+    constexpr int U = 4;
+    int gr = 0;
+    for (; gr + U <= P.k; gr += U) {
+      int cell[U];
+      bool ok[U];
+      bool rare = false;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ok[u] = cell_fast(c0, c1, c2, FR + 12 * (gr + u), P.lim, P.lg, &cell[u]) & pfast;
+        rare |= !ok[u];
+      }
+      float4 f[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) f[u] = cell_load(F + (long long)(gr + u) * P.cells, cell[u]);
+      if (__ballot(rare)) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (!ok[u]) {
+            const int ex = cell_exact_call(c0, c1, c2, R + 12 * (gr + u), O + 4 * (gr + u), P.W,
+                                           P.half, P.h, P.lg);
+            f[u] = cell_load(F + (long long)(gr + u) * P.cells, ex);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        tx += f[u].x;
+        ty += f[u].y;
+        tz += f[u].z;
+      }
+    }
+    for (; gr < P.k; ++gr) {
       const int cell = cell_index(c0, c1, c2, pfast, FR + 12 * gr, R + 12 * gr, O + 4 * gr, P);
-      const float4 f = F[(long long)gr * P.cells + cell];
+      const float4 f = cell_load(F + (long long)gr * P.cells, cell);
       tx += f.x;
       ty += f.y;
       tz += f.z;
