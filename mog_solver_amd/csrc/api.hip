@@ -339,9 +339,9 @@ DepositTune default_tune(const Params& P) {
   const int per = (int)((32 * 1024) / grid_bytes);
   t.grids_per_wg = per < 1 ? 1 : (per > 8 ? 8 : per);
   // measured optima on MI355X: g=16 (16 KB grids) 512 threads, 2 grids and
-  // 4096 workgroups; g=32 (128 KB) 1024 threads, 1 grid, 1024 workgroups
+  // 2048 workgroups; g=32 (128 KB) 1024 threads, 1 grid, 4096 workgroups
   t.threads = grid_bytes > 64 * 1024 ? 1024 : 512;
-  t.wg_target = grid_bytes > 64 * 1024 ? 1024 : 4096;
+  t.wg_target = grid_bytes > 64 * 1024 ? 4096 : 2048;
   return t;
 }
 

@@ -6,8 +6,10 @@
 // the three inverse transforms (main.py:638-646) and pack.glsl.
 //
 // Each length-g transform is a dense DFT contraction with a twiddle table
-// (FP64-derived, stored FP32).  The 3D transform of a grid is split into
-// three passes over an L2-resident complex scratch buffer:
+// (FP64-derived, stored FP32) for g <= 16, and for g >= 32 a radix-2
+// Stockham FFT over all lines of the slab at once, in LDS (g log2 g instead
+// of g^2 complex multiply-adds per line).  The 3D transform of a grid is
+// split into three passes over an L2-resident complex scratch buffer:
 //   A  per (grid, z) slab: real x-DFT, y-DFT                      -> S1
 //   B  per (grid, ky) slab: z-DFT, Green x i*k, inverse z-DFT      -> S1 (y+iz), S2 (x)
 //   C  per (grid, z) slab: inverse y-DFT, inverse x-DFT, pack float4
@@ -32,6 +34,47 @@ __device__ __forceinline__ float2 cmul_tw(float2 a, float2 w) {
   return make_float2(fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.y, w.x, a.x * w.y));
 }
 
+// Grid sizes from which the slab passes use lds_fft instead of dense DFTs.
+constexpr int kFftMinG = 32;
+
+// Radix-2 Stockham FFT (natural order in and out) of `lines` independent
+// length-g lines held in LDS, element (line l, position p) at p*ps + l*ls.
+// Each stage reads one buffer and writes the other:
+//   v0 = a[j], v1 = a[j + g/2] * e^{-+2 pi i k / (2 Ns)},  k = j mod Ns,
+//   b[(j / Ns) 2 Ns + k] = v0 + v1,  b[(j / Ns) 2 Ns + k + Ns] = v0 - v1,
+// for Ns = 1, 2, .., g/2 (forward: e^-, inverse: e^+, unnormalised).  twl
+// holds (cos, sin)(2 pi t / g).  Returns the buffer that holds the result.
+// Called by every thread of the workgroup; ends with a barrier.  When the
+// lines are the unit-stride direction (ls == 1) consecutive threads take
+// consecutive lines, otherwise consecutive positions, so that a wave's LDS
+// accesses fall in distinct banks.
+__device__ __forceinline__ float2* lds_fft(float2* a, float2* b, int g, int lg, int ps, int ls,
+                                           int lines, const float2* twl, bool inverse) {
+  const int half = g >> 1;
+  const int nb = lines * half;
+  const bool line_fast = ls == 1;
+  for (int st = 0; st < lg; ++st) {
+    const int Ns = 1 << st;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+      const int l = line_fast ? t & (lines - 1) : t >> (lg - 1);
+      const int j = line_fast ? t / lines : t & (half - 1);
+      const float2 v0 = a[j * ps + l * ls];
+      const float2 u = a[(j + half) * ps + l * ls];
+      const int k = j & (Ns - 1);
+      const float2 w = twl[k << (lg - 1 - st)];
+      const float2 v1 = inverse ? cmul_tw(u, w) : cmul_conj_tw(u, w);
+      const int d = ((j >> st) << (st + 1)) + k;
+      b[d * ps + l * ls] = make_float2(v0.x + v1.x, v0.y + v1.y);
+      b[(d + Ns) * ps + l * ls] = make_float2(v0.x - v1.x, v0.y - v1.y);
+    }
+    __syncthreads();
+    float2* c = a;
+    a = b;
+    b = c;
+  }
+  return a;
+}
+
 // Pass A on slab `slab` = (grid, z) of one simulation's arrays.  The pass
 // bodies are device functions so that the batched ensemble's persistent
 // kernels run exactly the same arithmetic as the single-simulation launch.
@@ -41,10 +84,22 @@ __device__ __forceinline__ void spec_fwd_xy_slab(const long long* __restrict__ m
                                                  long long slab) {
   extern __shared__ float2 sm[];
   float2* twl = sm;
-  float* re = (float*)(sm + g);          // g*g reals
-  float2* B = sm + g + (g * g + 1) / 2;  // g*g complex
   const int gg = g * g, m = g - 1;
   const long long base = slab * gg;
+  if (g >= kFftMinG) {  // [y][x] -> x-FFT -> y-FFT -> S[ky][kx]
+    float2* A = sm + g;
+    float2* B2 = A + gg;
+    for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
+    for (int i = threadIdx.x; i < gg; i += blockDim.x)
+      A[i] = make_float2((float)mass[base + i], 0.f);
+    __syncthreads();
+    float2* r = lds_fft(A, B2, g, lg, 1, g, g, twl, false);
+    r = lds_fft(r, r == A ? B2 : A, g, lg, g, 1, g, twl, false);
+    for (int i = threadIdx.x; i < gg; i += blockDim.x) S[base + i] = r[i];
+    return;
+  }
+  float* re = (float*)(sm + g);          // g*g reals
+  float2* B = sm + g + (g * g + 1) / 2;  // g*g complex
   for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < gg; i += blockDim.x) re[i] = (float)mass[base + i];
   __syncthreads();
@@ -91,13 +146,61 @@ __device__ __forceinline__ void spec_z_green_slab(float2* __restrict__ S, float2
                                                   int need_x, long long slab) {
   extern __shared__ float2 sm[];
   float2* twl = sm;
-  float2* L = sm + g;           // [z][kx]
-  float2* Z = L + g * g;        // [kz][kx]  (y + i z spectrum)
-  float2* X = Z + g * g;        // [kz][kx]  (x spectrum)
   const int gg = g * g, m = g - 1;
   const int grid = (int)(slab >> lg);
   const int ky = (int)(slab & m);
   const long long gbase = (long long)grid * g * gg;
+  if (g >= kFftMinG) {
+    // [z][kx] -> z-FFT -> Green into ZX[kz][2g] (Z row then X row) -> one
+    // inverse FFT over the 2g lines of ZX -> S (Z), S2 (X)
+    float2* A = sm + g;      // 2 g*g
+    float2* B2 = A + 2 * gg;  // 2 g*g
+    for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
+    for (int i = threadIdx.x; i < gg; i += blockDim.x) {
+      const int z = i >> lg, kx = i & m;
+      A[i] = S[gbase + (long long)z * gg + ky * g + kx];
+    }
+    __syncthreads();
+    float2* r = lds_fft(A, B2, g, lg, g, 1, g, twl, false);
+    float2* ZX = r == A ? B2 : A;
+    const float kyv = kd[ky];
+    for (int o = threadIdx.x; o < gg; o += blockDim.x) {
+      const int kz = o >> lg, kx = o & m;
+      const float2 acc = r[o];
+      const int fi = (kz * g + ky) * g + kx;
+      float2 zv, xv = make_float2(0.f, 0.f);
+      if (conv) {
+        const float2 t = conv[fi], u = conv[g * gg + fi];
+        zv = make_float2(fmaf(acc.x, t.x, -(acc.y * t.y)), fmaf(acc.x, t.y, acc.y * t.x));
+        xv = make_float2(fmaf(acc.x, u.x, -(acc.y * u.y)), fmaf(acc.x, u.y, acc.y * u.x));
+      } else {
+        const float gf = gtab[fi];
+        const float pr = acc.x * gf, pi = acc.y * gf;
+        const float kzv = kd[kz];
+        zv = make_float2(fmaf(-pr, kzv, -(pi * kyv)), fmaf(pr, kyv, -(pi * kzv)));
+        if (need_x) {
+          const float kxv = kd[kx];
+          xv = make_float2(-(pi * kxv), pr * kxv);
+        }
+      }
+      ZX[kz * 2 * g + kx] = zv;
+      ZX[kz * 2 * g + g + kx] = xv;
+    }
+    __syncthreads();
+    // with conv both halves are needed; without, X only when need_x
+    const int lines = (conv || need_x) ? 2 * g : g;
+    r = lds_fft(ZX, r, g, lg, 2 * g, 1, lines, twl, true);
+    for (int o = threadIdx.x; o < gg; o += blockDim.x) {
+      const int z = o >> lg, kx = o & m;
+      const long long at = gbase + (long long)z * gg + ky * g + kx;
+      S[at] = r[z * 2 * g + kx];
+      if (need_x) S2[at] = r[z * 2 * g + g + kx];
+    }
+    return;
+  }
+  float2* L = sm + g;           // [z][kx]
+  float2* Z = L + g * g;        // [kz][kx]  (y + i z spectrum)
+  float2* X = Z + g * g;        // [kz][kx]  (x spectrum)
   for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < gg; i += blockDim.x) {
     const int z = i >> lg, kx = i & m;
@@ -174,12 +277,49 @@ __device__ __forceinline__ void spec_inv_xy_slab(const float2* __restrict__ S,
                                                  int world, long long slab) {
   extern __shared__ float2 sm[];
   float2* twl = sm;
+  const int gg = g * g, m = g - 1;
+  const long long base = slab * gg;
+  if (g >= kFftMinG) {
+    // ZX[ky][2g] (Z row then X row) -> inverse FFT over ky (2g lines) ->
+    // inverse FFT over kx of the 2g rows (row 2y = Z, 2y+1 = X of line y)
+    float2* A = sm + g;       // 2 g*g
+    float2* B2 = A + 2 * gg;  // 2 g*g
+    for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
+    for (int i = threadIdx.x; i < gg; i += blockDim.x) {
+      const int ky = i >> lg, kx = i & m;
+      A[ky * 2 * g + kx] = S[base + i];
+      A[ky * 2 * g + g + kx] = need_x ? S2[base + i] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const int lines = need_x ? 2 * g : g;
+    float2* r = lds_fft(A, B2, g, lg, 2 * g, 1, lines, twl, true);
+    if (need_x) {
+      r = lds_fft(r, r == A ? B2 : A, g, lg, 1, g, 2 * g, twl, true);
+    } else {
+      r = lds_fft(r, r == A ? B2 : A, g, lg, 1, 2 * g, g, twl, true);
+    }
+    for (int o = threadIdx.x; o < gg; o += blockDim.x) {
+      const int y = o >> lg, x = o & m;
+      const float2 tz = r[y * 2 * g + x];
+      const float fy = tz.x, fz = tz.y;
+      const float fx = need_x ? r[y * 2 * g + g + x].x : fz;
+      const float mag = sqrtf(fmaf(fx, fx, fmaf(fy, fy, fz * fz)));
+      if (world) {
+        const float* R = rot + 12 * (slab >> lg);
+        const float wx = fmaf(R[0], fx, fmaf(R[1], fy, R[2] * fz));
+        const float wy = fmaf(R[4], fx, fmaf(R[5], fy, R[6] * fz));
+        const float wz = fmaf(R[8], fx, fmaf(R[9], fy, R[10] * fz));
+        force[base + o] = make_float4(wx, wy, wz, mag);
+      } else {
+        force[base + o] = make_float4(fx, fy, fz, mag);
+      }
+    }
+    return;
+  }
   float2* Z = sm + g;       // [ky][kx]
   float2* X = Z + g * g;    // [ky][kx]
   float2* TZ = X + g * g;   // [y][kx]
   float2* TX = TZ + g * g;  // [y][kx]
-  const int gg = g * g, m = g - 1;
-  const long long base = slab * gg;
   for (int i = threadIdx.x; i < g; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < gg; i += blockDim.x) {
     Z[i] = S[base + i];
@@ -310,16 +450,25 @@ __global__ __launch_bounds__(256) void batch_inv_xy_kernel(const BatchSim* __res
                    b.P.lg, b.need_x, rot + 12 * b.grid_off, 1, t - cls_off[j]);
 }
 
+// Dynamic LDS bytes of pass 0 (A), 1 (B) or 2 (C) for grids of size up to g
+// (the passes take the FFT layout from kFftMinG on).
+static size_t solve_lds(int g, int pass) {
+  const size_t gg = (size_t)g * g;
+  size_t n;
+  if (g >= kFftMinG)
+    n = g + (pass == 0 ? 2 * gg : 4 * gg);
+  else
+    n = g + (pass == 0 ? (gg + 1) / 2 + gg : pass == 1 ? 3 * gg : 4 * gg);
+  return n * sizeof(float2);
+}
+
 hipError_t launch_batch_solve(const BatchSim* sims, const int* cls_sim, const long long* cls_off,
                               int ncls, long long slabs, int gmax, const long long* grids,
                               float2* S1, float2* S2, float4* force, const float2* tw,
                               const float* gtab, const float* kd, const float2* conv,
                               const float* rot, hipStream_t s) {
   if (slabs <= 0 || ncls <= 0) return hipSuccess;
-  const int gg = gmax * gmax;
-  const size_t ldsA = (size_t)(gmax + (gg + 1) / 2 + gg) * sizeof(float2);
-  const size_t ldsB = (size_t)(gmax + 3 * gg) * sizeof(float2);
-  const size_t ldsC = (size_t)(gmax + 4 * gg) * sizeof(float2);
+  const size_t ldsA = solve_lds(gmax, 0), ldsB = solve_lds(gmax, 1), ldsC = solve_lds(gmax, 2);
   hipError_t e;
   if ((e = hipFuncSetAttribute((const void*)batch_fwd_xy_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)) != hipSuccess)
@@ -346,11 +495,9 @@ hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, flo
                         float4* force, const float2* tw, const float* gtab, const float* kd,
                         const float2* conv, int need_x, const float* rot, int world,
                         hipStream_t s) {
-  const int g = P.g, lg = P.lg, gg = g * g;
+  const int g = P.g, lg = P.lg;
   const unsigned slabs = (unsigned)(P.k * g);
-  const size_t ldsA = (size_t)(g + (gg + 1) / 2 + gg) * sizeof(float2);
-  const size_t ldsB = (size_t)(g + 3 * gg) * sizeof(float2);
-  const size_t ldsC = (size_t)(g + 4 * gg) * sizeof(float2);
+  const size_t ldsA = solve_lds(g, 0), ldsB = solve_lds(g, 1), ldsC = solve_lds(g, 2);
   hipError_t e;
   if ((e = hipFuncSetAttribute((const void*)spec_fwd_xy_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)) != hipSuccess)
