@@ -1,10 +1,13 @@
 """Autotuner of the MOG step's launch and algorithm choices.
 
-Times whole steps (``MogSolver.run`` without graph replay, so the periodic
-rebuild of the visiting order is included) on copies of the caller's state
-for candidate ``mog_tuning`` settings, and keeps the fastest.  The search is
-coordinate descent over the tuning fields, starting from the library's
-defaults:
+Times whole steps (``MogSolver.run`` with graph replay, as the step is run
+in production; the periodic rebuild of the visiting order happens between
+replays and is included) on copies of the caller's state for candidate
+``mog_tuning`` settings, and keeps the fastest.  Each candidate is timed as
+the best of three runs; the three fastest of a stage are then timed again
+over runs four times longer before the stage's winner is taken, so that one
+lucky measurement cannot pick a slower tiling.  The search is coordinate
+descent over the tuning fields, starting from the library's defaults:
 
 1. deposit tiling: threads per workgroup x grids per workgroup x workgroups
 2. steps between rebuilds of the Morton visiting order (0 = storage order)
@@ -32,20 +35,20 @@ def _key(solver, n: int) -> str:
     return f"{name}|n={n}|k={c.n_grids}|g={c.grid_size}|prec={c.precision}|compat={c.compat}"
 
 
-def time_steps(solver, pos, vel, steps: int = 6, reps: int = 2) -> float:
-    """Milliseconds per step of ``steps`` eager steps on copies of pos/vel
-    (best of ``reps``)."""
+def time_steps(solver, pos, vel, steps: int = 8, reps: int = 3) -> float:
+    """Milliseconds per step over ``steps`` graph-replayed steps on copies of
+    pos/vel (best of ``reps`` runs, after one warm-up step that captures)."""
     import torch
 
+    p = pos.clone()
+    v = vel.clone()
+    solver.run(p, v, 1, use_graph=True)  # warm: order, buffers, graph capture
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
     best = float("inf")
     for _ in range(reps):
-        p = pos.clone()
-        v = vel.clone()
-        solver.run(p, v, 1, use_graph=False)  # warm: order, buffers
-        start = torch.cuda.Event(enable_timing=True)
-        stop = torch.cuda.Event(enable_timing=True)
         start.record()
-        solver.run(p, v, steps, use_graph=False)
+        solver.run(p, v, steps, use_graph=True)
         stop.record()
         torch.cuda.synchronize()
         best = min(best, start.elapsed_time(stop) / steps)
@@ -70,7 +73,7 @@ def candidate_stages(g: int):
     ]
 
 
-def autotune(solver, pos, vel, steps: int = 6, cache_path: os.PathLike | None = DEFAULT_CACHE,
+def autotune(solver, pos, vel, steps: int = 8, cache_path: os.PathLike | None = DEFAULT_CACHE,
              use_cache: bool = True, verbose: bool = False) -> dict:
     """Pick and apply the fastest tuning for this state's size; returns
     ``{"tuning": ..., "ms_per_step": ..., "evaluated": ...}``."""
@@ -86,14 +89,13 @@ def autotune(solver, pos, vel, steps: int = 6, cache_path: os.PathLike | None = 
             solver.set_tuning(**cache[key]["tuning"])
             return dict(cache[key], cached=True)
     best = solver.get_tuning()
-    best_ms = time_steps(solver, pos, vel, steps)
-    evaluated = 1
+    evaluated = 0
     g = solver.cfg.grid_size
     for stage in candidate_stages(g):
-        stage_best = dict(best)
-        for cand in stage:
+        timed = []  # (ms, tuning), the incumbent included
+        for cand in [{}] + list(stage):
             trial = {**best, **cand}
-            if trial == best:
+            if cand and trial == best:
                 continue
             try:
                 solver.set_tuning(**trial)
@@ -103,9 +105,14 @@ def autotune(solver, pos, vel, steps: int = 6, cache_path: os.PathLike | None = 
             evaluated += 1
             if verbose:
                 print(f"[autotune] {cand} -> {ms:.4f} ms/step", flush=True)
-            if ms < best_ms:
-                best_ms, stage_best = ms, trial
-        best = stage_best
+            timed.append((ms, trial))
+        # confirm the three fastest over longer runs
+        finalists = sorted(timed, key=lambda t: t[0])[:3]
+        confirmed = []
+        for _, trial in finalists:
+            solver.set_tuning(**trial)
+            confirmed.append((time_steps(solver, pos, vel, 4 * steps), trial))
+        best_ms, best = min(confirmed, key=lambda t: t[0])
         solver.set_tuning(**best)
     result = {"tuning": best, "ms_per_step": best_ms, "evaluated": evaluated}
     if cache_path is not None:

@@ -313,6 +313,33 @@ def test_tuning_variants_agree(torch_cuda, g):
     assert O.rel_rms(vv.cpu().numpy()[:, :3], results[0][1][:, :3]) < 1e-5
 
 
+def test_autotune_keeps_the_step(torch_cuda):
+    """The autotuner runs on copies of the state and picks among tunings that
+    give the same step: after tuning, a step is bit-identical to the default
+    tuning's (with the DFT path the tuner chose, the one choice that changes
+    rounding)."""
+    from mog_solver_amd.autotune import autotune
+
+    torch = torch_cuda
+    k, g, W = 8, 16, 100.0
+    pos, vel = workloads.two_disks(40000, W, 100.0, g, seed=3)
+    s, _, _ = make_solver(k, g, W)
+    base = s.get_tuning()
+    p0, v0 = dev(torch, pos), dev(torch, vel)
+    res = autotune(s, p0, v0, steps=2, cache_path=None)
+    assert res["evaluated"] > 10 and not res["cached"] and res["ms_per_step"] > 0
+    tuned = s.get_tuning()
+    assert tuned == res["tuning"]
+    assert np.array_equal(p0.cpu().numpy(), pos) and np.array_equal(v0.cpu().numpy(), vel)
+    out = []
+    for t in (tuned, {**base, "spectral_mfma": tuned["spectral_mfma"]}):
+        s.set_tuning(**t)
+        p, v = dev(torch, pos), dev(torch, vel)
+        s.run(p, v, 3, use_graph=True)
+        out.append((p.cpu().numpy(), v.cpu().numpy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_tuning_rejects_invalid(torch_cuda):
     from mog_solver_amd._abi import MogError
 
