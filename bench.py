@@ -38,11 +38,13 @@ HBM_PEAK_GBS = 8000.0
 # 256 MiB copy reads as FETCH 128 MiB and WRITE 256 MiB, so FETCH_SIZE is
 # scaled by 2.0 and WRITE_SIZE by 1.0.  Figures and commands:
 # profiles/pmc_traffic_two_disks_n1e6_k240_g16.md.  Used only when the run
-# matches (workload, n, k, g, precision, grids per deposit workgroup).
+# matches (workload, n, k, g, precision, grids per deposit workgroup,
+# deposit workgroup target): the deposit's re-reads of the positions depend
+# on its tiling.
 PMC_TRAFFIC_KIB = {
-    ("two_disks", 1_000_000, 240, 16, "fp32", 2): {
-        "deposit": (909_256.0, 61_289.0),
-        "gather_update": (159_296.0, 64_615.0),
+    ("two_disks", 1_000_000, 240, 16, "fp32", 2, 2048): {
+        "deposit": (273_067.0, 33_644.0),
+        "gather_update": (159_296.0, 64_627.0),
     },
 }
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
@@ -225,7 +227,8 @@ def main():
     dom = max(("deposit", "gather_update", "solve"), key=lambda key: phases[key])
     achieved = alg_bytes[dom] / (phases[dom] / 1e3) / 1e9
     tuning = solver.get_tuning()
-    measured = PMC_TRAFFIC_KIB.get((args.workload, n, k, g, args.precision, tuning["deposit_grids_per_wg"]), {})
+    measured = PMC_TRAFFIC_KIB.get((args.workload, n, k, g, args.precision, tuning["deposit_grids_per_wg"],
+                                    tuning["deposit_wg_target"]), {})
     traffic = None
     if dom in measured:
         fetch_kib, write_kib = measured[dom]
