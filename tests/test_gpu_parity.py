@@ -92,7 +92,7 @@ def test_deposit_masses_and_large_values(torch_cuda):
 
 
 @pytest.mark.parametrize("g,compat", [(16, "reference"), (16, "physical"), (8, "reference"), (4, "reference"),
-                                      (32, "reference"), (64, "physical")])
+                                      (32, "reference"), (64, "physical"), (64, "reference")])
 def test_solve_matches_fp64(torch_cuda, g, compat):
     torch = torch_cuda
     k = 3 if g < 64 else 1
@@ -129,7 +129,8 @@ def clustered_grids(k, g, W, seed, n=40000):
 # convolution (main.py:250-278 kernel) against the FP64 oracle, on the VALU
 # (g < 16) and matrix-core (g >= 16) transforms; compat reference: the key
 # runs the FFT solver, bit-identically.
-@pytest.mark.parametrize("g,k,mfma", [(8, 4, False), (16, 3, True), (16, 3, False), (32, 2, True), (64, 1, True)])
+@pytest.mark.parametrize("g,k,mfma", [(8, 4, False), (16, 3, True), (16, 3, False), (32, 2, True), (64, 1, True),
+                                      (32, 2, False), (64, 1, False)])
 def test_convolution_solve_matches_fp64(torch_cuda, g, k, mfma):
     torch = torch_cuda
     W, G = 100.0, 100.0
