@@ -63,17 +63,6 @@ __device__ __forceinline__ int cell_index(float c0, float c1, float c2, int pfas
   return cell;
 }
 
-// cell_index with the fast row's addends given separately (see cell_fast_a).
-__device__ __forceinline__ int cell_index_a(float c0, float c1, float c2, int pfast,
-                                            const float* F, float a0, float a1, float a2,
-                                            const float* __restrict__ R,
-                                            const float* __restrict__ O, const Params& P) {
-  int cell;
-  const int ok = cell_fast_a(c0, c1, c2, F, a0, a1, a2, P.lim, P.lg, &cell) & pfast;
-  if (!ok) cell = cell_exact_call(c0, c1, c2, R, O, P.W, P.half, P.h, P.lg);
-  return cell;
-}
-
 // Cell `cell` (0 <= cell < 2^28) of a grid of float4: a 32-bit byte offset
 // on a wave-uniform base, so the load takes the scalar-base addressing form.
 __device__ __forceinline__ float4 cell_load(const float4* __restrict__ grid, int cell) {
