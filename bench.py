@@ -31,6 +31,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "steps/s & particle-grid ops/s at n=1e6,k=240,g=16 at fixed force error"
 HBM_PEAK_GBS = 8000.0
+# datasheet-derived: 256 CUs x 128 B/clk of LDS (or L1) x 2.4 GHz; dense FP32 vector
+LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12
+FP32_VECTOR_PEAK_TFS = 157.3
 
 # HBM-side bytes per launch, from two rocprofv3 --pmc runs of this script
 # (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the dispatches)
@@ -245,6 +248,31 @@ def main():
         "avg_launch_ms": round(phases[dom], 4),
     }
 
+    # achieved fractions per phase: HBM (algorithmic bytes), and the level
+    # that bounds the scatter/gather (8 B per LDS atomic read-modify-write,
+    # 16 B per cell fetch) and the transforms (flop count of the path used)
+    inter = float(n) * k
+    fft = g >= 32
+    dft_flops = (4.0 * 5.0 * g**3 * math.log2(g**3) * k) if fft else 96.0 * k * g**4
+    phase_fractions = {}
+    for key in ("clear", "deposit", "solve", "gather_update"):
+        sec = phases[key] / 1e3
+        hbm = alg_bytes[key] / sec / 1e9
+        entry = {"ms": round(phases[key], 4), "hbm_GBps": round(hbm, 1), "hbm_frac": round(hbm / HBM_PEAK_GBS, 5)}
+        if key == "deposit":
+            tbs = 8.0 * inter / sec / 1e12
+            entry.update(lds_atomic_TBps=round(tbs, 3), lds_frac=round(tbs / LDS_PEAK_TBS, 4))
+        elif key == "gather_update":
+            tbs = 16.0 * inter / sec / 1e12
+            entry.update(cell_fetch_TBps=round(tbs, 3), cache_frac=round(tbs / LDS_PEAK_TBS, 4))
+        elif key == "solve":
+            tfs = dft_flops / sec / 1e12
+            entry.update(dft="fft" if fft else "dense", dft_flops_per_launch=dft_flops,
+                         dft_TFLOPs=round(tfs, 3), fp32_frac=round(tfs / FP32_VECTOR_PEAK_TFS, 4))
+        phase_fractions[key] = entry
+    phase_fractions["peaks"] = {"hbm_GBps": HBM_PEAK_GBS, "lds_TBps": round(LDS_PEAK_TBS, 2),
+                                "fp32_vector_TFLOPs": FP32_VECTOR_PEAK_TFS}
+
     cpu = None
     force_err = force_err_sample = None
     if not args.no_cpu_baseline:
@@ -308,6 +336,7 @@ def main():
         "autotuned": tuned is not None,
         "phases_ms": {key: round(v, 4) for key, v in phases.items()},
         "roofline": roofline,
+        "phase_fractions": phase_fractions,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
