@@ -38,9 +38,6 @@ static __device__ __attribute__((noinline)) int cell_exact_call(float c0, float 
   return cell_exact(c0, c1, c2, R, O, W, half, h, lg);
 }
 
-// Cell of the centred position (c0,c1,c2) in one grid: the fast path when
-// the particle is admitted (pfast) and v is clear of every cell face,
-// otherwise the exact float32 sequence of the reference (cellmath.h).
 // A wave-uniform value copied into a vector register.  A VOP3 instruction
 // reads at most one scalar operand on this target, so an FMA whose
 // multiplier and addend are both wave-uniform would otherwise re-copy the
@@ -52,6 +49,9 @@ __device__ __forceinline__ float to_vgpr(float x) {
   return y;
 }
 
+// Cell of the centred position (c0,c1,c2) in one grid: the fast path when
+// the particle is admitted (pfast) and v is clear of every cell face,
+// otherwise the exact float32 sequence of the reference (cellmath.h).
 // F points at the grid's fast row (registers or scalar memory), R and O at
 // its packed rotation row and offset in global memory.
 __device__ __forceinline__ int cell_index(float c0, float c1, float c2, int pfast, const float* F,
