@@ -6,7 +6,7 @@
 // the three inverse transforms (main.py:638-646) and pack.glsl.
 //
 // Each length-g transform is a dense DFT contraction with a twiddle table
-// (FP64-derived, stored FP32) for g <= 16, and for g >= 32 a radix-2
+// (FP64-derived, stored FP32) for g <= 16, and for g >= 32 a radix-4/2
 // Stockham FFT over all lines of the slab at once, in LDS (g log2 g instead
 // of g^2 complex multiply-adds per line).  The 3D transform of a grid is
 // split into three passes over an L2-resident complex scratch buffer:
@@ -37,35 +37,74 @@ __device__ __forceinline__ float2 cmul_tw(float2 a, float2 w) {
 // Grid sizes from which the slab passes use lds_fft instead of dense DFTs.
 constexpr int kFftMinG = 32;
 
-// Radix-2 Stockham FFT (natural order in and out) of `lines` independent
-// length-g lines held in LDS, element (line l, position p) at p*ps + l*ls.
-// Each stage reads one buffer and writes the other:
-//   v0 = a[j], v1 = a[j + g/2] * e^{-+2 pi i k / (2 Ns)},  k = j mod Ns,
-//   b[(j / Ns) 2 Ns + k] = v0 + v1,  b[(j / Ns) 2 Ns + k + Ns] = v0 - v1,
-// for Ns = 1, 2, .., g/2 (forward: e^-, inverse: e^+, unnormalised).  twl
-// holds (cos, sin)(2 pi t / g).  Returns the buffer that holds the result.
-// Called by every thread of the workgroup; ends with a barrier.  When the
-// lines are the unit-stride direction (ls == 1) consecutive threads take
-// consecutive lines, otherwise consecutive positions, so that a wave's LDS
-// accesses fall in distinct banks.
+// Mixed radix-4/2 Stockham FFT (natural order in and out) of `lines`
+// independent length-g lines held in LDS, element (line l, position p) at
+// p*ps + l*ls.  Stages of radix 4 (one of radix 2 when log2 g is odd); with
+// Ns the product of the radices of the earlier stages, q = g / R, a stage
+// reads one buffer and writes the other:
+//   v_r = a[j + r q] * w^r,  w = e^{-+2 pi i (j mod Ns) / (R Ns)},  r < R,
+//   (V_0 .. V_{R-1}) = the R-point DFT of v,
+//   b[(j / Ns) R Ns + (j mod Ns) + r Ns] = V_r,
+// for j < q (forward: e^-, inverse: e^+, unnormalised).  twl holds (cos,
+// sin)(2 pi t / g).  Returns the buffer that holds the result.  Called by
+// every thread of the workgroup; ends with a barrier.  When the lines are
+// the unit-stride direction (ls == 1) consecutive threads take consecutive
+// lines, otherwise consecutive positions, so that a wave's LDS accesses
+// fall in distinct banks.
 __device__ __forceinline__ float2* lds_fft(float2* a, float2* b, int g, int lg, int ps, int ls,
                                            int lines, const float2* twl, bool inverse) {
-  const int half = g >> 1;
-  const int nb = lines * half;
   const bool line_fast = ls == 1;
-  for (int st = 0; st < lg; ++st) {
-    const int Ns = 1 << st;
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-      const int l = line_fast ? t & (lines - 1) : t >> (lg - 1);
-      const int j = line_fast ? t / lines : t & (half - 1);
-      const float2 v0 = a[j * ps + l * ls];
-      const float2 u = a[(j + half) * ps + l * ls];
-      const int k = j & (Ns - 1);
-      const float2 w = twl[k << (lg - 1 - st)];
-      const float2 v1 = inverse ? cmul_tw(u, w) : cmul_conj_tw(u, w);
-      const int d = ((j >> st) << (st + 1)) + k;
-      b[d * ps + l * ls] = make_float2(v0.x + v1.x, v0.y + v1.y);
-      b[(d + Ns) * ps + l * ls] = make_float2(v0.x - v1.x, v0.y - v1.y);
+  int Ns = 1, rem = lg, lNs = 0;
+  while (rem > 0) {
+    if (rem >= 2) {  // radix 4
+      const int q = g >> 2, lq = lg - 2;
+      const int nb = lines * q;
+      for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+        const int l = line_fast ? t & (lines - 1) : t >> lq;
+        const int j = line_fast ? t / lines : t & (q - 1);
+        float2 v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = a[(j + r * q) * ps + l * ls];
+        const int k = j & (Ns - 1);
+        const int ti = k << (lg - 2 - lNs);  // k * g / (4 Ns)
+#pragma unroll
+        for (int r = 1; r < 4; ++r) {
+          const float2 w = twl[(r * ti) & (g - 1)];
+          v[r] = inverse ? cmul_tw(v[r], w) : cmul_conj_tw(v[r], w);
+        }
+        // 4-point DFT; s = -i (forward) or +i (inverse) applied to v1 - v3
+        const float2 s02 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+        const float2 d02 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+        const float2 s13 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+        const float2 d13 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+        const float2 rd13 = inverse ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+        const int d = ((j >> lNs) << (lNs + 2)) + k;
+        b[d * ps + l * ls] = make_float2(s02.x + s13.x, s02.y + s13.y);
+        b[(d + Ns) * ps + l * ls] = make_float2(d02.x + rd13.x, d02.y + rd13.y);
+        b[(d + 2 * Ns) * ps + l * ls] = make_float2(s02.x - s13.x, s02.y - s13.y);
+        b[(d + 3 * Ns) * ps + l * ls] = make_float2(d02.x - rd13.x, d02.y - rd13.y);
+      }
+      Ns <<= 2;
+      lNs += 2;
+      rem -= 2;
+    } else {  // radix 2
+      const int half = g >> 1;
+      const int nb = lines * half;
+      for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+        const int l = line_fast ? t & (lines - 1) : t >> (lg - 1);
+        const int j = line_fast ? t / lines : t & (half - 1);
+        const float2 v0 = a[j * ps + l * ls];
+        const float2 u = a[(j + half) * ps + l * ls];
+        const int k = j & (Ns - 1);
+        const float2 w = twl[k << (lg - 1 - lNs)];
+        const float2 v1 = inverse ? cmul_tw(u, w) : cmul_conj_tw(u, w);
+        const int d = ((j >> lNs) << (lNs + 1)) + k;
+        b[d * ps + l * ls] = make_float2(v0.x + v1.x, v0.y + v1.y);
+        b[(d + Ns) * ps + l * ls] = make_float2(v0.x - v1.x, v0.y - v1.y);
+      }
+      Ns <<= 1;
+      lNs += 1;
+      rem -= 1;
     }
     __syncthreads();
     float2* c = a;
