@@ -6,7 +6,7 @@
 // the three inverse transforms (main.py:638-646) and pack.glsl.
 //
 // Each length-g transform is a dense DFT contraction with a twiddle table
-// (FP64-derived, stored FP32) for g <= 16, and for g >= 32 a radix-4/2
+// (FP64-derived, stored FP32) for g <= 8, and for g >= 16 a radix-4/2
 // Stockham FFT over all lines of the slab at once, in LDS (g log2 g instead
 // of g^2 complex multiply-adds per line).  The 3D transform of a grid is
 // split into three passes over an L2-resident complex scratch buffer:
@@ -35,7 +35,7 @@ __device__ __forceinline__ float2 cmul_tw(float2 a, float2 w) {
 }
 
 // Grid sizes from which the slab passes use lds_fft instead of dense DFTs.
-constexpr int kFftMinG = 32;
+constexpr int kFftMinG = 16;
 
 // Mixed radix-4/2 Stockham FFT (natural order in and out) of `lines`
 // independent length-g lines held in LDS, element (line l, position p) at
