@@ -1,7 +1,7 @@
 """Force-MSE harness of the 2D MOG path (grid size G vs ensemble size N).
 
 ``python -m mog_solver_amd.mse2d [--n-side 512] [--sizes 8 16 32 64]
-[--ensembles 1 32 128] [--trials 3]``
+[--ensembles 1 32 128] [--trials 3] [--time-steps 50]``
 
 The reference's paper reports the 2D force MSE of MOG against an analytic
 solution for G in {8, 16, 32, 64} and N in {1, 32, 128} (paper.tex:43-54,
@@ -19,7 +19,9 @@ of the reference's sources, so this harness is designed from that text:
   neutralising background of the periodic solve;
 * the MOG force at every particle from N randomly rotated and offset
   G x G grids (``Mog2D``), trials with independent transforms;
-* MSE = mean over particles |F_MOG - F|^2, and relative to mean |F|^2.
+* MSE = mean over particles |F_MOG - F|^2, and relative to mean |F|^2;
+* with ``--time-steps`` (default 50), the 2D step's throughput on the same
+  particles with the largest ensemble, per grid size.
 
 The MOG coupling scales with h^2, so only the relative MSE compares across
 G.  The absolute values depend on units and on the (unpublished) setup, so
@@ -105,6 +107,37 @@ def mse_table(n_side=512, sizes=(8, 16, 32, 64), ensembles=(1, 32, 128), trials=
     return {"n": int(pos_h.shape[0]), "W": W, "G_const": G, "sigma": sigma, "radius": radius, "rows": rows}
 
 
+def step_throughput(pos_h, sizes=(8, 16, 32, 64), n_grids=128, steps=50, W=100.0, G=100.0, dt=1e-4,
+                    damping=0.999):
+    """Milliseconds per 2D step (deposit, solve, gather + update of
+    ``mog2d_step``; launches, no graph) and particle-grid interactions per
+    second for the harness's particles, per grid size, with ``n_grids``
+    grids; the state is a copy, advanced ``3 + steps`` steps."""
+    import torch
+
+    from .mog2d import Mog2D
+
+    rows = []
+    for g in sizes:
+        m = Mog2D(n_grids, g, W, G, seed=7)
+        p = torch.from_numpy(pos_h).cuda()
+        v = torch.zeros_like(p)
+        for _ in range(3):
+            m.step(p, v, dt, damping)
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(steps):
+            m.step(p, v, dt, damping)
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / steps
+        m.close()
+        rows.append({"G": g, "N": n_grids, "ms_per_step": round(ms, 4),
+                     "interactions_per_s": round(pos_h.shape[0] * n_grids / (ms / 1e3), 1)})
+    return rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--n-side", type=int, default=512, help="lattice side of the stratified sample")
@@ -112,6 +145,8 @@ def main(argv=None):
     ap.add_argument("--ensembles", type=int, nargs="+", default=[1, 32, 128])
     ap.add_argument("--trials", type=int, default=3)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--time-steps", type=int, default=50,
+                    help="timed 2D steps per grid size with the largest ensemble (0: skip)")
     args = ap.parse_args(argv)
     import torch
 
@@ -120,6 +155,12 @@ def main(argv=None):
     out = mse_table(args.n_side, tuple(args.sizes), tuple(args.ensembles), args.trials, seed=args.seed)
     for r in out["rows"]:
         print(f"G={r['G']:3d} N={r['N']:4d}  MSE {r['mse']:.4e}  rel {r['rel_mse']:.4e}", file=sys.stderr)
+    if args.time_steps > 0:
+        pos_h = sample_blob(args.n_side, seed=args.seed)
+        out["throughput"] = step_throughput(pos_h, tuple(args.sizes), max(args.ensembles), args.time_steps)
+        for r in out["throughput"]:
+            print(f"G={r['G']:3d} N={r['N']:4d}  {r['ms_per_step']:.4f} ms/step  "
+                  f"{r['interactions_per_s']:.3e} interactions/s", file=sys.stderr)
     print(json.dumps(out))
 
 
