@@ -255,6 +255,12 @@ void mog2d_destroy(mog2d_ctx* ctx);
 /* Average force at the particles (async on `stream`). */
 int mog2d_forces(mog2d_ctx* ctx, const float* pos_n4, int64_t n, float* avg_n4, void* stream);
 
+/* Clear + deposit only: the k g^2 int64 milli-mass grids ([grid][y][x]) into
+ * grids_out (device).  No counterpart in the reference (whose 2D branch has
+ * no deposit); the deposit half of mog2d_forces, for tests and callers that
+ * want the density. */
+int mog2d_deposit(mog2d_ctx* ctx, const float* pos_n4, int64_t n, int64_t* grids_out, void* stream);
+
 /* One step: force, then vel -= F dt; vel *= damping; pos += vel dt; pos mod W
  * (update.glsl:83-86), z untouched when vel.z is 0. */
 int mog2d_step(mog2d_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, double dt, double damping,

@@ -59,6 +59,7 @@ EXPORTED = (
     "mog2d_create",
     "mog2d_destroy",
     "mog2d_forces",
+    "mog2d_deposit",
     "mog2d_step",
     "mog_last_error",
 )
@@ -200,6 +201,8 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     lib.mog2d_destroy.argtypes = [vp]
     lib.mog2d_forces.restype = ctypes.c_int
     lib.mog2d_forces.argtypes = [vp, vp, i64, vp, vp]
+    lib.mog2d_deposit.restype = ctypes.c_int
+    lib.mog2d_deposit.argtypes = [vp, vp, i64, vp, vp]
     lib.mog2d_step.restype = ctypes.c_int
     lib.mog2d_step.argtypes = [vp, vp, vp, i64, d, d, vp]
     lib.mog_last_error.restype = ctypes.c_char_p

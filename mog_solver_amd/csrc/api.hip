@@ -523,8 +523,10 @@ struct mog_batch {
 struct mog2d_ctx {
   int k, g, lg;
   double W, G;
+  float lim;  // fast-path acceptance 0.5 - margin (< 0: exact sequence only)
   float* d_rot;
   float* d_off;
+  float* d_frot;
   float2* d_tw;
   float* d_gtab;
   float* d_kd;
@@ -537,7 +539,8 @@ struct mog2d_ctx {
 namespace {
 
 void mog2d_release(mog2d_ctx* c) {
-  void* ptrs[] = {c->d_rot, c->d_off, c->d_tw, c->d_gtab, c->d_kd, c->d_grids, c->d_force, c->d_avg};
+  void* ptrs[] = {c->d_rot, c->d_off, c->d_frot, c->d_tw, c->d_gtab, c->d_kd, c->d_grids, c->d_force,
+                  c->d_avg};
   for (void* q : ptrs) (void)hipFree(q);
 }
 
@@ -1415,6 +1418,7 @@ int mog2d_create(int32_t n_grids, int32_t grid_size, double world_size, double G
   };
   A(dalloc(&c->d_rot, (size_t)n_grids * 12));
   A(dalloc(&c->d_off, (size_t)n_grids * 4));
+  A(dalloc(&c->d_frot, (size_t)n_grids * 12));
   A(dalloc(&c->d_tw, g));
   A(dalloc(&c->d_gtab, g * g));
   A(dalloc(&c->d_kd, g));
@@ -1442,8 +1446,21 @@ int mog2d_create(int32_t n_grids, int32_t grid_size, double world_size, double G
       const double k2 = kx * kx + ky * ky;
       gtab[iy * g + ix] = (ix | iy) ? (float)(-2.0 * M_PI * G / k2 / (1000.0 * (double)(g * g))) : 0.0f;
     }
+  // fast rows and margin of the 3D path: the planar rows have zero z
+  // coefficients, and the 3D bound covers the shorter 2D sequence
+  const float halff = (float)world_size * 0.5f, hf = (float)(world_size / (double)grid_size);
+  std::vector<float> frot((size_t)n_grids * 12);
+  double rmax = 0.0, omax = 0.0;
+  for (int i = 0; i < n_grids; ++i) {
+    make_fast_row(rot_k12 + 12 * i, off_k4 + 4 * i, halff, hf, frot.data() + 12 * i);
+    for (int j = 0; j < 12; ++j) rmax = fmax(rmax, fabs((double)rot_k12[12 * i + j]));
+    for (int m = 0; m < 2; ++m) omax = fmax(omax, fabs((double)off_k4[4 * i + m]));
+  }
+  const double margin = fast_margin(world_size, grid_size, rmax, omax);
+  c->lim = margin < 0.05 ? (float)(0.5 - margin) : -1.0f;
   A(hipMemcpy(c->d_rot, rot_k12, (size_t)n_grids * 12 * sizeof(float), hipMemcpyHostToDevice));
   A(hipMemcpy(c->d_off, off_k4, (size_t)n_grids * 4 * sizeof(float), hipMemcpyHostToDevice));
+  A(hipMemcpy(c->d_frot, frot.data(), frot.size() * sizeof(float), hipMemcpyHostToDevice));
   A(hipMemcpy(c->d_tw, tw.data(), g * sizeof(float2), hipMemcpyHostToDevice));
   A(hipMemcpy(c->d_gtab, gtab.data(), g * g * sizeof(float), hipMemcpyHostToDevice));
   A(hipMemcpy(c->d_kd, kd.data(), g * sizeof(float), hipMemcpyHostToDevice));
@@ -1468,9 +1485,22 @@ int mog2d_forces(mog2d_ctx* ctx, const float* pos_n4, int64_t n, float* avg_n4, 
   if (n > 0 && (!pos_n4 || !avg_n4 || !aligned16(pos_n4) || !aligned16(avg_n4)))
     return fail(nullptr, MOG_EINVAL, "pos/avg must be 16-byte aligned device pointers");
   hipError_t e = launch_mog2d_forces((const float4*)pos_n4, n, ctx->k, ctx->lg, (float)ctx->W,
-                                     ctx->d_rot, ctx->d_off, ctx->d_tw, ctx->d_gtab, ctx->d_kd,
-                                     ctx->d_grids, ctx->d_force, (float4*)avg_n4, (hipStream_t)stream);
+                                     ctx->d_rot, ctx->d_off, ctx->d_frot, ctx->lim, ctx->d_tw,
+                                     ctx->d_gtab, ctx->d_kd, ctx->d_grids, ctx->d_force,
+                                     (float4*)avg_n4, (hipStream_t)stream);
   return e == hipSuccess ? MOG_OK : fail(nullptr, MOG_EHIP, "mog2d_forces: %s", hipGetErrorString(e));
+}
+
+int mog2d_deposit(mog2d_ctx* ctx, const float* pos_n4, int64_t n, int64_t* grids_out, void* stream) {
+  if (!ctx) return fail(nullptr, MOG_EINVAL, "ctx is NULL");
+  if (n < 0 || n > (int64_t)2147483647) return fail(nullptr, MOG_EINVAL, "n must be in [0, 2^31)");
+  if (!grids_out) return fail(nullptr, MOG_EINVAL, "grids_out is NULL");
+  if (n > 0 && (!pos_n4 || !aligned16(pos_n4)))
+    return fail(nullptr, MOG_EINVAL, "pos must be a 16-byte aligned device pointer");
+  hipError_t e = launch_mog2d_deposit((const float4*)pos_n4, n, ctx->k, ctx->lg, (float)ctx->W,
+                                      ctx->d_rot, ctx->d_off, ctx->d_frot, ctx->lim,
+                                      (long long*)grids_out, (hipStream_t)stream);
+  return e == hipSuccess ? MOG_OK : fail(nullptr, MOG_EHIP, "mog2d_deposit: %s", hipGetErrorString(e));
 }
 
 int mog2d_step(mog2d_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, double dt, double damping,
@@ -1491,8 +1521,9 @@ int mog2d_step(mog2d_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, double d
     ctx->avg_cap = n;
   }
   hipError_t e = launch_mog2d_forces((const float4*)pos_n4, n, ctx->k, ctx->lg, (float)ctx->W,
-                                     ctx->d_rot, ctx->d_off, ctx->d_tw, ctx->d_gtab, ctx->d_kd,
-                                     ctx->d_grids, ctx->d_force, ctx->d_avg, s);
+                                     ctx->d_rot, ctx->d_off, ctx->d_frot, ctx->lim, ctx->d_tw,
+                                     ctx->d_gtab, ctx->d_kd, ctx->d_grids, ctx->d_force, ctx->d_avg,
+                                     s);
   if (e == hipSuccess) {
     Params P{};
     P.W = (float)ctx->W;

@@ -4,6 +4,8 @@
 //
 //   deposit  NGP int64 milli-mass, one workgroup per (particle chunk, grid)
 //            with the g^2 histogram in LDS (u32 partials, 64-bit flush);
+//            cells from the fast path of the 3D kernels (two FMAs per
+//            axis, face test, exact fallback);
 //   solve    one workgroup per grid, the whole grid in LDS: real x-DFT,
 //            y-DFT, Green's function -2 pi G / k^2 x i k (Nyquist-positive
 //            wave numbers), one inverse transform of Fx + i Fy (the
@@ -33,11 +35,32 @@ __device__ __forceinline__ int cell2d(float c0, float c1, const float* __restric
   return (axis_cell_exact(l1, W, h, gm) << lg) | axis_cell_exact(l0, W, h, gm);
 }
 
+// Fast 2D cell from the first two rows of the grid's fast row (the 3D row of
+// make_fast_row; its z coefficients are zero in the planar layout), with
+// the 3D path's face test: w_m = l_m / h - 1/2 as two FMAs, cell floor(v) =
+// rint(w).  Two products and one add round less than the 3D sequence's
+// three and two, so the 3D margin bounds this path too.  Falls back to the
+// exact sequence for undecided pairs and particles outside |c| <= CB.
+__device__ __forceinline__ int cell2d_index(float c0, float c1, const float* __restrict__ F,
+                                            const float* __restrict__ R,
+                                            const float* __restrict__ O, float W, float half,
+                                            float h, int lg, float lim, float CB) {
+  const float w0 = fmaf(F[1], c1, fmaf(F[0], c0, F[3]));
+  const float w1 = fmaf(F[5], c1, fmaf(F[4], c0, F[7]));
+  const float r0 = rintf(w0), r1 = rintf(w1);
+  const bool ok = (fmaxf(fabsf(w0 - r0), fabsf(w1 - r1)) < lim) & (fabsf(c0) <= CB) &
+                  (fabsf(c1) <= CB);
+  const int gm = (1 << lg) - 1;
+  if (ok) return (((int)r1 & gm) << lg) | ((int)r0 & gm);
+  return cell2d(c0, c1, R, O, W, half, h, lg);
+}
+
 __global__ __launch_bounds__(256) void deposit2d_kernel(const float4* __restrict__ pos,
                                                         long long n, long long chunk,
                                                         const float* __restrict__ rot,
-                                                        const float* __restrict__ off, float W,
-                                                        float h, int lg,
+                                                        const float* __restrict__ off,
+                                                        const float* __restrict__ frot, float W,
+                                                        float h, int lg, float lim,
                                                         unsigned long long* __restrict__ grids) {
   extern __shared__ unsigned int hist[];
   const int cells = 1 << (2 * lg);
@@ -55,7 +78,8 @@ __global__ __launch_bounds__(256) void deposit2d_kernel(const float4* __restrict
     const float4 q = pos[p];
     const unsigned long long milli = milli_of(q.w);
     if (milli == 0ull) continue;
-    const int cell = cell2d(q.x - half, q.y - half, R, O, W, half, h, lg);
+    const int cell =
+        cell2d_index(q.x - half, q.y - half, frot + 12 * grid, R, O, W, half, h, lg, lim, 1.5f * W);
     if (milli <= lds_limit)
       atomicAdd(&hist[cell], (unsigned int)milli);
     else
@@ -147,8 +171,9 @@ __global__ __launch_bounds__(256) void solve2d_kernel(const long long* __restric
 __global__ __launch_bounds__(256) void gather2d_kernel(const float4* __restrict__ pos,
                                                        long long n, int k,
                                                        const float* __restrict__ rot,
-                                                       const float* __restrict__ off, float W,
-                                                       float h, int lg,
+                                                       const float* __restrict__ off,
+                                                       const float* __restrict__ frot, float W,
+                                                       float h, int lg, float lim,
                                                        const float2* __restrict__ force,
                                                        float4* __restrict__ avg) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -160,7 +185,8 @@ __global__ __launch_bounds__(256) void gather2d_kernel(const float4* __restrict_
   float tx = 0.f, ty = 0.f;
   for (int gr = 0; gr < k; ++gr) {
     const float* R = rot + 12 * gr;
-    const int cell = cell2d(c0, c1, R, off + 4 * gr, W, half, h, lg);
+    const int cell = cell2d_index(c0, c1, frot + 12 * gr, R, off + 4 * gr, W, half, h, lg, lim,
+                                  1.5f * W);
     const float2 F = force[gr * cells + cell];
     // transpose(rot) * F: world_m = sum_j rot[4m + j] F_j
     tx += fmaf(R[0], F.x, R[1] * F.y);
@@ -170,22 +196,29 @@ __global__ __launch_bounds__(256) void gather2d_kernel(const float4* __restrict_
   avg[i] = make_float4(tx / fk, ty / fk, 0.f, 0.f);
 }
 
-hipError_t launch_mog2d_forces(const float4* pos, long long n, int k, int lg, float W,
-                               const float* rot, const float* off, const float2* tw,
-                               const float* gtab, const float* kd, long long* grids,
-                               float2* force, float4* avg, hipStream_t s) {
+hipError_t launch_mog2d_deposit(const float4* pos, long long n, int k, int lg, float W,
+                                const float* rot, const float* off, const float* frot, float lim,
+                                long long* grids, hipStream_t s) {
   const int g = 1 << lg, cells = g * g;
   const float h = W / (float)g;
   hipError_t e = launch_fill_zero(grids, (long long)k * cells, s);
+  if (e != hipSuccess || n <= 0) return e;
+  const long long chunk = 16384;
+  const long long chunks = (n + chunk - 1) / chunk;
+  hipLaunchKernelGGL(deposit2d_kernel, dim3((unsigned)chunks, (unsigned)k), dim3(256),
+                     (size_t)cells * sizeof(unsigned int), s, pos, n, chunk, rot, off, frot, W, h,
+                     lg, lim, (unsigned long long*)grids);
+  return hipGetLastError();
+}
+
+hipError_t launch_mog2d_forces(const float4* pos, long long n, int k, int lg, float W,
+                               const float* rot, const float* off, const float* frot, float lim,
+                               const float2* tw, const float* gtab, const float* kd,
+                               long long* grids, float2* force, float4* avg, hipStream_t s) {
+  const int g = 1 << lg, cells = g * g;
+  const float h = W / (float)g;
+  hipError_t e = launch_mog2d_deposit(pos, n, k, lg, W, rot, off, frot, lim, grids, s);
   if (e != hipSuccess) return e;
-  if (n > 0) {
-    const long long chunk = 16384;
-    const long long chunks = (n + chunk - 1) / chunk;
-    hipLaunchKernelGGL(deposit2d_kernel, dim3((unsigned)chunks, (unsigned)k), dim3(256),
-                       (size_t)cells * sizeof(unsigned int), s, pos, n, chunk, rot, off, W, h, lg,
-                       (unsigned long long*)grids);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
   const size_t lds = (size_t)(g + cells / 2 + 1 + 2 * cells) * sizeof(float2);
   if ((e = hipFuncSetAttribute((const void*)solve2d_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)) != hipSuccess)
@@ -195,7 +228,7 @@ hipError_t launch_mog2d_forces(const float4* pos, long long n, int k, int lg, fl
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (n > 0) {
     hipLaunchKernelGGL(gather2d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos, n,
-                       k, rot, off, W, h, lg, force, avg);
+                       k, rot, off, frot, W, h, lg, lim, force, avg);
     e = hipGetLastError();
   }
   return e;

@@ -240,10 +240,16 @@ hipError_t build_batch_order(const float4* pos, long long n, long long blocks,
 // 2D MOG: clear, deposit, per-grid 2D spectral solve and gather of the
 // average world force (avg = (Fx, Fy, 0, 0)).  grids: k g^2 int64 scratch,
 // force: k g^2 float2 scratch; tw / gtab / kd: the 2D tables.
+// frot: the grids' fast rows (make_fast_row), lim: 0.5 - margin (< 0
+// disables the fast path).
 hipError_t launch_mog2d_forces(const float4* pos, long long n, int k, int lg, float W,
-                               const float* rot, const float* off, const float2* tw,
-                               const float* gtab, const float* kd, long long* grids,
-                               float2* force, float4* avg, hipStream_t s);
+                               const float* rot, const float* off, const float* frot, float lim,
+                               const float2* tw, const float* gtab, const float* kd,
+                               long long* grids, float2* force, float4* avg, hipStream_t s);
+// clear + deposit only: the k g^2 int64 density grids of the 2D path.
+hipError_t launch_mog2d_deposit(const float4* pos, long long n, int k, int lg, float W,
+                                const float* rot, const float* off, const float* frot, float lim,
+                                long long* grids, hipStream_t s);
 
 hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const DepositTask* tasks,
                                 int ntasks, const float* rot, const float* off, const float* frot,

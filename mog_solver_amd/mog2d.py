@@ -83,6 +83,18 @@ class Mog2D:
                                                          _stream_ptr(stream)))
         return out
 
+    def deposit(self, pos, out=None, stream=None):
+        """Density grids (k, g, g) int64 milli-mass ([grid][y][x]) of ``pos``."""
+        import torch
+
+        _check_particles(pos, "pos")
+        if out is None:
+            out = torch.empty((self.n_grids, self.grid_size, self.grid_size), dtype=torch.int64,
+                              device=pos.device)
+        _abi.check(self.lib, None, self.lib.mog2d_deposit(self._ctx, _ptr(pos), pos.shape[0], _ptr(out),
+                                                          _stream_ptr(stream)))
+        return out
+
     def step(self, pos, vel, dt, damping=1.0, stream=None):
         _check_particles(pos, "pos")
         _check_particles(vel, "vel")

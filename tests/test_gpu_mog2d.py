@@ -30,6 +30,38 @@ def positions(n, g, seed):
     return pos
 
 
+@pytest.mark.parametrize("k,g", [(6, 16), (3, 4), (2, 64), (7, 32), (1, 2), (4, 8)])
+def test_deposit_bit_exact(torch_cuda, k, g):
+    """Density grids equal the oracle's bit for bit, including particles on
+    and next to cell faces and outside the box (the fast cell path with the
+    exact float32 sequence as fallback)."""
+    from mog_solver_amd.mog2d import Mog2D, transforms_2d
+
+    torch = torch_cuda
+    pos = positions(30000, g, seed=k * 7 + g)
+    # values one float32 step either side of every face
+    h = np.float32(W / g)
+    faces = np.arange(g + 1, dtype=np.float32) * h
+    near = np.concatenate([np.nextafter(faces, np.float32(-1)), np.nextafter(faces, np.float32(W + 1))])
+    grid = np.stack(np.meshgrid(near, near[::3]), -1).reshape(-1, 2).astype(np.float32)
+    extra = np.zeros((grid.shape[0], 4), np.float32)
+    extra[:, :2] = grid
+    extra[:, 3] = 1.0
+    pos = np.concatenate([pos, extra])
+    rot, off = transforms_2d(k, g, W, seed=g + k)
+    m = Mog2D(k, g, W, G, rot=rot, off=off)
+    got = m.deposit(torch.from_numpy(pos).cuda()).cpu().numpy()
+    want = O2.deposit_2d(pos, rot, off, g, W)
+    assert np.array_equal(got, want)
+    # identity transforms put grid faces on the lattice: every face value is
+    # decided by the exact sequence
+    rot_i = np.zeros((1, 12), np.float32)
+    rot_i[0, [0, 5, 10]] = 1.0
+    m1 = Mog2D(1, g, W, G, rot=rot_i, off=np.zeros((1, 4), np.float32))
+    got1 = m1.deposit(torch.from_numpy(pos).cuda()).cpu().numpy()
+    assert np.array_equal(got1, O2.deposit_2d(pos, rot_i, np.zeros((1, 4), np.float32), g, W))
+
+
 @pytest.mark.parametrize("k,g", [(5, 16), (3, 4), (2, 64), (7, 32), (1, 2)])
 def test_forces_match_oracle(torch_cuda, k, g):
     from mog_solver_amd.mog2d import Mog2D, transforms_2d
