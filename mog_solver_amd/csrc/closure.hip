@@ -111,10 +111,10 @@ __global__ __launch_bounds__(256) void closure_features_kernel(
     const float4* __restrict__ force, const float4* __restrict__ force_dc,
     const long long* __restrict__ grids, Params P, float rho0, const int* __restrict__ order,
     float4* __restrict__ feat, float4* __restrict__ mean_out) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  // visiting order (spatially sorted when given): neighbouring lanes read
-  // neighbouring cells
+  // visiting order (spatially sorted when given): neighbouring lanes, and
+  // neighbouring workgroups of one XCD, read neighbouring cells
   const long long i = order ? (long long)order[t] : t;
   const float4 p = pos[i];
   const float c0 = p.x - P.half, c1 = p.y - P.half, c2 = p.z - P.half;

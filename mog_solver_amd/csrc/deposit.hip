@@ -65,7 +65,11 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
   __shared__ unsigned char defer_g[kDeferCap];  // grid within the group
   __shared__ unsigned int defer_n;
   const int cells = P.cells;
-  const int grid0 = blockIdx.y * G;
+  // tiles in chunk-major order: the groups of one chunk share an XCD's L2
+  const int groups = (P.k + G - 1) / G;
+  const unsigned int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const long long chunk_id = tile / groups;
+  const int grid0 = (int)(tile % groups) * G;
   const int ng = min(G, P.k - grid0);
   const int total = ng * cells;
   for (int i = threadIdx.x; i < total; i += blockDim.x) hist[i] = 0u;
@@ -82,7 +86,7 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
   }
   __syncthreads();
 
-  const long long p0 = (long long)blockIdx.x * chunk;
+  const long long p0 = chunk_id * chunk;
   const long long p1 = min(n, p0 + chunk);
   const unsigned int lds_limit32 =
       (unsigned int)(lds_limit < 0xFFFFFFFFull ? lds_limit : 0xFFFFFFFFull);
@@ -276,7 +280,10 @@ __global__ __launch_bounds__(1024) void deposit_sorted_kernel(
     unsigned short* __restrict__ cells_out) {
   extern __shared__ unsigned int hist[];
   const int cells = P.cells;
-  const int grid0 = blockIdx.y * G;
+  const int groups = (P.k + G - 1) / G;
+  const unsigned int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const long long chunk_id = tile / groups;
+  const int grid0 = (int)(tile % groups) * G;
   const int ng = min(G, P.k - grid0);
   const int total = ng * cells;
   for (int i = threadIdx.x; i < total; i += blockDim.x) hist[i] = 0u;
@@ -288,7 +295,7 @@ __global__ __launch_bounds__(1024) void deposit_sorted_kernel(
     for (int j = 0; j < 12; ++j) F[gi][j] = frot[12 * gr + j];
   }
   __syncthreads();
-  const long long p0 = (long long)blockIdx.x * chunk;
+  const long long p0 = chunk_id * chunk;
   const long long p1 = min(n, p0 + chunk);
   for (long long base = p0; base < p1; base += blockDim.x) {  // same trip count for all lanes
     const long long p = base + threadIdx.x;
@@ -351,7 +358,7 @@ static hipError_t launch_lds(const Params& P, const float4* pos, long long n, lo
   // the exact dynamic size: the kernels also hold static LDS
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  const dim3 grid((unsigned)chunks, (unsigned)groups);
+  const dim3 grid((unsigned)(chunks * groups));
   if (!sorted)
     hipLaunchKernelGGL(deposit_lds_kernel<G>, grid, dim3(threads), lds, s, pos, n, chunk, rot, off,
                        frot, P, lds_limit, grids);
@@ -392,6 +399,9 @@ hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, co
   const int groups = (P.k + G - 1) / G;
   const int threads = t.threads;
   long long chunks = (t.wg_target + groups - 1) / groups;
+  // one-dimensional launch of chunks x groups workgroups: under 2^32 lanes
+  const long long max_chunks = 0xFFFFFFFFll / ((long long)threads * groups);
+  if (chunks > max_chunks) chunks = max_chunks;
   if (chunks < 1) chunks = 1;
   long long chunk = (n + chunks - 1) / chunks;
   if (chunk < threads) chunk = threads;

@@ -43,7 +43,8 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
     for (int i = threadIdx.x; i < 3 * P.k; i += blockDim.x) adds[i] = frot[12 * (i / 3) + 4 * (i % 3) + 3];
     __syncthreads();
   }
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // neighbouring slots of the visiting order on one XCD: they share grid cells
+  const long long t = (long long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const long long i = order ? (long long)order[t] : t;
   float4 p = pos[i];
