@@ -45,9 +45,13 @@ FP32_VECTOR_PEAK_TFS = 157.3
 # deposit workgroup target): the deposit's re-reads of the positions depend
 # on its tiling.
 PMC_TRAFFIC_KIB = {
+    ("two_disks", 1_000_000, 240, 16, "fp32", 2, 4096): {
+        "deposit": (9_585.8, 61_289.1),
+        "gather_update": (124_124.5, 66_044.3),
+    },
     ("two_disks", 1_000_000, 240, 16, "fp32", 2, 2048): {
-        "deposit": (273_067.0, 33_644.0),
-        "gather_update": (159_296.0, 64_627.0),
+        "deposit": (10_627.5, 33_642.9),
+        "gather_update": (124_121.8, 66_047.9),
     },
 }
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
@@ -122,6 +126,9 @@ def parse():
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tuning instead of autotuning first")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=INT",
+                    help="set one tuning field after autotuning (or instead of it with "
+                         "--no-autotune), e.g. deposit_wg_target=2048; repeatable")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-MiB device buffer three times (the reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -195,6 +202,16 @@ def main():
         print(f"[bench] autotuned ({tuned['evaluated']} candidates): {tuned['tuning']}",
               file=sys.stderr, flush=True)
 
+    if args.tune:
+        fields = {}
+        for item in args.tune:
+            key, sep, val = item.partition("=")
+            if not sep:
+                raise SystemExit(f"--tune expects KEY=INT, got {item!r}")
+            fields[key.strip()] = int(val)
+        solver.set_tuning(**fields)
+        print(f"[bench] tuning set: {solver.get_tuning()}", file=sys.stderr, flush=True)
+
     use_graph = not args.no_graph
     solver.run(pos, vel, args.warmup, use_graph=use_graph)
     torch.cuda.synchronize()
@@ -252,7 +269,9 @@ def main():
     # that bounds the scatter/gather (8 B per LDS atomic read-modify-write,
     # 16 B per cell fetch) and the transforms (flop count of the path used)
     inter = float(n) * k
-    fft = g >= 32
+    # the FP32 vector path runs radix-4/2 FFTs from g = 16 up; the matrix-core
+    # paths and smaller grids run dense DFTs
+    fft = g >= 16 and args.precision == "fp32" and not tuning.get("spectral_mfma", 0)
     dft_flops = (4.0 * 5.0 * g**3 * math.log2(g**3) * k) if fft else 96.0 * k * g**4
     phase_fractions = {}
     for key in ("clear", "deposit", "solve", "gather_update"):

@@ -329,19 +329,18 @@ int check_particles(mog_ctx* ctx, const void* pos, const void* vel, int64_t n, b
 }
 
 // Default deposit tiling (measured on MI355X, two-disk and Plummer
-// workloads): about 32 KB of LDS counters per workgroup of 512 threads and
-// about 2048 workgroups; when one grid alone needs more than 64 KB (g = 32)
-// the CU holds a single workgroup, so it gets 1024 threads and ~1024
-// workgroups.
+// workloads): about 32 KB of LDS counters per workgroup of 512 threads;
+// when one grid alone needs more than 64 KB (g = 32) the CU holds a single
+// workgroup, so it gets 1024 threads. About 4096 workgroups either way.
 DepositTune default_tune(const Params& P) {
   DepositTune t;
   const size_t grid_bytes = (size_t)P.cells * 4;
   const int per = (int)((32 * 1024) / grid_bytes);
   t.grids_per_wg = per < 1 ? 1 : (per > 8 ? 8 : per);
-  // measured optima on MI355X: g=16 (16 KB grids) 512 threads, 2 grids and
-  // 2048 workgroups; g=32 (128 KB) 1024 threads, 1 grid, 4096 workgroups
+  // measured optima on MI355X: g=16 (16 KB grids) 512 threads, 2 grids;
+  // g=32 (128 KB) 1024 threads, 1 grid; 4096 workgroups for both
   t.threads = grid_bytes > 64 * 1024 ? 1024 : 512;
-  t.wg_target = grid_bytes > 64 * 1024 ? 4096 : 2048;
+  t.wg_target = 4096;
   return t;
 }
 

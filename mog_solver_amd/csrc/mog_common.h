@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #include "cellmath.h"
+#include "tiling.h"
 
 namespace mog {
 
@@ -68,16 +69,6 @@ __device__ __forceinline__ int cell_index(float c0, float c1, float c2, int pfas
 __device__ __forceinline__ float4 cell_load(const float4* __restrict__ grid, int cell) {
   return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(grid) +
                                           ((unsigned)cell << 4));
-}
-
-// Workgroup L of a launch runs on XCD L % 8: dispatch deals workgroups
-// round-robin over the eight XCDs, each with its own L2. xcd_tile maps L to
-// a tile number so that every XCD walks one contiguous range of the T tiles:
-// tiles that are neighbours in tile order, and share inputs, then meet in
-// the same L2. A bijection on [0, T) for any T.
-__device__ __forceinline__ unsigned int xcd_tile(unsigned int L, unsigned int T) {
-  const unsigned int x = L & 7u, s = L >> 3, q = T >> 3, r = T & 7u;
-  return x * q + min(x, r) + s;
 }
 
 // Position of the cached cell of (slot t, grid) in the uint16 cell cache:
