@@ -23,66 +23,106 @@ namespace mog {
 // cannot decide, settled by whole waves at the end of the task.
 constexpr int kBatchDeferCap = 512;
 
-// One group of GU grids (gi0 .. gi0+GU-1, all < ng) of one particle: the GU
+// One particle's inputs to a deposit group.
+struct DepositParticle {
+  float c0, c1, c2;    // centred position
+  int pfast;           // fast-path admission
+  bool small, big;     // 32-bit LDS partial / 64-bit global add
+  unsigned int m32;    // milli-mass (small)
+  float mass;
+  unsigned int pofs;   // offset within the task
+};
+
+// A group of UP particles x GU grids (gi0 .. gi0+GU-1, all < ng): the UP x GU
 // fast cells with no branch between them, their atomics, then one
 // wave-uniform test for the rare paths (undecided pairs go to the task's
 // queue; huge masses to the 64-bit global grids).
-template <int GU>
+template <int UP, int GU>
 __device__ __forceinline__ void deposit_group(
-    int gi0, int gbase, int cells, const Params& P, float c0, float c1, float c2,
-    int pfast, bool small, bool big, unsigned int m32, float mass, unsigned int pofs,
+    int gi0, int gbase, int cells, const Params& P, const DepositParticle (&q)[UP],
     const float* __restrict__ frot, const float* __restrict__ rot, const float* __restrict__ off,
     unsigned int* hist, unsigned long long* __restrict__ dst, unsigned int* defer_p,
     unsigned short* defer_g, unsigned int* defer_n) {
-  int cell[GU];
-  bool ok[GU];
-  bool rare = big;
+  int cell[UP][GU];
+  bool ok[UP][GU];
+  bool rare = false;
 #pragma unroll
-  for (int u = 0; u < GU; ++u)
-    ok[u] = cell_fast(c0, c1, c2, frot + 12 * (gbase + gi0 + u), P.lim, P.lg, &cell[u]) & pfast;
+  for (int a = 0; a < UP; ++a) {
+    rare |= q[a].big;
 #pragma unroll
-  for (int u = 0; u < GU; ++u) {
-    // hidden from the optimiser: a known-zero add would become an atomic
-    // load and a wait
-    unsigned int v = small && ok[u] ? m32 : 0u;
-    asm volatile("" : "+v"(v));
-    atomicAdd(&hist[(gi0 + u) * cells + cell[u]], v);
-    rare |= small && !ok[u];
+    for (int u = 0; u < GU; ++u)
+      ok[a][u] = cell_fast(q[a].c0, q[a].c1, q[a].c2, frot + 12 * (gbase + gi0 + u), P.lim, P.lg,
+                           &cell[a][u]) &
+                 q[a].pfast;
   }
+#pragma unroll
+  for (int a = 0; a < UP; ++a)
+#pragma unroll
+    for (int u = 0; u < GU; ++u) {
+      // hidden from the optimiser: a known-zero add would become an atomic
+      // load and a wait
+      unsigned int v = q[a].small && ok[a][u] ? q[a].m32 : 0u;
+      asm volatile("" : "+v"(v));
+      atomicAdd(&hist[(gi0 + u) * cells + cell[a][u]], v);
+      rare |= q[a].small && !ok[a][u];
+    }
   if (!__ballot(rare)) return;  // wave-uniform
 #pragma unroll
-  for (int u = 0; u < GU; ++u) {
-    const int gi = gi0 + u;
-    const int gr = gbase + gi;
-    if (big) {
-      const int c = ok[u] ? cell[u]
-                          : cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half,
-                                            P.h, P.lg);
-      atomicAdd(&dst[(long long)gi * cells + c], milli_of(mass));
-    }
-    const bool defer = small && !ok[u];
-    const unsigned long long need = __ballot(defer);
-    if (need) {  // queue: one reservation per wave
-      const int lane = threadIdx.x & 63;
-      const int leader = __ffsll((long long)need) - 1;
-      unsigned int base = 0u;
-      if (lane == leader) base = atomicAdd(defer_n, (unsigned int)__popcll(need));
-      base = __builtin_amdgcn_readlane(base, leader);
-      if (defer) {
-        const unsigned int slot =
-            base + __builtin_amdgcn_mbcnt_hi((unsigned int)(need >> 32),
-                                             __builtin_amdgcn_mbcnt_lo((unsigned int)need, 0u));
-        if (slot < (unsigned int)kBatchDeferCap) {
-          defer_p[slot] = pofs;
-          defer_g[slot] = (unsigned short)gi;
-        } else {
-          const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half,
-                                         P.h, P.lg);
-          atomicAdd(&hist[gi * cells + ex], m32);
+  for (int a = 0; a < UP; ++a)
+#pragma unroll
+    for (int u = 0; u < GU; ++u) {
+      const int gi = gi0 + u;
+      const int gr = gbase + gi;
+      if (q[a].big) {
+        const int c = ok[a][u] ? cell[a][u]
+                               : cell_exact_call(q[a].c0, q[a].c1, q[a].c2, rot + 12 * gr,
+                                                 off + 4 * gr, P.W, P.half, P.h, P.lg);
+        atomicAdd(&dst[(long long)gi * cells + c], milli_of(q[a].mass));
+      }
+      const bool defer = q[a].small && !ok[a][u];
+      const unsigned long long need = __ballot(defer);
+      if (need) {  // queue: one reservation per wave
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)need) - 1;
+        unsigned int base = 0u;
+        if (lane == leader) base = atomicAdd(defer_n, (unsigned int)__popcll(need));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (defer) {
+          const unsigned int slot =
+              base + __builtin_amdgcn_mbcnt_hi((unsigned int)(need >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((unsigned int)need, 0u));
+          if (slot < (unsigned int)kBatchDeferCap) {
+            defer_p[slot] = q[a].pofs;
+            defer_g[slot] = (unsigned short)gi;
+          } else {
+            const int ex = cell_exact_call(q[a].c0, q[a].c1, q[a].c2, rot + 12 * gr, off + 4 * gr,
+                                           P.W, P.half, P.h, P.lg);
+            atomicAdd(&hist[gi * cells + ex], q[a].m32);
+          }
         }
       }
     }
-  }
+}
+
+// Inputs of the particle at `p` (mass 0 when dead).
+__device__ __forceinline__ DepositParticle deposit_particle(const float4 qv, bool in, long long p,
+                                                            long long p0, const Params& P,
+                                                            unsigned int lds_limit32) {
+  // milli-mass as in the single-simulation deposit: 32-bit LDS partials
+  // while they cannot overflow, the 64-bit global grids otherwise
+  DepositParticle d;
+  const float mm = in ? qv.w * 1000.0f : 0.0f;
+  const bool live = mm >= 1.0f;
+  d.m32 = live && mm < 4294967296.0f ? (unsigned int)mm : 0u;
+  d.small = live && mm < 4294967296.0f && d.m32 <= lds_limit32;
+  d.big = live && !d.small;
+  d.mass = qv.w;
+  d.c0 = qv.x - P.half;
+  d.c1 = qv.y - P.half;
+  d.c2 = qv.z - P.half;
+  d.pfast = particle_fast(d.c0, d.c1, d.c2, P.CB);
+  d.pofs = (unsigned int)(p - p0);
+  return d;
 }
 
 __global__ __launch_bounds__(1024) void batch_deposit_kernel(
@@ -117,29 +157,38 @@ __global__ __launch_bounds__(1024) void batch_deposit_kernel(
         (unsigned int)(lds_limit < 0xFFFFFFFFull ? lds_limit : 0xFFFFFFFFull);
     const int gbase = b.grid_off + task.grid0;
     unsigned long long* dst = grids + b.cell_off + (long long)task.grid0 * cells;
-    // the next particle is loaded while this one is deposited
-    float4 qn = p0 + threadIdx.x < p1 ? pos[p0 + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
-    for (long long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-      const float4 q = qn;
-      qn = p + blockDim.x < p1 ? pos[p + blockDim.x] : make_float4(0.f, 0.f, 0.f, 0.f);
-      // milli-mass as in the single-simulation deposit: 32-bit LDS partials
-      // while they cannot overflow, the 64-bit global grids otherwise
-      const float mm = q.w * 1000.0f;
-      const bool live = mm >= 1.0f;
-      const unsigned int m32 = live && mm < 4294967296.0f ? (unsigned int)mm : 0u;
-      const bool small = live && mm < 4294967296.0f && m32 <= lds_limit32;
-      const bool big = live && !small;
-      const float c0 = q.x - P.half, c1 = q.y - P.half, c2 = q.z - P.half;
-      const int pfast = particle_fast(c0, c1, c2, P.CB);
-      int gi0 = 0;
-      for (; gi0 + 4 <= ng; gi0 += 4)
-        deposit_group<4>(gi0, gbase, cells, P, c0, c1, c2, pfast, small, big, m32, q.w,
-                         (unsigned int)(p - p0), frot, rot, off, hist, dst, defer_p, defer_g,
-                         &defer_n);
-      for (; gi0 < ng; ++gi0)
-        deposit_group<1>(gi0, gbase, cells, P, c0, c1, c2, pfast, small, big, m32, q.w,
-                         (unsigned int)(p - p0), frot, rot, off, hist, dst, defer_p, defer_g,
-                         &defer_n);
+    if (ng == 1) {
+      // one grid: two particles per lane and iteration, the next two loaded
+      // while these are deposited
+      const long long stride = 2LL * blockDim.x;
+      long long p = p0 + threadIdx.x;
+      float4 n0 = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+      float4 n1 = p + blockDim.x < p1 ? pos[p + blockDim.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+      for (; p < p1; p += stride) {  // lane-varying trip count: no barrier inside
+        const float4 a = n0, b2 = n1;
+        const long long pb = p + blockDim.x;
+        n0 = p + stride < p1 ? pos[p + stride] : make_float4(0.f, 0.f, 0.f, 0.f);
+        n1 = pb + stride < p1 ? pos[pb + stride] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const DepositParticle q[2] = {deposit_particle(a, true, p, p0, P, lds_limit32),
+                                      deposit_particle(b2, pb < p1, pb, p0, P, lds_limit32)};
+        deposit_group<2, 1>(0, gbase, cells, P, q, frot, rot, off, hist, dst, defer_p, defer_g,
+                            &defer_n);
+      }
+    } else {
+      // the next particle is loaded while this one is deposited
+      float4 qn = p0 + threadIdx.x < p1 ? pos[p0 + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+      for (long long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        const float4 qv = qn;
+        qn = p + blockDim.x < p1 ? pos[p + blockDim.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const DepositParticle q[1] = {deposit_particle(qv, true, p, p0, P, lds_limit32)};
+        int gi0 = 0;
+        for (; gi0 + 4 <= ng; gi0 += 4)
+          deposit_group<1, 4>(gi0, gbase, cells, P, q, frot, rot, off, hist, dst, defer_p, defer_g,
+                              &defer_n);
+        for (; gi0 < ng; ++gi0)
+          deposit_group<1, 1>(gi0, gbase, cells, P, q, frot, rot, off, hist, dst, defer_p, defer_g,
+                              &defer_n);
+      }
     }
     __syncthreads();
     // the deferred pairs, one per lane
