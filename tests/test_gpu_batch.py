@@ -125,3 +125,36 @@ def test_batch_ensemble_workload_runs(torch_cuda):
     want = individual_steps(torch, cfgs, batch, sts, 3)
     for s, (gp, (wp, _)) in enumerate(zip(batch.views(p), want)):
         assert np.array_equal(gp.cpu().numpy(), wp), f"sim {s}"
+
+
+def test_batch_masses_match_individual(torch_cuda):
+    """Fractional, zero, negative and very large masses (the 64-bit global
+    path of the batched deposit), in one-grid tasks (g = 32, two particles
+    per lane) and multi-grid tasks (g = 16), with chunk sizes that leave a
+    partial last pair: the same state as the individual simulations."""
+    from mog_solver_amd.batch import MogBatch
+
+    torch = torch_cuda
+    sims = [(3, 32, 9001), (9, 16, 7003)]
+    cfgs = [SolverConfig.from_dict(dict(n_grids=k, grid_size=g, world_size=100.0, G=100.0, dt=1e-6,
+                                        damping=0.999, solver="fft", num_particles=n))
+            for k, g, n in sims]
+    rng = np.random.default_rng(21)
+    sts = []
+    for _, _, n in sims:
+        pos = np.zeros((n, 4), np.float32)
+        pos[:, :3] = rng.uniform(0, 100, (n, 3))
+        pos[:, 3] = rng.uniform(0, 3, n).astype(np.float32)
+        pos[:50, 3] = 0.0
+        pos[50:100, 3] = -1.0
+        pos[100:140, 3] = np.float32(5e9)  # milli-mass 5e12: beyond 32 bits
+        sts.append((pos, np.zeros((n, 4), np.float32)))
+    batch = MogBatch(cfgs, [p.shape[0] for p, _ in sts], seeds=[3, 4])
+    want = individual_steps(torch, cfgs, batch, sts, 1)
+    for chunk, threads in [(65536, 1024), (1000, 256), (3333, 512)]:
+        batch.set_tuning(deposit_chunk=chunk, deposit_threads=threads)
+        p, v = batch.pack(sts)
+        batch.step(p, v)
+        for s, (gp, gv, (wp, wv)) in enumerate(zip(batch.views(p), batch.views(v), want)):
+            assert np.array_equal(gp.cpu().numpy(), wp), (chunk, threads, s)
+            assert np.array_equal(gv.cpu().numpy(), wv), (chunk, threads, s)

@@ -74,8 +74,10 @@ def test_deposit_c1_and_permutation(torch_cuda):
     assert np.all(want.sum(axis=(1, 2, 3)) == 65536 * 1000)
 
 
-def test_deposit_masses_and_large_values(torch_cuda):
-    """Fractional, zero, negative and very large masses (64-bit path)."""
+@pytest.mark.parametrize("k,g", [(12, 16), (3, 32)])
+def test_deposit_masses_and_large_values(torch_cuda, k, g):
+    """Fractional, zero, negative and very large masses (64-bit path), in
+    the multi-grid (g = 16) and one-grid (g = 32) deposit tilings."""
     torch = torch_cuda
     rng = np.random.default_rng(11)
     n = 50000
@@ -85,8 +87,8 @@ def test_deposit_masses_and_large_values(torch_cuda):
     pos[:100, 3] = 0.0
     pos[100:200, 3] = -1.0
     pos[200:300, 3] = np.float32(5e9)  # milli-mass 5e12: beyond 32 bits
-    s, rot, off = make_solver(12, 16)
-    want = O.deposit(pos, rot, off, O.Params.make(12, 16, 100.0))
+    s, rot, off = make_solver(k, g)
+    want = O.deposit(pos, rot, off, O.Params.make(k, g, 100.0))
     got = s.deposit(dev(torch, pos)).cpu().numpy()
     assert np.array_equal(got, want)
 
