@@ -160,6 +160,61 @@ bool green_mask32(int ix, int iy, int iz, int g, double W) {
 // DFT matrices for the matrix-core transforms, as the A-operand tables
 // [Re W | -Im W] and [Im W | Re W] (g rows of 2g floats each), forward
 // W = exp(-2 pi i jk/g) into Wf, inverse W = exp(+2 pi i jk/g) into Wi.
+// 16-bit operands as the device converts them: BF16 and FP16 by
+// round-to-nearest-even.
+static uint16_t bf16_bits(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);  // finite inputs only (DFT coefficients)
+  return (uint16_t)(u >> 16);
+}
+static float bf16_value(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float x;
+  memcpy(&x, &u, 4);
+  return x;
+}
+static uint16_t f16_bits(float x) {
+  const _Float16 h = (_Float16)x;
+  uint16_t b;
+  memcpy(&b, &h, 2);
+  return b;
+}
+
+// Floats that follow a direction's FP32 tables: the 16x16x32 A-operand
+// fragments of its two row tables, 6 parts x (g/16)^2 tiles x 64 lanes x 8
+// values of 16 bits (none below g = 16).
+size_t dft_fragment_floats(int g) {
+  if (g < 16) return 0;
+  const size_t t = (size_t)(g / 16);
+  return 6 * t * t * 64 * 8 / 2;
+}
+
+// Fragment (part p, row tile rt, k-block kb, lane l) = A[16 rt + l % 16]
+// [32 kb + 8 (l / 16) + j], j < 8, of WA (parts 0, 1, 4) or WB (parts 2, 3,
+// 5): BF16 hi and lo = bf16(a - hi) (parts 0-3), FP16 (parts 4, 5); the
+// layout of one lane's operand register for the 16x16x32 products.
+void build_dft_fragments(int g, const float* W, uint16_t* out) {
+  if (g < 16) return;
+  const int t = g / 16, g2 = 2 * g;
+  const size_t nf = (size_t)t * t * 64;
+  for (int rt = 0; rt < t; ++rt)
+    for (int kb = 0; kb < t; ++kb)
+      for (int l = 0; l < 64; ++l)
+        for (int j = 0; j < 8; ++j) {
+          const int row = 16 * rt + (l & 15), k = 32 * kb + 8 * (l >> 4) + j;
+          const size_t at = (((size_t)rt * t + kb) * 64 + l) * 8 + j;
+          const float a = W[row * g2 + k], b = W[2 * g * g + row * g2 + k];
+          const uint16_t ah = bf16_bits(a), bh = bf16_bits(b);
+          out[at] = ah;
+          out[nf * 8 + at] = bf16_bits(a - bf16_value(ah));
+          out[2 * nf * 8 + at] = bh;
+          out[3 * nf * 8 + at] = bf16_bits(b - bf16_value(bh));
+          out[4 * nf * 8 + at] = f16_bits(a);
+          out[5 * nf * 8 + at] = f16_bits(b);
+        }
+}
+
 void build_dft_tables(int g, float* Wf, float* Wi) {
   const int g2 = 2 * g;
   for (int j = 0; j < g; ++j)
@@ -682,7 +737,8 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
       dalloc(&c->d_S2, c->need_x ? cells : 1) != hipSuccess ||
       dalloc(&c->d_force, cells) != hipSuccess || dalloc(&c->d_tw, g) != hipSuccess ||
       dalloc(&c->d_gtab, g * g * g) != hipSuccess || dalloc(&c->d_kd, g) != hipSuccess ||
-      dalloc(&c->d_Wf, 4 * g * g) != hipSuccess || dalloc(&c->d_Wi, 4 * g * g) != hipSuccess) {
+      dalloc(&c->d_Wf, 4 * g * g + dft_fragment_floats((int)g)) != hipSuccess ||
+      dalloc(&c->d_Wi, 4 * g * g + dft_fragment_floats((int)g)) != hipSuccess) {
     release(c);
     delete c;
     return fail(nullptr, MOG_ENOMEM, "device allocation failed for k=%d g=%d", cfg->n_grids,
@@ -691,15 +747,18 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
   float2* tw = (float2*)malloc(g * sizeof(float2));
   float* gtab = (float*)malloc(g * g * g * sizeof(float));
   float* kd = (float*)malloc(g * sizeof(float));
-  float* Wf = (float*)malloc(4 * g * g * sizeof(float));
-  float* Wi = (float*)malloc(4 * g * g * sizeof(float));
+  const size_t wsz = 4 * g * g + dft_fragment_floats((int)g);
+  float* Wf = (float*)malloc(wsz * sizeof(float));
+  float* Wi = (float*)malloc(wsz * sizeof(float));
   build_tables(cfg, tw, gtab, kd);
   build_dft_tables((int)g, Wf, Wi);
+  build_dft_fragments((int)g, Wf, (uint16_t*)(Wf + 4 * g * g));
+  build_dft_fragments((int)g, Wi, (uint16_t*)(Wi + 4 * g * g));
   e = hipMemcpy(c->d_tw, tw, g * sizeof(float2), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_gtab, gtab, g * g * g * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_kd, kd, g * sizeof(float), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(c->d_Wf, Wf, 4 * g * g * sizeof(float), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(c->d_Wi, Wi, 4 * g * g * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_Wf, Wf, wsz * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_Wi, Wi, wsz * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(c->d_grids, 0, cells * sizeof(long long));
   if (e == hipSuccess && cfg->solver == MOG_SOLVER_CONVOLUTION &&
       cfg->compat == MOG_COMPAT_PHYSICAL) {

@@ -8,7 +8,11 @@
 //   Im(Out) = [Im W |  Re W] . [Re In ; Im In]
 // i.e. two real GEMMs of K = 2g (K = g for real input) per 16x16 output
 // tile, the DFT matrix as the A operand (FP64-derived table) and the data,
-// held in LDS as padded real/imaginary planes, as the B operand.  The DFT
+// held in LDS as padded real/imaginary planes, as the B operand.  For the
+// 16-bit products the host also stores the A operand already split (BF16
+// hi/lo) or converted (FP16), in each lane's register layout, after the FP32
+// tables: one 16-byte load per operand and k-block instead of eight
+// scattered loads and their conversions per tile.  The DFT
 // matrix is symmetric, so the transform along the fast axis x uses the same
 // product on the transposed slab (strides swapped on read and write).
 //
@@ -76,25 +80,27 @@ __device__ void wave_cgemm(const float* __restrict__ WA, const float* __restrict
         ai = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b, ai, 0, 0, 0);
       }
     } else {
+      // A operands: the host-built fragments that follow the FP32 tables
+      const uint4* AF = reinterpret_cast<const uint4*>(WA + 4 * g * g);
+      const int nf = tpr * tpr * 64;  // fragments per part
+      const int rt = r0 >> 4;
       for (int k0 = 0; k0 < K; k0 += 32) {
         const int kb = k0 + 8 * (lane >> 4);
-        float fa1[8], fa2[8], fb[8];
+        const int fi = ((rt * tpr + (k0 >> 5)) << 6) + lane;
+        float fb[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           const int k = kb + j;
-          const bool live = k < K;
-          fa1[j] = live ? WA[arow * g2 + k] : 0.0f;
-          fa2[j] = live ? WB[arow * g2 + k] : 0.0f;
-          fb[j] = live ? bval(in, g, real_in, k, bcol) * scale : 0.0f;
+          fb[j] = k < K ? bval(in, g, real_in, k, bcol) * scale : 0.0f;
         }
         if (PREC == PREC_BF16X3) {
-          v8bf a1h, a1l, a2h, a2l, bh, bl;
+          const v8bf a1h = __builtin_bit_cast(v8bf, AF[fi]);
+          const v8bf a1l = __builtin_bit_cast(v8bf, AF[nf + fi]);
+          const v8bf a2h = __builtin_bit_cast(v8bf, AF[2 * nf + fi]);
+          const v8bf a2l = __builtin_bit_cast(v8bf, AF[3 * nf + fi]);
+          v8bf bh, bl;
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            a1h[j] = (__bf16)fa1[j];
-            a1l[j] = (__bf16)(fa1[j] - (float)a1h[j]);
-            a2h[j] = (__bf16)fa2[j];
-            a2l[j] = (__bf16)(fa2[j] - (float)a2h[j]);
             bh[j] = (__bf16)fb[j];
             bl[j] = (__bf16)(fb[j] - (float)bh[j]);
           }
@@ -105,13 +111,11 @@ __device__ void wave_cgemm(const float* __restrict__ WA, const float* __restrict
           ai = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bl, ai, 0, 0, 0);
           ai = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bh, ai, 0, 0, 0);
         } else {
-          v8h a1, a2, b;
+          const v8h a1 = __builtin_bit_cast(v8h, AF[4 * nf + fi]);
+          const v8h a2 = __builtin_bit_cast(v8h, AF[5 * nf + fi]);
+          v8h b;
 #pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            a1[j] = (_Float16)fa1[j];
-            a2[j] = (_Float16)fa2[j];
-            b[j] = (_Float16)fb[j];
-          }
+          for (int j = 0; j < 8; ++j) b[j] = (_Float16)fb[j];
           ar = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b, ar, 0, 0, 0);
           ai = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2, b, ai, 0, 0, 0);
         }
