@@ -181,13 +181,14 @@ static uint16_t f16_bits(float x) {
   return b;
 }
 
-// Floats that follow a direction's FP32 tables: the 16x16x32 A-operand
-// fragments of its two row tables, 6 parts x (g/16)^2 tiles x 64 lanes x 8
-// values of 16 bits (none below g = 16).
+// Floats that follow a direction's FP32 tables: the A-operand fragments of
+// its two row tables, for the 16x16x32 products 6 parts x (g/16)^2 tiles x
+// 64 lanes x 8 values of 16 bits, then for the 16x16x4 FP32 products 2 parts
+// x (g/16) row tiles x g/2 k-steps x 64 lanes (none below g = 16).
 size_t dft_fragment_floats(int g) {
   if (g < 16) return 0;
   const size_t t = (size_t)(g / 16);
-  return 6 * t * t * 64 * 8 / 2;
+  return 6 * t * t * 64 * 8 / 2 + 2 * t * (size_t)(g / 2) * 64;
 }
 
 // Fragment (part p, row tile rt, k-block kb, lane l) = A[16 rt + l % 16]
@@ -213,6 +214,19 @@ void build_dft_fragments(int g, const float* W, uint16_t* out) {
           out[4 * nf * 8 + at] = f16_bits(a);
           out[5 * nf * 8 + at] = f16_bits(b);
         }
+  // FP32 parts for the 16x16x4 products: (row tile rt, k-step ks, lane l)
+  // holds A[16 rt + l % 16][4 ks + l / 16]
+  float* f32 = reinterpret_cast<float*>(out + 6 * nf * 8);
+  const int nk = g / 2;
+  const size_t nf32 = (size_t)t * nk * 64;
+  for (int rt = 0; rt < t; ++rt)
+    for (int ks = 0; ks < nk; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int row = 16 * rt + (l & 15), k = 4 * ks + (l >> 4);
+        const size_t at = ((size_t)rt * nk + ks) * 64 + l;
+        f32[at] = W[row * g2 + k];
+        f32[nf32 + at] = W[2 * g * g + row * g2 + k];
+      }
 }
 
 void build_dft_tables(int g, float* Wf, float* Wi) {

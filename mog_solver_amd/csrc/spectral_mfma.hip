@@ -8,11 +8,12 @@
 //   Im(Out) = [Im W |  Re W] . [Re In ; Im In]
 // i.e. two real GEMMs of K = 2g (K = g for real input) per 16x16 output
 // tile, the DFT matrix as the A operand (FP64-derived table) and the data,
-// held in LDS as padded real/imaginary planes, as the B operand.  For the
-// 16-bit products the host also stores the A operand already split (BF16
-// hi/lo) or converted (FP16), in each lane's register layout, after the FP32
-// tables: one 16-byte load per operand and k-block instead of eight
-// scattered loads and their conversions per tile.  The DFT
+// held in LDS as padded real/imaginary planes, as the B operand.  The host
+// also stores the A operand in each lane's register layout after the FP32
+// row tables: split (BF16 hi/lo) or converted (FP16) for the 16-bit
+// products, reordered for the FP32 ones, so a lane's operand is one
+// coalesced load (16 bytes per k-block, 4 bytes per k-step) instead of
+// scattered row reads and their conversions per tile.  The DFT
 // matrix is symmetric, so the transform along the fast axis x uses the same
 // product on the transposed slab (strides swapped on read and write).
 //
@@ -71,10 +72,15 @@ __device__ void wave_cgemm(const float* __restrict__ WA, const float* __restrict
     const int arow = r0 + (lane & 15);
     const int bcol = c0 + (lane & 15);
     if (PREC == PREC_FP32) {
+      // A operands from the host-built FP32 fragments (after the 16-bit ones)
+      const float* AF32 = WA + 4 * g * g + 6 * tpr * tpr * 256;
+      const int nk = g >> 1, nf32 = tpr * nk * 64;
+      const int rt = r0 >> 4;
       for (int k0 = 0; k0 < K; k0 += 4) {
         const int k = k0 + (lane >> 4);
-        const float a1 = WA[arow * g2 + k];
-        const float a2 = WB[arow * g2 + k];
+        const int fi = ((rt * nk + (k0 >> 2)) << 6) + lane;
+        const float a1 = AF32[fi];
+        const float a2 = AF32[nf32 + fi];
         const float b = bval(in, g, real_in, k, bcol);
         ar = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b, ar, 0, 0, 0);
         ai = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b, ai, 0, 0, 0);
