@@ -53,27 +53,25 @@ __device__ __forceinline__ float bval(const Planes& in, int g, bool real_in, int
 }
 
 // One wave computes the output tiles t = t0, t0 + tstep, ... of
-// Out = W . In with W given by its two K = 2g row tables WA = [Re W | -Im W]
-// and WB = [Im W | Re W] (row-major, g rows of 2g floats).
+// Out = W . In.  W points at a direction's table: the two K = 2g row tables
+// WA = [Re W | -Im W] and WB = [Im W | Re W] (row-major, g rows of 2g
+// floats), then their A-operand fragments, which are what the products read.
 // `scale` multiplies the data before FP16 conversion (1 otherwise); the
 // result is multiplied by 1/scale.
 template <int PREC>
-__device__ void wave_cgemm(const float* __restrict__ WA, const float* __restrict__ WB, int g,
-                           const Planes in, bool real_in, const Planes out, int t0, int tstep,
-                           float scale) {
+__device__ void wave_cgemm(const float* __restrict__ W, int g, const Planes in, bool real_in,
+                           const Planes out, int t0, int tstep, float scale) {
   const int lane = threadIdx.x & 63;
   const int tpr = g >> 4;  // tiles per row of tiles
   const int nt = tpr * tpr;
   const int K = real_in ? g : 2 * g;
-  const int g2 = 2 * g;
   for (int t = t0; t < nt; t += tstep) {
     const int r0 = (t / tpr) * 16, c0 = (t % tpr) * 16;
     v4f ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
-    const int arow = r0 + (lane & 15);
     const int bcol = c0 + (lane & 15);
     if (PREC == PREC_FP32) {
       // A operands from the host-built FP32 fragments (after the 16-bit ones)
-      const float* AF32 = WA + 4 * g * g + 6 * tpr * tpr * 256;
+      const float* AF32 = W + 4 * g * g + 6 * tpr * tpr * 256;
       const int nk = g >> 1, nf32 = tpr * nk * 64;
       const int rt = r0 >> 4;
       for (int k0 = 0; k0 < K; k0 += 4) {
@@ -87,7 +85,7 @@ __device__ void wave_cgemm(const float* __restrict__ WA, const float* __restrict
       }
     } else {
       // A operands: the host-built fragments that follow the FP32 tables
-      const uint4* AF = reinterpret_cast<const uint4*>(WA + 4 * g * g);
+      const uint4* AF = reinterpret_cast<const uint4*>(W + 4 * g * g);
       const int nf = tpr * tpr * 64;  // fragments per part
       const int rt = r0 >> 4;
       for (int k0 = 0; k0 < K; k0 += 32) {
@@ -175,8 +173,7 @@ __global__ __launch_bounds__(256) void mfma_fwd_xy_kernel(const long long* __res
   float* T = X + plane;      // re, im at +plane
   float* U = T + 2 * plane;  // re, im at +plane
   float* red = U + 2 * plane;
-  const float* WA = Wf;
-  const float* WB = Wf + 2 * g * g;
+  const float* W = Wf;
   const int gg = g * g, m = g - 1;
   const long long base = (long long)blockIdx.x * gg;
   for (int i = threadIdx.x; i < gg; i += blockDim.x)
@@ -185,11 +182,11 @@ __global__ __launch_bounds__(256) void mfma_fwd_xy_kernel(const long long* __res
   const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   float sc = slab_scale<PREC>(X, g, lg, ld, 0, false, red);
   // x: T = X . W  <=>  T^T = W . X^T
-  wave_cgemm<PREC>(WA, WB, g, Planes{X, 1, ld, 0}, true, Planes{T, 1, ld, plane}, wave, nw, sc);
+  wave_cgemm<PREC>(W, g, Planes{X, 1, ld, 0}, true, Planes{T, 1, ld, plane}, wave, nw, sc);
   __syncthreads();
   sc = slab_scale<PREC>(T, g, lg, ld, plane, true, red);
   // y: U = W . T
-  wave_cgemm<PREC>(WA, WB, g, Planes{T, ld, 1, plane}, false, Planes{U, ld, 1, plane}, wave, nw,
+  wave_cgemm<PREC>(W, g, Planes{T, ld, 1, plane}, false, Planes{U, ld, 1, plane}, wave, nw,
                    sc);
   __syncthreads();
   for (int i = threadIdx.x; i < gg; i += blockDim.x) {
@@ -228,7 +225,7 @@ __global__ __launch_bounds__(256) void mfma_z_green_kernel(float2* __restrict__ 
   __syncthreads();
   const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   float sc = slab_scale<PREC>(L, g, lg, ld, plane, true, red);
-  wave_cgemm<PREC>(Wf, Wf + 2 * g * g, g, Planes{L, ld, 1, plane}, false, Planes{R, ld, 1, plane},
+  wave_cgemm<PREC>(Wf, g, Planes{L, ld, 1, plane}, false, Planes{R, ld, 1, plane},
                    wave, nw, sc);
   __syncthreads();
   const float kyv = kd[ky];
@@ -258,12 +255,12 @@ __global__ __launch_bounds__(256) void mfma_z_green_kernel(float2* __restrict__ 
   }
   __syncthreads();
   sc = slab_scale<PREC>(Z, g, lg, ld, plane, true, red);
-  wave_cgemm<PREC>(Wi, Wi + 2 * g * g, g, Planes{Z, ld, 1, plane}, false, Planes{R, ld, 1, plane},
+  wave_cgemm<PREC>(Wi, g, Planes{Z, ld, 1, plane}, false, Planes{R, ld, 1, plane},
                    wave, nw, sc);
   if (need_x) {
     __syncthreads();
     sc = slab_scale<PREC>(X, g, lg, ld, plane, true, red);
-    wave_cgemm<PREC>(Wi, Wi + 2 * g * g, g, Planes{X, ld, 1, plane}, false,
+    wave_cgemm<PREC>(Wi, g, Planes{X, ld, 1, plane}, false,
                      Planes{L, ld, 1, plane}, wave, nw, sc);
   }
   __syncthreads();
@@ -291,8 +288,7 @@ __global__ __launch_bounds__(256) void mfma_inv_xy_kernel(const float2* __restri
   float* T = Z + 2 * plane;
   float* X = T + 2 * plane;
   float* red = X + 2 * plane;
-  const float* WA = Wi;
-  const float* WB = Wi + 2 * g * g;
+  const float* W = Wi;
   const int gg = g * g, m = g - 1;
   const long long base = (long long)blockIdx.x * gg;
   const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -305,10 +301,10 @@ __global__ __launch_bounds__(256) void mfma_inv_xy_kernel(const float2* __restri
   }
   __syncthreads();
   float sc = slab_scale<PREC>(Z, g, lg, ld, plane, true, red);
-  wave_cgemm<PREC>(WA, WB, g, Planes{Z, ld, 1, plane}, false, Planes{T, ld, 1, plane}, wave, nw, sc);
+  wave_cgemm<PREC>(W, g, Planes{Z, ld, 1, plane}, false, Planes{T, ld, 1, plane}, wave, nw, sc);
   __syncthreads();
   sc = slab_scale<PREC>(T, g, lg, ld, plane, true, red);
-  wave_cgemm<PREC>(WA, WB, g, Planes{T, 1, ld, plane}, false, Planes{Z, 1, ld, plane}, wave, nw, sc);
+  wave_cgemm<PREC>(W, g, Planes{T, 1, ld, plane}, false, Planes{Z, 1, ld, plane}, wave, nw, sc);
   if (need_x) {
     __syncthreads();
     for (int i = threadIdx.x; i < gg; i += blockDim.x) {
@@ -319,11 +315,11 @@ __global__ __launch_bounds__(256) void mfma_inv_xy_kernel(const float2* __restri
     }
     __syncthreads();
     sc = slab_scale<PREC>(X, g, lg, ld, plane, true, red);
-    wave_cgemm<PREC>(WA, WB, g, Planes{X, ld, 1, plane}, false, Planes{T, ld, 1, plane}, wave, nw,
+    wave_cgemm<PREC>(W, g, Planes{X, ld, 1, plane}, false, Planes{T, ld, 1, plane}, wave, nw,
                      sc);
     __syncthreads();
     sc = slab_scale<PREC>(T, g, lg, ld, plane, true, red);
-    wave_cgemm<PREC>(WA, WB, g, Planes{T, 1, ld, plane}, false, Planes{X, 1, ld, plane}, wave, nw,
+    wave_cgemm<PREC>(W, g, Planes{T, 1, ld, plane}, false, Planes{X, 1, ld, plane}, wave, nw,
                      sc);
   }
   __syncthreads();
