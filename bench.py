@@ -106,6 +106,22 @@ def cpu_pool_forces(ex, workers, pos_h, rot, off, g, W, G, compat):
     return total / k, time.perf_counter() - t0
 
 
+def parse_tune(items) -> dict:
+    """``["key=INT", ...]`` of ``--tune`` as a dict (SystemExit on a malformed
+    item; unknown keys are rejected by ``MogSolver.set_tuning``)."""
+    fields = {}
+    for item in items:
+        key, sep, val = item.partition("=")
+        key = key.strip()
+        try:
+            if not sep or not key:
+                raise ValueError(item)
+            fields[key] = int(val)
+        except ValueError:
+            raise SystemExit(f"--tune expects KEY=INT, got {item!r}") from None
+    return fields
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,13 +219,7 @@ def main():
               file=sys.stderr, flush=True)
 
     if args.tune:
-        fields = {}
-        for item in args.tune:
-            key, sep, val = item.partition("=")
-            if not sep:
-                raise SystemExit(f"--tune expects KEY=INT, got {item!r}")
-            fields[key.strip()] = int(val)
-        solver.set_tuning(**fields)
+        solver.set_tuning(**parse_tune(args.tune))
         print(f"[bench] tuning set: {solver.get_tuning()}", file=sys.stderr, flush=True)
 
     use_graph = not args.no_graph
