@@ -62,7 +62,7 @@ def candidate_stages(g: int):
         for G in (1, 2, 4, 8):
             if G > 1 and G * grid_bytes > 160 * 1024:
                 continue
-            for wg in (1024, 2048, 4096):
+            for wg in (128, 256, 512, 1024, 2048, 4096):
                 tiles.append(dict(deposit_threads=threads, deposit_grids_per_wg=G, deposit_wg_target=wg))
     return [
         tiles,
