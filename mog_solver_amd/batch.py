@@ -128,7 +128,7 @@ def autotune(batch: MogBatch, pos, vel, steps=5, verbose=False):
 
     p, v = pos.clone(), vel.clone()
     best = None
-    for chunk in (4096, 8192, 16384, 32768, 65536):
+    for chunk in (4096, 8192, 16384, 32768, 65536, 131072, 262144):
         for threads in (512, 1024):
             batch.set_tuning(deposit_chunk=chunk, deposit_threads=threads)
             batch.run(p, v, 1)
