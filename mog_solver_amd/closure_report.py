@@ -11,8 +11,9 @@
    (relative RMS); every ``--direct-every`` steps the k_full, k_small and
    closure forces are compared with FP64 direct summation (softening one
    cell) at a ``--subsample`` of particles.
-4. Time a plain k_full step (hipGraph replay) and a closure step (features,
-   MLP, update), both with inputs resident on the device.
+4. Autotune both solvers' tilings (unless ``--no-autotune``), then time a
+   plain k_full step (hipGraph replay) and a closure step (features, MLP,
+   update), both with inputs resident on the device.
 
 Prints one JSON object.
 """
@@ -48,6 +49,8 @@ def main(argv=None):
     ap.add_argument("--subsample", type=int, default=4096)
     ap.add_argument("--time-steps", type=int, default=100)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time both solvers with the library's default tuning")
     args = ap.parse_args(argv)
 
     import torch
@@ -136,6 +139,18 @@ def main(argv=None):
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / steps
 
+    if not args.no_autotune:  # both solvers, on copies of the state
+        from .autotune import autotune
+
+        # the k_small solver is tuned on its plain step, which can do without
+        # the Morton visiting order; the closure's feature pass cannot, so
+        # that order is kept
+        resort = small.get_tuning()["resort_every"]
+        out["tuning"] = {name: autotune(sv, p, v, cache_path=None)["tuning"]
+                         for name, sv in (("k_full", full), ("k_small", small))}
+        small.set_tuning(resort_every=resort)
+        out["tuning"]["k_small"] = small.get_tuning()
+        log(f"autotuned: {out['tuning']}")
     K = args.time_steps
     t_full = clock(lambda s: full.run(p, v, s, use_graph=True), K)
     runner = CL.ClosureSolver(full, small, mlp, rho0, audit_every=0)
