@@ -131,7 +131,10 @@ int mog_update(mog_ctx* ctx, float* pos_n4, float* vel_n4, const float* avg_forc
 /* mog_step with HIP events recorded between its kernels on `stream`; waits
  * for the stream and writes the durations in milliseconds of the phases
  * {grid clear, deposit, spectral solve, gather+update} to phase_ms[0..3].
- * The reference has no timing facility; this exists for the benchmark. */
+ * The clear phase holds no kernel (only the sorted-deposit tuning's
+ * permutation): the grids a step deposits into were cleared by the
+ * previous step's gather+update.  The reference has no timing facility;
+ * this exists for the benchmark. */
 int mog_step_profiled(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, float* phase_ms,
                       void* stream);
 
@@ -200,7 +203,9 @@ int mog_get_tuning(mog_ctx* ctx, mog_tuning* out);
 int mog_set_tuning(mog_ctx* ctx, const mog_tuning* t);
 
 /* Device pointers of the context's own density grids (int64) and force
- * grids (float4) as left by the last mog_step; NULL on error. */
+ * grids (float4) as left by the last mog_step; NULL on error.  The density
+ * pointer alternates between two buffers from one step to the next, so it
+ * is valid until the next step. */
 const int64_t* mog_grids_ptr(mog_ctx* ctx);
 const float* mog_force_ptr(mog_ctx* ctx);
 

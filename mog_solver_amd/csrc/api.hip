@@ -28,7 +28,11 @@ struct mog_ctx {
   float* d_rot;
   float* d_off;
   float* d_frot;  // fast-path rows (cellmath.h make_fast_row)
+  // density grids: d_grids holds the last step's (mog_grids_ptr); the step
+  // deposits into d_grids_next, which is all zero between steps, and its
+  // gather clears the old d_grids, after which the two swap
   long long* d_grids;
+  long long* d_grids_next;
   float2* d_S1;
   float2* d_S2;
   float4* d_force;
@@ -64,9 +68,12 @@ struct mog_ctx {
   float4* d_spos;
   unsigned short* d_cells;
 
-  // graph cache of mog_run
+  // graph cache of mog_run: one captured step per assignment of the two
+  // density buffers (gexec deposits into g_dep, gexec_alt into the other)
   hipStream_t cap_stream;
   hipGraphExec_t gexec;
+  hipGraphExec_t gexec_alt;
+  const long long* g_dep;
   const void* g_pos;
   const void* g_vel;
   long long g_n;
@@ -359,11 +366,13 @@ hipError_t dalloc(T** p, size_t count) {
 void release(mog_ctx* c) {
   if (!c) return;
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->gexec_alt) (void)hipGraphExecDestroy(c->gexec_alt);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   (void)hipFree(c->d_rot);
   (void)hipFree(c->d_off);
   (void)hipFree(c->d_frot);
   (void)hipFree(c->d_grids);
+  (void)hipFree(c->d_grids_next);
   (void)hipFree(c->d_S1);
   (void)hipFree(c->d_S2);
   (void)hipFree(c->d_force);
@@ -499,39 +508,55 @@ hipError_t solve_dispatch(mog_ctx* ctx, const long long* grids, float4* force, i
                       ctx->d_conv, ctx->need_x, ctx->d_rot, world, s);
 }
 
+void drop_graphs(mog_ctx* ctx) {
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->gexec_alt) (void)hipGraphExecDestroy(ctx->gexec_alt);
+  ctx->gexec = ctx->gexec_alt = nullptr;
+}
+
+// One step on stream s: deposit into `dep` (all zero), solve, gather +
+// update, and clear `old` on the way (the gather's side job).  The caller
+// then makes dep the current density and old the zero spare (swap_grids)
+// once the step is launched, not when it is only captured.
 hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const int* order,
-                        hipStream_t s) {
+                        long long* dep, long long* old, hipStream_t s) {
   const Params& P = ctx->P;
+  const long long cells = (long long)P.k * P.cells;
   if (order && ctx->sorted_deposit && ctx->d_spos && ctx->d_cells) {
     // sorted step: positions in visiting order, deposit with aggregated
     // atomics (and a cell cache), gather from the cache or recomputed cells
     const bool cache = ctx->cell_cache;
     hipError_t e = launch_permute((const float4*)pos, order, n, ctx->d_spos, s);
-    if (e == hipSuccess) e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
     if (e == hipSuccess)
       e = launch_deposit_ex(P, ctx->d_spos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                            (unsigned long long*)ctx->d_grids, cache ? ctx->d_cells : nullptr,
-                            true, ctx->tune, s);
-    if (e == hipSuccess) e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
+                            (unsigned long long*)dep, cache ? ctx->d_cells : nullptr, true,
+                            ctx->tune, s);
+    if (e == hipSuccess) e = solve_dispatch(ctx, dep, ctx->d_force, 1, s);
     if (e == hipSuccess) {
-      if (cache)
+      if (cache) {
         e = launch_gather_cached(P, (float4*)pos, (float4*)vel, n, order, ctx->d_cells,
                                  ctx->d_force, s);
-      else
+        if (e == hipSuccess) e = launch_fill_zero(old, cells, s);
+      } else {
         e = launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
-                                 ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s);
+                                 ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s, old, cells);
+      }
     }
     return e;
   }
-  hipError_t e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
+  hipError_t e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
+                                (unsigned long long*)dep, ctx->tune, s);
   if (e != hipSuccess) return e;
-  e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                     (unsigned long long*)ctx->d_grids, ctx->tune, s);
-  if (e != hipSuccess) return e;
-  e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
+  e = solve_dispatch(ctx, dep, ctx->d_force, 1, s);
   if (e != hipSuccess) return e;
   return launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
-                              ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s);
+                              ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s, old, cells);
+}
+
+void swap_grids(mog_ctx* ctx) {
+  long long* t = ctx->d_grids;
+  ctx->d_grids = ctx->d_grids_next;
+  ctx->d_grids_next = t;
 }
 
 
@@ -747,7 +772,8 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
   const size_t k = (size_t)cfg->n_grids, g = (size_t)cfg->grid_size, cells = k * g * g * g;
   if (dalloc(&c->d_rot, k * 12) != hipSuccess || dalloc(&c->d_off, k * 4) != hipSuccess ||
       dalloc(&c->d_frot, k * 12) != hipSuccess ||
-      dalloc(&c->d_grids, cells) != hipSuccess || dalloc(&c->d_S1, cells) != hipSuccess ||
+      dalloc(&c->d_grids, cells) != hipSuccess || dalloc(&c->d_grids_next, cells) != hipSuccess ||
+      dalloc(&c->d_S1, cells) != hipSuccess ||
       dalloc(&c->d_S2, c->need_x ? cells : 1) != hipSuccess ||
       dalloc(&c->d_force, cells) != hipSuccess || dalloc(&c->d_tw, g) != hipSuccess ||
       dalloc(&c->d_gtab, g * g * g) != hipSuccess || dalloc(&c->d_kd, g) != hipSuccess ||
@@ -774,6 +800,7 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
   if (e == hipSuccess) e = hipMemcpy(c->d_Wf, Wf, wsz * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_Wi, Wi, wsz * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(c->d_grids, 0, cells * sizeof(long long));
+  if (e == hipSuccess) e = hipMemset(c->d_grids_next, 0, cells * sizeof(long long));
   if (e == hipSuccess && cfg->solver == MOG_SOLVER_CONVOLUTION &&
       cfg->compat == MOG_COMPAT_PHYSICAL) {
     // the reference runs the FFT solver for either key (main.py:671-672);
@@ -832,10 +859,7 @@ int mog_set_transforms(mog_ctx* ctx, const float* rot_k12, const float* off_k4) 
   if (e != hipSuccess) return hip_fail(ctx, e, "mog_set_transforms");
   // The fast path is only admitted with a small margin (sane transforms).
   P.lim = margin < 0.05 ? (float)(0.5 - margin) : -1.0f;
-  if (ctx->gexec) {  // kernel arguments of a captured step changed
-    (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
+  drop_graphs(ctx);  // kernel arguments of a captured step changed
   ctx->have_transforms = true;
   return MOG_OK;
 }
@@ -847,8 +871,11 @@ int mog_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, void* stream) {
   if (n == 0) return MOG_OK;
   const int* order = nullptr;
   hipError_t e = refresh_order(ctx, pos, n, (hipStream_t)stream, &order);
-  if (e == hipSuccess) e = enqueue_step(ctx, pos, vel, n, order, (hipStream_t)stream);
-  return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_step");
+  if (e == hipSuccess)
+    e = enqueue_step(ctx, pos, vel, n, order, ctx->d_grids_next, ctx->d_grids, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "mog_step");
+  swap_grids(ctx);
+  return MOG_OK;
 }
 
 int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int32_t use_graph,
@@ -864,8 +891,10 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
     for (int i = 0; i < nsteps; ++i) {
       const int* order = nullptr;
       if ((e = refresh_order(ctx, pos, n, s, &order)) != hipSuccess ||
-          (e = enqueue_step(ctx, pos, vel, n, order, s)) != hipSuccess)
+          (e = enqueue_step(ctx, pos, vel, n, order, ctx->d_grids_next, ctx->d_grids, s)) !=
+              hipSuccess)
         return hip_fail(ctx, e, "mog_run");
+      swap_grids(ctx);
     }
     return MOG_OK;
   }
@@ -873,34 +902,40 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
   // direct launches between graph replays (the sort is not captured).
   const int* const order_buf_before = ctx->d_order;
   const bool ordered = ensure_order(ctx, n);
-  if (ctx->gexec && (ctx->d_order != order_buf_before || ordered != ctx->g_ordered)) {
-    (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
+  if (ctx->gexec && (ctx->d_order != order_buf_before || ordered != ctx->g_ordered)) drop_graphs(ctx);
   if (!ctx->gexec || ctx->g_pos != pos || ctx->g_vel != vel || ctx->g_n != n) {
-    if (ctx->gexec) {
-      (void)hipGraphExecDestroy(ctx->gexec);
-      ctx->gexec = nullptr;
-    }
+    drop_graphs(ctx);
     if (!ctx->cap_stream &&
         (e = hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking)) != hipSuccess)
       return hip_fail(ctx, e, "hipStreamCreate");
-    if ((e = hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
-      return hip_fail(ctx, e, "hipStreamBeginCapture");
-    hipError_t ee = enqueue_step(ctx, pos, vel, n, ordered ? ctx->d_order : nullptr, ctx->cap_stream);
-    hipGraph_t graph = nullptr;
-    e = hipStreamEndCapture(ctx->cap_stream, &graph);
-    if (ee != hipSuccess) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return hip_fail(ctx, ee, "capture");
+    // both assignments of the density buffers: a replay deposits into the
+    // current spare and clears the current density
+    long long* const bufs[2] = {ctx->d_grids_next, ctx->d_grids};
+    hipGraphExec_t* const slot[2] = {&ctx->gexec, &ctx->gexec_alt};
+    for (int a = 0; a < 2; ++a) {
+      if ((e = hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal)) !=
+          hipSuccess) {
+        drop_graphs(ctx);
+        return hip_fail(ctx, e, "hipStreamBeginCapture");
+      }
+      hipError_t ee = enqueue_step(ctx, pos, vel, n, ordered ? ctx->d_order : nullptr, bufs[a],
+                                   bufs[1 - a], ctx->cap_stream);
+      hipGraph_t graph = nullptr;
+      e = hipStreamEndCapture(ctx->cap_stream, &graph);
+      if (ee != hipSuccess || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        drop_graphs(ctx);
+        return hip_fail(ctx, ee != hipSuccess ? ee : e, "capture");
+      }
+      e = hipGraphInstantiate(slot[a], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        *slot[a] = nullptr;
+        drop_graphs(ctx);
+        return hip_fail(ctx, e, "hipGraphInstantiate");
+      }
     }
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
-    e = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-      ctx->gexec = nullptr;
-      return hip_fail(ctx, e, "hipGraphInstantiate");
-    }
+    ctx->g_dep = bufs[0];
     ctx->g_pos = pos;
     ctx->g_vel = vel;
     ctx->g_n = n;
@@ -911,7 +946,9 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
       const int* order = nullptr;
       if ((e = refresh_order(ctx, pos, n, s, &order)) != hipSuccess) return hip_fail(ctx, e, "mog_run");
     }
-    if ((e = hipGraphLaunch(ctx->gexec, s)) != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+    hipGraphExec_t gx = ctx->d_grids_next == ctx->g_dep ? ctx->gexec : ctx->gexec_alt;
+    if ((e = hipGraphLaunch(gx, s)) != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+    swap_grids(ctx);
   }
   return MOG_OK;
 }
@@ -933,25 +970,32 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
   const bool cache = sorted && ctx->cell_cache;
   for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
   if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+  // the step's own sequence (its clear phase is empty: the grids it deposits
+  // into were cleared by the previous step's gather)
+  long long* const dep = ctx->d_grids_next;
+  long long* const old = ctx->d_grids;
+  const long long cells = (long long)P.k * P.cells;
   if (e == hipSuccess && sorted) e = launch_permute((const float4*)pos, order, n, ctx->d_spos, s);
-  if (e == hipSuccess) e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
   if (e == hipSuccess) e = hipEventRecord(ev[1], s);
   if (e == hipSuccess)
     e = launch_deposit_ex(P, sorted ? ctx->d_spos : (const float4*)pos, n, ctx->d_rot, ctx->d_off,
-                          ctx->d_frot, (unsigned long long*)ctx->d_grids,
-                          cache ? ctx->d_cells : nullptr, sorted, ctx->tune, s);
+                          ctx->d_frot, (unsigned long long*)dep, cache ? ctx->d_cells : nullptr,
+                          sorted, ctx->tune, s);
   if (e == hipSuccess) e = hipEventRecord(ev[2], s);
-  if (e == hipSuccess) e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
+  if (e == hipSuccess) e = solve_dispatch(ctx, dep, ctx->d_force, 1, s);
   if (e == hipSuccess) e = hipEventRecord(ev[3], s);
   if (e == hipSuccess) {
-    if (cache)
+    if (cache) {
       e = launch_gather_cached(P, (float4*)pos, (float4*)vel, n, order, ctx->d_cells, ctx->d_force,
                                s);
-    else
+      if (e == hipSuccess) e = launch_fill_zero(old, cells, s);
+    } else {
       e = launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
-                               ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s);
+                               ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s, old, cells);
+    }
   }
   if (e == hipSuccess) e = hipEventRecord(ev[4], s);
+  if (e == hipSuccess) swap_grids(ctx);
   if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
   for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventElapsedTime(&phase_ms[i], ev[i], ev[i + 1]);
   for (int i = 0; i < 5; ++i)
@@ -1019,10 +1063,7 @@ int mog_set_integrator(mog_ctx* ctx, double dt, double damping, double timescale
   ctx->cfg.timescale = timescale;
   ctx->P.dtp = (float)(dt * timescale);
   ctx->P.damping = (float)damping;
-  if (ctx->gexec) {  // kernel arguments of a captured step changed
-    (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
+  drop_graphs(ctx);  // kernel arguments of a captured step changed
   return MOG_OK;
 }
 
@@ -1194,10 +1235,7 @@ int mog_set_tuning(mog_ctx* ctx, const mog_tuning* t) {
   ctx->spectral_mfma = t->spectral_mfma != 0;
   if (realloc) free_order(ctx);  // buffers follow the sorted-deposit choice
   ctx->order_pos = nullptr;      // rebuild the order on the next step
-  if (ctx->gexec) {
-    (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
+  drop_graphs(ctx);
   return MOG_OK;
 }
 

@@ -36,9 +36,13 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, long long n,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
     const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode,
-    const int* __restrict__ order) {
+    const int* __restrict__ order, long long* __restrict__ zero, long long zero_count) {
   constexpr int U = 4;
   extern __shared__ float adds[];  // LDS3: adds[3 g + m] = frot[12 g + 4 m + 3]
+  // side job: clear `zero` (the density grids the next step deposits into)
+  for (long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x; z < zero_count;
+       z += (long long)gridDim.x * blockDim.x)
+    zero[z] = 0;
   if (LDS3) {
     for (int i = threadIdx.x; i < 3 * P.k; i += blockDim.x) adds[i] = frot[12 * (i / 3) + 4 * (i % 3) + 3];
     __syncthreads();
@@ -185,23 +189,28 @@ __global__ __launch_bounds__(256) void update_kernel(float4* __restrict__ pos,
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
                                 const float* rot, const float* off, const float* frot,
                                 const float4* force, int world, float4* avg_out, int mode,
-                                const int* order, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
+                                const int* order, hipStream_t s, long long* zero,
+                                long long zero_count) {
+  if (n <= 0) return zero_count > 0 ? launch_fill_zero(zero, zero_count, s) : hipSuccess;
   const long long blocks = (n + 255) / 256;
   const size_t lds = (size_t)P.k * 3 * sizeof(float);
   const bool lds3 = lds <= 16 * 1024;  // k <= 1365: a few KB per workgroup
   if (world && lds3)
     hipLaunchKernelGGL((gather_update_kernel<true, true>), dim3((unsigned)blocks), dim3(256), lds, s,
-                       pos, vel, n, rot, off, frot, force, P, avg_out, mode, order);
+                       pos, vel, n, rot, off, frot, force, P, avg_out, mode, order, zero,
+                       zero_count);
   else if (world)
     hipLaunchKernelGGL((gather_update_kernel<true, false>), dim3((unsigned)blocks), dim3(256), 0, s,
-                       pos, vel, n, rot, off, frot, force, P, avg_out, mode, order);
+                       pos, vel, n, rot, off, frot, force, P, avg_out, mode, order, zero,
+                       zero_count);
   else if (lds3)
     hipLaunchKernelGGL((gather_update_kernel<false, true>), dim3((unsigned)blocks), dim3(256), lds,
-                       s, pos, vel, n, rot, off, frot, force, P, avg_out, mode, order);
+                       s, pos, vel, n, rot, off, frot, force, P, avg_out, mode, order, zero,
+                       zero_count);
   else
     hipLaunchKernelGGL((gather_update_kernel<false, false>), dim3((unsigned)blocks), dim3(256), 0,
-                       s, pos, vel, n, rot, off, frot, force, P, avg_out, mode, order);
+                       s, pos, vel, n, rot, off, frot, force, P, avg_out, mode, order, zero,
+                       zero_count);
   return hipGetLastError();
 }
 

@@ -148,10 +148,13 @@ hipError_t launch_solve_mfma(const Params& P, int precision, const long long* gr
                              float2* S2, float4* force, const float* Wf, const float* Wi,
                              const float* gtab, const float* kd, const float2* conv, int need_x,
                              const float* rot, int world, hipStream_t s);
+// zero != NULL: the launch also clears zero[0, zero_count) (the step's
+// spare density grids), spread over all of its lanes.
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
                                 const float* rot, const float* off, const float* frot,
                                 const float4* force, int world, float4* avg_out, int mode,
-                                const int* order, hipStream_t s);
+                                const int* order, hipStream_t s, long long* zero = nullptr,
+                                long long zero_count = 0);
 // Morton visiting order of the particles (order.hip): temp storage size of
 // the radix sort, and the order itself (keys_a/keys_b/idx_a: n-element
 // scratch; order: n-element output).

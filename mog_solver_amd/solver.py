@@ -176,7 +176,9 @@ class MogSolver:
 
     def step_profiled(self, pos, vel, stream=None):
         """One step; returns the HIP-event durations (ms) of its phases
-        {clear, deposit, solve, gather_update} (synchronises the stream)."""
+        {clear, deposit, solve, gather_update} (synchronises the stream).
+        "clear" holds no kernel: the previous step's gather cleared the grids
+        this one deposits into."""
         _check_particles(pos, "pos")
         _check_particles(vel, "vel")
         ms = (ctypes.c_float * 4)()
