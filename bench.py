@@ -47,11 +47,11 @@ FP32_VECTOR_PEAK_TFS = 157.3
 PMC_TRAFFIC_KIB = {
     ("two_disks", 1_000_000, 240, 16, "fp32", 2, 4096): {
         "deposit": (9_585.8, 61_289.1),
-        "gather_update": (124_124.5, 66_044.3),
+        "gather_update": (124_184.8, 73_676.0),
     },
     ("two_disks", 1_000_000, 240, 16, "fp32", 2, 2048): {
         "deposit": (10_627.5, 33_642.9),
-        "gather_update": (124_121.8, 66_047.9),
+        "gather_update": (124_184.8, 73_672.3),
     },
 }
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
