@@ -37,9 +37,11 @@ __device__ __forceinline__ float2 cmul_tw(float2 a, float2 w) {
 // Grid sizes from which the slab passes use lds_fft instead of dense DFTs.
 constexpr int kFftMinG = 16;
 
-// Mixed radix-4/2 Stockham FFT (natural order in and out) of `lines`
+// Mixed radix-8/4/2 Stockham FFT (natural order in and out) of `lines`
 // independent length-g lines held in LDS, element (line l, position p) at
-// p*ps + l*ls.  Stages of radix 4 (one of radix 2 when log2 g is odd); with
+// p*ps + l*ls.  Stages of radix 8 while at least three factors of two
+// remain (but not exactly four: g = 16 runs 4 x 4, g = 32 8 x 4, g = 64
+// 8 x 8), then radix 4, then radix 2; with
 // Ns the product of the radices of the earlier stages, q = g / R, a stage
 // reads one buffer and writes the other:
 //   v_r = a[j + r q] * w^r,  w = e^{-+2 pi i (j mod Ns) / (R Ns)},  r < R,
@@ -56,7 +58,57 @@ __device__ __forceinline__ float2* lds_fft(float2* a, float2* b, int g, int lg, 
   const bool line_fast = ls == 1;
   int Ns = 1, rem = lg, lNs = 0;
   while (rem > 0) {
-    if (rem >= 2) {  // radix 4
+    if (rem >= 3 && rem != 4) {  // radix 8 (g = 32: 8 x 4; g = 64: 8 x 8)
+      const int q = g >> 3, lq = lg - 3;
+      const int nb = lines * q;
+      for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+        const int l = line_fast ? t & (lines - 1) : t >> lq;
+        const int j = line_fast ? t / lines : t & (q - 1);
+        float2 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = a[(j + r * q) * ps + l * ls];
+        const int k = j & (Ns - 1);
+        const int ti = k << (lg - 3 - lNs);  // k * g / (8 Ns)
+#pragma unroll
+        for (int r = 1; r < 8; ++r) {
+          const float2 w = twl[(r * ti) & (g - 1)];
+          v[r] = inverse ? cmul_tw(v[r], w) : cmul_conj_tw(v[r], w);
+        }
+        // 8-point DFT as two 4-point DFTs (even, odd inputs) joined with
+        // the eighth roots of unity: V_m = E_m + W^m O_m, V_m+4 = E_m - W^m O_m
+        float2 E[4], O[4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const float2 x0 = v[h], x1 = v[h + 2], x2 = v[h + 4], x3 = v[h + 6];
+          const float2 s02 = make_float2(x0.x + x2.x, x0.y + x2.y);
+          const float2 d02 = make_float2(x0.x - x2.x, x0.y - x2.y);
+          const float2 s13 = make_float2(x1.x + x3.x, x1.y + x3.y);
+          const float2 d13 = make_float2(x1.x - x3.x, x1.y - x3.y);
+          const float2 rd13 = inverse ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+          float2* X = h == 0 ? E : O;
+          X[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+          X[1] = make_float2(d02.x + rd13.x, d02.y + rd13.y);
+          X[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+          X[3] = make_float2(d02.x - rd13.x, d02.y - rd13.y);
+        }
+        // W^1 = (1 -+ i) c, W^2 = -+i, W^3 = (-1 -+ i) c, c = sqrt(2)/2
+        const float c = 0.70710678118654752f;
+        const float sg = inverse ? 1.0f : -1.0f;
+        float2 t1 = make_float2((O[1].x - sg * O[1].y) * c, (O[1].y + sg * O[1].x) * c);
+        float2 t2 = make_float2(-sg * O[2].y, sg * O[2].x);
+        float2 t3 = make_float2((-O[3].x - sg * O[3].y) * c, (-O[3].y + sg * O[3].x) * c);
+        const float2 T[4] = {O[0], t1, t2, t3};
+        const int d = ((j >> lNs) << (lNs + 3)) + k;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          b[(d + m * Ns) * ps + l * ls] = make_float2(E[m].x + T[m].x, E[m].y + T[m].y);
+          b[(d + (m + 4) * Ns) * ps + l * ls] = make_float2(E[m].x - T[m].x, E[m].y - T[m].y);
+        }
+      }
+      Ns <<= 3;
+      lNs += 3;
+      rem -= 3;
+    } else if (rem >= 2) {  // radix 4
       const int q = g >> 2, lq = lg - 2;
       const int nb = lines * q;
       for (int t = threadIdx.x; t < nb; t += blockDim.x) {
