@@ -139,6 +139,8 @@ def parse():
     ap.add_argument("--cpu-grids", type=int, default=None,
                     help="grids of the oracle's timed CPU sample (all n particles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=100,
+                    help="steps of the FP64 momentum/energy drift report (0 = off)")
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tuning instead of autotuning first")
@@ -199,17 +201,6 @@ def main():
     vel = torch.from_numpy(vel_h).cuda()
     torch.cuda.synchronize()
 
-    # force error against the FP64 oracle on a grid sample (first cpu_grids
-    # grids, every particle), from the initial state
-    if args.cpu_grids is None:  # about 10 s of single-core oracle work on this host
-        args.cpu_grids = max(1, int(48 * 1_000_000 / n))
-    kk = max(1, min(args.cpu_grids, k))
-    sample_cfg = SolverConfig.from_dict(dict(cfg_d, n_grids=kk))
-    sample = MogSolver(sample_cfg, rot=rot[:kk], off=off[:kk])
-    gpu_avg = sample.forces(pos).cpu().numpy()[:, :3].astype(np.float64)
-    sample.close()
-    gpu_full = solver.forces(pos).cpu().numpy()[:, :3].astype(np.float64)
-
     tuned = None
     if not args.no_autotune:  # outside the timed region, on copies of the state
         from mog_solver_amd.autotune import autotune
@@ -221,6 +212,21 @@ def main():
     if args.tune:
         solver.set_tuning(**parse_tune(args.tune))
         print(f"[bench] tuning set: {solver.get_tuning()}", file=sys.stderr, flush=True)
+
+    # force error against the FP64 oracle, from the initial state, with the
+    # tuning that is timed below (the DFT path is the one choice that changes
+    # rounding): all k grids, and a sample of the first cpu_grids grids
+    # (every particle) for the one-core baseline
+    if args.cpu_grids is None:  # about 10 s of single-core oracle work on this host
+        args.cpu_grids = max(1, int(48 * 1_000_000 / n))
+    kk = max(1, min(args.cpu_grids, k))
+    timed_tuning = solver.get_tuning()
+    sample_cfg = SolverConfig.from_dict(dict(cfg_d, n_grids=kk))
+    sample = MogSolver(sample_cfg, rot=rot[:kk], off=off[:kk])
+    sample.set_tuning(spectral_mfma=timed_tuning["spectral_mfma"])
+    gpu_avg = sample.forces(pos).cpu().numpy()[:, :3].astype(np.float64)
+    sample.close()
+    gpu_full = solver.forces(pos).cpu().numpy()[:, :3].astype(np.float64)
 
     use_graph = not args.no_graph
     solver.run(pos, vel, args.warmup, use_graph=use_graph)
@@ -304,6 +310,28 @@ def main():
     phase_fractions["peaks"] = {"hbm_GBps": HBM_PEAK_GBS, "lds_TBps": round(LDS_PEAK_TBS, 2),
                                 "fp32_vector_TFLOPs": FP32_VECTOR_PEAK_TFS}
 
+    # FP64 diagnostics over --drift-steps steps from the initial state (on a
+    # copy, outside the timed region): momentum and energy drift
+    drift = None
+    if args.drift_steps > 0:
+        p_d = torch.from_numpy(pos_h).cuda()
+        v_d = torch.from_numpy(vel_h).cuda()
+        d0 = solver.diagnostics(p_d, v_d)
+        solver.run(p_d, v_d, args.drift_steps, use_graph=use_graph)
+        d1 = solver.diagnostics(p_d, v_d)
+        mv = float(np.sum(pos_h[:, 3].astype(np.float64) * np.linalg.norm(vel_h[:, :3].astype(np.float64), axis=1)))
+        e0, e1 = d0["kinetic"] + d0["potential"], d1["kinetic"] + d1["potential"]
+        drift = {
+            "steps": args.drift_steps,
+            "kinetic": [d0["kinetic"], d1["kinetic"]],
+            "potential": [d0["potential"], d1["potential"]],
+            "energy_change_rel_potential": float(f"{(e1 - e0) / abs(d0['potential']):.4e}"),
+            "momentum_change_rel_sum_m_v": float(f"{np.linalg.norm(d1['momentum'] - d0['momentum']) / mv:.4e}"),
+            "note": "FP64 (mog_diagnostics); damping removes kinetic energy every step, so E is not conserved",
+        }
+        del p_d, v_d
+        print(f"[bench] drift over {args.drift_steps} steps: {drift}", file=sys.stderr, flush=True)
+
     cpu = None
     force_err = force_err_sample = None
     if not args.no_cpu_baseline:
@@ -363,7 +391,11 @@ def main():
         "force_rel_rms_vs_fp64": None if force_err is None else float(f"{force_err:.3e}"),
         "force_rel_rms_vs_fp64_first_grids": None if force_err_sample is None else float(f"{force_err_sample:.3e}"),
         "force_sample_grids": kk,
+        "force_error_dft_path": "matrix-core fp32 DFT" if (args.precision == "fp32" and timed_tuning["spectral_mfma"]
+                                                             and g >= 16) else
+                                ("vector FFT" if args.precision == "fp32" else f"matrix-core {args.precision} DFT"),
         "tuning": tuning,
+        "diagnostics": drift,
         "autotuned": tuned is not None,
         "phases_ms": {key: round(v, 4) for key, v in phases.items()},
         "roofline": roofline,

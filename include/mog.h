@@ -153,6 +153,21 @@ int mog_init_perlin(float* pos_n4, float* vel_n4, int64_t n, double world_size,
                     double noise_scale, double density_contrast, uint32_t seed,
                     double base_mass, double spawn_buffer, int32_t spawn_shape, void* stream);
 
+/* FP64 diagnostics of a particle state (no counterpart in the reference,
+ * which prints only its configuration, main.py:147-155).  Writes
+ * out8[0..7] (DEVICE pointer, 8 doubles, async on `stream`) =
+ *   {sum m vx, sum m vy, sum m vz, 1/2 sum m |v|^2, U, sum m, 0, 0}
+ * with m = pos.w and U the mean over the k grids of the potential energy
+ * 1/2 sum_c rho(c) phi(c) of the NGP density of `pos`: rho the milli-mass
+ * per cell / 1000 (complex.glsl:23) and phi_hat = -2 pi G rho_hat / k^2
+ * (greens.glsl:40-47, float32 k2 > 1e-4 mask, DC zeroed), the potential
+ * whose gradient the force grids hold (the step accelerates by -grad phi,
+ * update.glsl:83).  U is evaluated as sum_k T(k) |m_hat(k)|^2 by Parseval,
+ * every operation FP64, sums in a fixed order.  Uses the context's spare
+ * density grids (left all zero) and allocates its tables on the first call. */
+int mog_diagnostics(mog_ctx* ctx, const float* pos_n4, const float* vel_n4, int64_t n, double* out8,
+                    void* stream);
+
 /* ---- Learned closure (no counterpart in the reference) ----------------
  * Features of the ensemble of `ctx` (typically few grids, e.g. k = 24) at
  * the particles: runs the context's clear, deposit and spectral solve, a
@@ -205,7 +220,8 @@ int mog_set_tuning(mog_ctx* ctx, const mog_tuning* t);
 /* Device pointers of the context's own density grids (int64) and force
  * grids (float4) as left by the last mog_step; NULL on error.  The density
  * pointer alternates between two buffers from one step to the next, so it
- * is valid until the next step. */
+ * is valid until the next step.  After a step that returned an error their
+ * contents are unspecified until the next successful step. */
 const int64_t* mog_grids_ptr(mog_ctx* ctx);
 const float* mog_force_ptr(mog_ctx* ctx);
 

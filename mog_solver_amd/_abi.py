@@ -39,6 +39,7 @@ EXPORTED = (
     "mog_step_profiled",
     "mog_set_integrator",
     "mog_init_perlin",
+    "mog_diagnostics",
     "mog_closure_features",
     "mog_mlp_param_count",
     "mog_mlp_forward",
@@ -161,6 +162,8 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     lib.mog_init_perlin.argtypes = [vp, vp, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_uint32, ctypes.c_double, ctypes.c_double, i32, vp]
     d = ctypes.c_double
+    lib.mog_diagnostics.restype = ctypes.c_int
+    lib.mog_diagnostics.argtypes = [vp, vp, vp, i64, vp, vp]
     lib.mog_closure_features.restype = ctypes.c_int
     lib.mog_closure_features.argtypes = [vp, vp, i64, d, vp, vp, vp]
     lib.mog_mlp_param_count.restype = ctypes.c_int

@@ -14,8 +14,11 @@ descent over the tuning fields, starting from the library's defaults:
 3. deposit in visiting order (wave-aggregated adds) and the cell cache
 4. matrix-core or vector DFT for FP32
 
-Results do not depend on the tuning (the step is bit-identical across the
-deposit and ordering choices), so tuning is free to pick any candidate.
+Stages 1-3 do not change results: the step is bit-identical across the
+deposit tiling, visiting-order and cell-cache choices.  Stage 4 does: the
+matrix-core FP32 DFT rounds differently from the vector FFTs, so the forces
+it gives differ from the default path's by FP32 rounding (both stay within
+the solve's tolerance against the FP64 oracle, a few 1e-7 relative RMS).
 Winners are cached in a JSON file keyed by device, n, k, g and precision.
 """
 from __future__ import annotations

@@ -1,8 +1,10 @@
 """Build recipe of the HIP library ``libmog.so`` (gfx950).
 
-``build()`` compiles every ``csrc/*.hip`` translation unit with hipcc into
-one shared library inside the package (``mog_solver_amd/lib/libmog.so``),
-so the built file travels with the source tree.  Flags:
+``build()`` compiles every ``csrc/*.hip`` translation unit with hipcc (in
+parallel, one process per unit, objects in a temporary directory) and links
+them into one shared library inside the package
+(``mog_solver_amd/lib/libmog.so``), so the built file travels with the
+source tree.  Flags:
 
 * ``-ffp-contract=off``: the cell-index arithmetic must round every product
   and sum separately to reproduce the reference's float32 sequence bit for
@@ -16,6 +18,8 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -24,14 +28,14 @@ LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libmog.so"
 ARCH = os.environ.get("MOG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "deposit.hip", "spectral.hip", "spectral_mfma.hip", "gather.hip", "order.hip", "init.hip", "closure.hip", "batch.hip", "mog2d.hip"]
+SOURCES = ["api.hip", "deposit.hip", "spectral.hip", "spectral_mfma.hip", "gather.hip", "order.hip", "init.hip",
+           "closure.hip", "batch.hip", "mog2d.hip", "energy.hip", "noalloc.hip"]
 
-HIPCC_FLAGS = [
+COMMON_FLAGS = [
     "-O3",
     "-std=c++17",
     f"--offload-arch={ARCH}",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang",
@@ -41,6 +45,10 @@ HIPCC_FLAGS = [
     "-Wall",
     "-Wno-unused-function",
 ]
+# kept for callers that print or reuse the flag list
+HIPCC_FLAGS = [*COMMON_FLAGS, "-shared"]
+
+_NOISE = "is not a recognized feature for this target"
 
 
 def hipcc() -> str:
@@ -49,21 +57,36 @@ def hipcc() -> str:
     return str(cand) if cand.exists() else "hipcc"
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile the library; raises CalledProcessError on failure."""
-    LIBDIR.mkdir(exist_ok=True)
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, *[str(CSRC / s) for s in SOURCES], "-o", str(tmp)]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
-    noise = "is not a recognized feature for this target"
-    err = "\n".join(l for l in res.stderr.splitlines() if noise not in l)
-    if res.returncode != 0:
-        sys.stderr.write(err + "\n")
-        raise subprocess.CalledProcessError(res.returncode, cmd, res.stdout, res.stderr)
-    if verbose and err.strip():
-        sys.stderr.write(err + "\n")
+    err = "\n".join(l for l in res.stderr.splitlines() if _NOISE not in l)
+    return res, err
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile the library; raises CalledProcessError on failure."""
+    LIBDIR.mkdir(exist_ok=True)
+    sources = [s for s in SOURCES if (CSRC / s).exists()]
+    jobs = jobs or max(1, min(len(sources), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with tempfile.TemporaryDirectory(prefix="mog_build_") as tmpdir:
+        objs = [str(Path(tmpdir) / (Path(s).stem + ".o")) for s in sources]
+        cmds = [[hipcc(), *COMMON_FLAGS, "-c", str(CSRC / s), "-o", o] for s, o in zip(sources, objs)]
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            results = list(ex.map(lambda c: (c, *_run(c, verbose)), cmds))
+        for cmd, res, err in results:
+            if res.returncode != 0:
+                sys.stderr.write(err + "\n")
+                raise subprocess.CalledProcessError(res.returncode, cmd, res.stdout, res.stderr)
+            if verbose and err.strip():
+                sys.stderr.write(err + "\n")
+        tmp = LIB.with_suffix(".so.tmp")
+        link = [hipcc(), f"--offload-arch={ARCH}", "-fPIC", "-shared", *objs, "-o", str(tmp)]
+        res, err = _run(link, verbose)
+        if res.returncode != 0:
+            sys.stderr.write(err + "\n")
+            raise subprocess.CalledProcessError(res.returncode, link, res.stdout, res.stderr)
     os.replace(tmp, LIB)
     return LIB
 

@@ -44,6 +44,14 @@ struct mog_ctx {
   // grids (allocated by the first mog_closure_features call)
   float* d_gtab_dc;
   float4* d_force_dc;
+  // FP64 diagnostics (allocated by the first mog_diagnostics call):
+  // twiddles exp(-2 pi i j / g), the energy table T(k), reduction scratch
+  double2* d_diag_tw;
+  double* d_diag_tab;
+  double* d_diag_scratch;
+  // the spare density grids may hold a partial deposit (a step failed after
+  // its deposit was enqueued): the next step clears them first
+  bool spare_dirty;
   float* d_Wf;  // MFMA DFT tables (g >= 16): forward, inverse
   float* d_Wi;
   DepositTune tune;
@@ -382,6 +390,9 @@ void release(mog_ctx* c) {
   (void)hipFree(c->d_conv);
   (void)hipFree(c->d_gtab_dc);
   (void)hipFree(c->d_force_dc);
+  (void)hipFree(c->d_diag_tw);
+  (void)hipFree(c->d_diag_tab);
+  (void)hipFree(c->d_diag_scratch);
   (void)hipFree(c->d_order);
   (void)hipFree(c->d_idx);
   (void)hipFree(c->d_keys_a);
@@ -557,6 +568,16 @@ void swap_grids(mog_ctx* ctx) {
   long long* t = ctx->d_grids;
   ctx->d_grids = ctx->d_grids_next;
   ctx->d_grids_next = t;
+}
+
+// A step that fails after its deposit was enqueued leaves part of a
+// density in the spare grids (no swap follows); the next step clears them
+// before it deposits into them.
+hipError_t clean_spare(mog_ctx* ctx, hipStream_t s) {
+  if (!ctx->spare_dirty) return hipSuccess;
+  hipError_t e = launch_fill_zero(ctx->d_grids_next, (long long)ctx->P.k * ctx->P.cells, s);
+  if (e == hipSuccess) ctx->spare_dirty = false;
+  return e;
 }
 
 
@@ -870,10 +891,14 @@ int mog_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, void* stream) {
   if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
   if (n == 0) return MOG_OK;
   const int* order = nullptr;
-  hipError_t e = refresh_order(ctx, pos, n, (hipStream_t)stream, &order);
+  hipError_t e = clean_spare(ctx, (hipStream_t)stream);
+  if (e == hipSuccess) e = refresh_order(ctx, pos, n, (hipStream_t)stream, &order);
   if (e == hipSuccess)
     e = enqueue_step(ctx, pos, vel, n, order, ctx->d_grids_next, ctx->d_grids, (hipStream_t)stream);
-  if (e != hipSuccess) return hip_fail(ctx, e, "mog_step");
+  if (e != hipSuccess) {
+    ctx->spare_dirty = true;
+    return hip_fail(ctx, e, "mog_step");
+  }
   swap_grids(ctx);
   return MOG_OK;
 }
@@ -886,14 +911,17 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
   if (nsteps < 0) return fail(ctx, MOG_EINVAL, "nsteps must be >= 0");
   if (n == 0 || nsteps == 0) return MOG_OK;
   hipStream_t s = (hipStream_t)stream;
-  hipError_t e;
+  hipError_t e = clean_spare(ctx, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "mog_run");
   if (!use_graph) {
     for (int i = 0; i < nsteps; ++i) {
       const int* order = nullptr;
       if ((e = refresh_order(ctx, pos, n, s, &order)) != hipSuccess ||
           (e = enqueue_step(ctx, pos, vel, n, order, ctx->d_grids_next, ctx->d_grids, s)) !=
-              hipSuccess)
+              hipSuccess) {
+        ctx->spare_dirty = true;
         return hip_fail(ctx, e, "mog_run");
+      }
       swap_grids(ctx);
     }
     return MOG_OK;
@@ -947,7 +975,10 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
       if ((e = refresh_order(ctx, pos, n, s, &order)) != hipSuccess) return hip_fail(ctx, e, "mog_run");
     }
     hipGraphExec_t gx = ctx->d_grids_next == ctx->g_dep ? ctx->gexec : ctx->gexec_alt;
-    if ((e = hipGraphLaunch(gx, s)) != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+    if ((e = hipGraphLaunch(gx, s)) != hipSuccess) {
+      ctx->spare_dirty = true;
+      return hip_fail(ctx, e, "hipGraphLaunch");
+    }
     swap_grids(ctx);
   }
   return MOG_OK;
@@ -962,7 +993,8 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
   hipStream_t s = (hipStream_t)stream;
   const Params& P = ctx->P;
   const int* order = nullptr;
-  hipError_t eo = refresh_order(ctx, pos, n, s, &order);
+  hipError_t eo = clean_spare(ctx, s);
+  if (eo == hipSuccess) eo = refresh_order(ctx, pos, n, s, &order);
   if (eo != hipSuccess) return hip_fail(ctx, eo, "mog_step_profiled");
   hipEvent_t ev[5] = {};
   hipError_t e = hipSuccess;
@@ -1000,6 +1032,7 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
   for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventElapsedTime(&phase_ms[i], ev[i], ev[i + 1]);
   for (int i = 0; i < 5; ++i)
     if (ev[i]) (void)hipEventDestroy(ev[i]);
+  if (e != hipSuccess) ctx->spare_dirty = true;
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_step_profiled");
 }
 
@@ -1142,6 +1175,78 @@ int mog_closure_features(mog_ctx* ctx, const float* pos, int64_t n, double rho0,
                                 ctx->d_force, ctx->d_force_dc, ctx->d_grids, (float)rho0, order,
                                 (float4*)feat, (float4*)mean, s);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_closure_features");
+}
+
+int mog_diagnostics(mog_ctx* ctx, const float* pos, const float* vel, int64_t n, double* out8,
+                    void* stream) {
+  int rc = check_particles(ctx, pos, vel, n, true);
+  if (rc != MOG_OK) return rc;
+  if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
+  if (!out8 || ((uintptr_t)out8 & 7u) != 0)
+    return fail(ctx, MOG_EINVAL, "out8 must be an 8-byte aligned device pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const Params& P = ctx->P;
+  const int g = P.g;
+  if (!ctx->d_diag_tab) {
+    // FP64 twiddles and T(k) = G_hat(k) / (2 g^3 10^6): the Green's
+    // function of build_tables (same float32 k2 mask) in milli-mass units
+    std::vector<double2> tw(g);
+    for (int j = 0; j < g; ++j) {
+      const double th = 2.0 * M_PI * (double)j / (double)g;
+      tw[j] = make_double2(cos(th), -sin(th));
+    }
+    const double W = ctx->cfg.world_size;
+    auto sh = [g](int i) { return i <= g / 2 ? i : i - g; };
+    const double scale = 1.0 / (2.0 * (double)g * g * g * 1e6);
+    std::vector<double> tab((size_t)g * g * g);
+    for (int iz = 0; iz < g; ++iz)
+      for (int iy = 0; iy < g; ++iy)
+        for (int ix = 0; ix < g; ++ix) {
+          double v = 0.0;
+          if ((ix | iy | iz) != 0 && green_mask32(ix, iy, iz, g, W)) {
+            const double kx = sh(ix) * 2.0 * M_PI / W, ky = sh(iy) * 2.0 * M_PI / W,
+                         kz = sh(iz) * 2.0 * M_PI / W;
+            v = -2.0 * M_PI * ctx->cfg.G / (kx * kx + ky * ky + kz * kz) * scale;
+          }
+          tab[((size_t)iz * g + iy) * g + ix] = v;
+        }
+    if (dalloc(&ctx->d_diag_tw, (size_t)g) != hipSuccess ||
+        dalloc(&ctx->d_diag_tab, tab.size()) != hipSuccess ||
+        dalloc(&ctx->d_diag_scratch, diag_scratch_doubles(P.k, g)) != hipSuccess) {
+      (void)hipFree(ctx->d_diag_tw);
+      (void)hipFree(ctx->d_diag_tab);
+      (void)hipFree(ctx->d_diag_scratch);
+      ctx->d_diag_tw = nullptr;
+      ctx->d_diag_tab = nullptr;
+      ctx->d_diag_scratch = nullptr;
+      return fail(ctx, MOG_ENOMEM, "mog_diagnostics: tables");
+    }
+    hipError_t ec = hipMemcpy(ctx->d_diag_tw, tw.data(), tw.size() * sizeof(double2),
+                              hipMemcpyHostToDevice);
+    if (ec == hipSuccess)
+      ec = hipMemcpy(ctx->d_diag_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (ec != hipSuccess) {
+      (void)hipFree(ctx->d_diag_tab);
+      ctx->d_diag_tab = nullptr;
+      return hip_fail(ctx, ec, "mog_diagnostics");
+    }
+  }
+  // the density of the current positions in the spare grids (all zero
+  // between steps), which are cleared again afterwards
+  const long long cells = (long long)P.k * P.cells;
+  hipError_t e = clean_spare(ctx, s);
+  if (e == hipSuccess)
+    e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
+                       (unsigned long long*)ctx->d_grids_next, ctx->tune, s);
+  if (e == hipSuccess)
+    e = launch_diagnostics((const float4*)pos, (const float4*)vel, n, ctx->d_grids_next, P.k, g,
+                           ctx->d_diag_tw, ctx->d_diag_tab, ctx->d_diag_scratch, out8, s);
+  if (e == hipSuccess) e = launch_fill_zero(ctx->d_grids_next, cells, s);
+  if (e != hipSuccess) {
+    ctx->spare_dirty = true;
+    return hip_fail(ctx, e, "mog_diagnostics");
+  }
+  return MOG_OK;
 }
 
 int mog_mlp_param_count(void) { return (int)mlp_param_count(); }

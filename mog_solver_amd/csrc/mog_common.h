@@ -182,6 +182,14 @@ hipError_t launch_adam(float* prm, const float* grad, float* m1, float* m2, floa
                        float b2, float eps, int step, hipStream_t s);
 hipError_t launch_update(const Params& P, float4* pos, float4* vel, const float4* avg,
                          long long n, hipStream_t s);
+// FP64 diagnostics (energy.hip): out[0..7] = (sum m v (3), 1/2 sum m |v|^2,
+// mean grid potential energy, sum m, 0, 0) of the particles and the k int64
+// density grids; tw[j] = exp(-2 pi i j / g), tab = T(k) = G_hat(k) /
+// (2 g^3 10^6) as (g,g,g) [kz][ky][kx]; scratch: diag_scratch_doubles(k, g).
+size_t diag_scratch_doubles(int k, int g);
+hipError_t launch_diagnostics(const float4* pos, const float4* vel, long long n,
+                              const long long* grids, int k, int g, const double2* tw,
+                              const double* tab, double* scratch, double* out, hipStream_t s);
 
 
 // ---------------------------------------------------------------------------

@@ -233,6 +233,23 @@ class MogSolver:
         force = self.solve(grids, stream=stream)
         return self.gather(pos, force, stream=stream)
 
+    def diagnostics(self, pos, vel, stream=None) -> dict:
+        """FP64 momentum (3,), kinetic energy, grid potential energy (mean
+        over the k grids of 1/2 sum rho phi, the potential whose gradient
+        the force grids hold) and total mass of a state (``mog_diagnostics``;
+        synchronises the stream)."""
+        torch = _torch()
+        _check_particles(pos, "pos")
+        _check_particles(vel, "vel")
+        if pos.shape[0] != vel.shape[0]:
+            raise ValueError("pos and vel must have the same length")
+        out = torch.zeros(8, dtype=torch.float64, device=pos.device)
+        self._check(self.lib.mog_diagnostics(self._ctx, _ptr(pos), _ptr(vel), pos.shape[0], _ptr(out),
+                                             _stream_ptr(stream)))
+        o = out.cpu().numpy()
+        return {"momentum": o[0:3].copy(), "kinetic": float(o[3]), "potential": float(o[4]),
+                "mass": float(o[5])}
+
 
 class Simulation:
     """Headless MOG simulation with the reference's configuration and names.

@@ -585,6 +585,53 @@ def step(pos, vel, rot, off, cfg, compat="reference", fft="numpy", flush=False):
     return p, v, avg
 
 
+# ---------------------------------------------------------------------------
+# FP64 diagnostics (no reference counterpart: the reference prints only its
+# configuration, main.py:147-155)
+# ---------------------------------------------------------------------------
+
+
+def potential_energy_grids(grids, g, W, G):
+    """FP64 potential energy of each grid's density, (k,) array.
+
+    With ``rho = grid / 1000`` (mass per cell, complex.glsl:23) and the
+    potential ``phi`` of the step's spectral Green's function
+    (``phi_hat = G_hat rho_hat``, ``G_hat = -2 pi G / k^2`` where float32
+    ``k2 > 1e-4``, DC zeroed; greens.glsl:40-47), the energy
+    ``U = 1/2 sum_c rho(c) phi(c)`` equals, by Parseval for the
+    unnormalised DFT, ``1/(2 g^3) sum_k G_hat(k) |rho_hat(k)|^2``.  The force
+    grids of ``solve`` are ``grad phi`` of this same ``phi``.
+    """
+    rho = np.asarray(grids).astype(np.float64) / 1000.0
+    rho_hat = np.fft.fftn(rho, axes=(1, 2, 3))
+    gf = greens_factor(g, W, G)
+    return 0.5 / float(g**3) * np.einsum("kzyx,zyx->k", np.abs(rho_hat) ** 2, gf)
+
+
+def diagnostics(pos, vel, rot, off, n_grids, grid_size, world_size, G):
+    """FP64 momentum, kinetic and potential energy of a particle state.
+
+    ``momentum = sum m v`` (3,), ``kinetic = 1/2 sum m |v|^2``,
+    ``potential`` = mean over the k grids of ``potential_energy_grids`` of
+    their NGP density (the energy whose gradient the k-averaged force
+    approximates: the step accelerates by ``-avgForce``, update.glsl:83),
+    ``mass = sum m``.  Masses are the float32 ``pos.w``; the potential uses
+    the deposited milli-mass, like the force.
+    """
+    p = np.asarray(pos, dtype=np.float32)
+    v = np.asarray(vel, dtype=np.float32)[:, :3].astype(np.float64)
+    m = p[:, 3].astype(np.float64)
+    P = Params.make(n_grids, grid_size, world_size)
+    grids = deposit(p, rot, off, P)
+    U = potential_energy_grids(grids, grid_size, world_size, G)
+    return {
+        "momentum": (m[:, None] * v).sum(axis=0),
+        "kinetic": 0.5 * float(np.sum(m * np.sum(v * v, axis=1))),
+        "potential": float(np.mean(U)),
+        "mass": float(m.sum()),
+    }
+
+
 def rel_rms(a, ref):
     """sqrt(sum |a - ref|^2 / sum |ref|^2) over particles."""
     a = np.asarray(a, dtype=np.float64)[:, :3]

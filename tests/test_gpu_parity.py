@@ -93,9 +93,13 @@ def test_deposit_masses_and_large_values(torch_cuda, k, g):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("g,compat", [(16, "reference"), (16, "physical"), (8, "reference"), (4, "reference"),
-                                      (32, "reference"), (64, "physical"), (64, "reference")])
-def test_solve_matches_fp64(torch_cuda, g, compat):
+# FP32 solve against FP64 on both transform paths: the vector FFTs
+# (spectral_mfma 0, the default) and the matrix-core DFTs (spectral_mfma 1,
+# used from g = 16 up when selected, e.g. by the autotuner), with the
+# Green's-function multiply, in both compat modes.
+@pytest.mark.parametrize("g,compat,mfma", [(8, "reference", 0), (4, "reference", 0)] + [
+    (g, compat, mfma) for g in (16, 32, 64) for compat in ("reference", "physical") for mfma in (0, 1)])
+def test_solve_matches_fp64(torch_cuda, g, compat, mfma):
     torch = torch_cuda
     k = 3 if g < 64 else 1
     W, G = 100.0, 100.0
@@ -105,6 +109,8 @@ def test_solve_matches_fp64(torch_cuda, g, compat):
     pos[:, :3] = np.mod(rng.normal(50, 12, (n, 3)), W)
     pos[:, 3] = 1.0
     s, rot, off = make_solver(k, g, W, G, compat=compat)
+    s.set_tuning(spectral_mfma=mfma)
+    assert s.get_tuning()["spectral_mfma"] == mfma
     P = O.Params.make(k, g, W)
     grids = O.deposit(pos, rot, off, P)
     got = s.solve(dev(torch, grids)).cpu().numpy().astype(np.float64)
@@ -310,10 +316,17 @@ def test_tuning_variants_agree(torch_cuda, g):
         results.append((p.cpu().numpy(), vv.cpu().numpy()))
     for (p, vv), v in zip(results[1:], variants[1:]):
         assert np.array_equal(p, results[0][0]) and np.array_equal(vv, results[0][1]), v
-    s.set_tuning(**{**base, "spectral_mfma": 0})
+    # the other DFT path (matrix cores when the default is the vector FFT):
+    # FP32 rounding differences only
+    flipped = 1 - base["spectral_mfma"]
+    s.set_tuning(**{**base, "spectral_mfma": flipped})
+    assert s.get_tuning()["spectral_mfma"] == flipped
     p, vv = dev(torch, pos), dev(torch, vel)
-    s.run(p, vv, 3, use_graph=False)
-    assert O.rel_rms(vv.cpu().numpy()[:, :3], results[0][1][:, :3]) < 1e-5
+    s.run(p, vv, 2, use_graph=True)
+    s.step(p, vv)
+    got_v = vv.cpu().numpy()[:, :3]
+    assert not np.array_equal(got_v, results[0][1][:, :3])  # a different path ran
+    assert O.rel_rms(got_v, results[0][1][:, :3]) < 1e-5
 
 
 def test_autotune_keeps_the_step(torch_cuda):

@@ -403,3 +403,88 @@ def test_convolution_key_semantics_and_agreement_with_fft():
         a, b = conv[16 + d // 2, 16 + d, 16, :3], fft[16 + d // 2, 16 + d, 16, :3]
         assert np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.999
         assert 0.85 < np.linalg.norm(a) / np.linalg.norm(b) < 1.15
+
+
+# ---------------------------------------------------------------------------
+# Kick sign and the FP64 diagnostics (update.glsl:83-86; the step
+# accelerates by -avgForce, avgForce = grad phi)
+# ---------------------------------------------------------------------------
+
+
+def two_clusters(n=8192, sigma=6.0, sep=40.0, seed=0):
+    rng = np.random.default_rng(seed)
+    a = rng.normal([W / 2 - sep / 2, W / 2, W / 2], sigma, (n // 2, 3))
+    b = rng.normal([W / 2 + sep / 2, W / 2, W / 2], sigma, (n // 2, 3))
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = np.mod(np.concatenate([a, b]), W)
+    pos[:, 3] = 1.0
+    return pos, np.zeros_like(pos)
+
+
+def _infall(update=None, steps=20, dt=1e-4):
+    """Two cold clusters under the oracle step (physical compat, no
+    damping): (separation before, after, diagnostics before, after)."""
+    k, g, Gc = 16, 16, 100.0
+    pos, vel = two_clusters()
+    rot, off = O.se3_transforms(k, g, W, 0)
+    cfg = dict(n_grids=k, grid_size=g, world_size=W, G=Gc, dt=dt, damping=1.0)
+    half = pos.shape[0] // 2
+    sep0 = pos[half:, 0].mean() - pos[:half, 0].mean()
+    d0 = O.diagnostics(pos, vel, rot, off, k, g, W, Gc)
+    p, v = pos, vel
+    for _ in range(steps):
+        avg, _ = O.forces(p, rot, off, k, g, W, Gc, compat="physical")
+        p, v = (update or O.update32)(p, v, avg, dt, 1.0, W)
+    d1 = O.diagnostics(p, v, rot, off, k, g, W, Gc)
+    return sep0, p[half:, 0].mean() - p[:half, 0].mean(), d0, d1
+
+
+def test_kick_sign_infall_and_energy_exchange():
+    """Two cold clusters fall towards each other, the kinetic energy they
+    gain is paid by the potential energy (|dE| < 20 % of dKE; NGP grids do
+    not conserve energy exactly), and the momentum stays zero.  A '+=' kick
+    (repulsion) fails every one of these; the planted mistake below shows
+    it."""
+    sep0, sep1, d0, d1 = _infall()
+    assert sep1 < sep0 - 0.05
+    dke = d1["kinetic"] - d0["kinetic"]
+    de = (d1["kinetic"] + d1["potential"]) - (d0["kinetic"] + d0["potential"])
+    assert dke > 0 and abs(de) < 0.2 * dke, (de, dke)
+    vrms = math.sqrt(2 * d1["kinetic"] / d1["mass"])
+    assert np.linalg.norm(d1["momentum"]) < 1e-6 * d1["mass"] * vrms
+
+
+def test_flipped_kick_is_caught():
+    """A planted '+=' kick in a copy of the oracle's integrator: the clusters
+    separate and the energy is not conserved."""
+
+    def update_plus(pos, vel, avg, dt, damping, world_size, timescale=1.0):
+        return O.update32(pos, vel, -np.asarray(avg), dt, damping, world_size, timescale)
+
+    sep0, sep1, d0, d1 = _infall(update=update_plus)
+    dke = d1["kinetic"] - d0["kinetic"]
+    de = (d1["kinetic"] + d1["potential"]) - (d0["kinetic"] + d0["potential"])
+    assert sep1 > sep0
+    assert abs(de) > 0.5 * dke
+
+
+def test_potential_energy_parseval_and_single_mode():
+    """potential_energy_grids against (i) 1/2 sum rho phi with phi from a
+    real-space inverse transform (an independent path through the same
+    Green's function) and (ii) the closed form of one Fourier mode,
+    rho = A cos(2 pi n x / W): U = G_hat(k) A^2 g^3 / 4."""
+    g, Gc = 16, 100.0
+    rng = np.random.default_rng(5)
+    grids = rng.integers(0, 5000, (3, g, g, g)).astype(np.int64)
+    U = O.potential_energy_grids(grids, g, W, Gc)
+    rho = grids.astype(np.float64) / 1000.0
+    phi = np.fft.ifftn(np.fft.fftn(rho, axes=(1, 2, 3)) * O.greens_factor(g, W, Gc)[None], axes=(1, 2, 3)).real
+    direct = 0.5 * np.sum(rho * phi, axis=(1, 2, 3))
+    assert np.allclose(U, direct, rtol=1e-12)
+    assert np.all(U < 0)
+    A, nx = 2.5, 3
+    x = np.arange(g)
+    rho1 = np.broadcast_to(A * np.cos(2 * np.pi * nx * x / g), (g, g, g))
+    U1 = O.potential_energy_grids(1000.0 * rho1[None], g, W, Gc)[0]
+    kx = 2 * np.pi * nx / W
+    assert np.isclose(U1, -2 * np.pi * Gc / kx**2 * A**2 * g**3 / 4, rtol=1e-12)
