@@ -128,6 +128,20 @@ int mog_gather(mog_ctx* ctx, const float* pos_n4, int64_t n, const float* force,
 int mog_update(mog_ctx* ctx, float* pos_n4, float* vel_n4, const float* avg_force_n4, int64_t n,
                void* stream);
 
+/* mog_step without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  mog_step
+ * allocates the gather's Morton visiting order (keys, indices, radix-sort
+ * temporaries) on its first call for a particle count; this variant keeps
+ * it in a device-resident array of the library instead, 20 MiB per GPU
+ * (held until the process ends), shared by every call on that GPU and
+ * keeping the last call's contents.  Limit: n <= 1048576 (2^20) particles,
+ * larger n returns MOG_EINVAL before any launch.  Results are bit-identical
+ * to mog_step (the sorted-deposit tuning is not used here; it does not
+ * change results).  Calls that overlap on one GPU (other threads or
+ * streams, any context) race on that array. */
+int mog_step_noalloc(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, void* stream);
+
 /* mog_step with HIP events recorded between its kernels on `stream`; waits
  * for the stream and writes the durations in milliseconds of the phases
  * {grid clear, deposit, spectral solve, gather+update} to phase_ms[0..3].

@@ -37,6 +37,7 @@ EXPORTED = (
     "mog_gather",
     "mog_update",
     "mog_step_profiled",
+    "mog_step_noalloc",
     "mog_set_integrator",
     "mog_init_perlin",
     "mog_diagnostics",
@@ -154,6 +155,8 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     lib.mog_gather.argtypes = [vp, vp, i64, vp, vp, vp]
     lib.mog_update.restype = ctypes.c_int
     lib.mog_update.argtypes = [vp, vp, vp, vp, i64, vp]
+    lib.mog_step_noalloc.restype = ctypes.c_int
+    lib.mog_step_noalloc.argtypes = [vp, vp, vp, i64, vp]
     lib.mog_step_profiled.restype = ctypes.c_int
     lib.mog_step_profiled.argtypes = [vp, vp, vp, i64, vp, vp]
     lib.mog_set_integrator.restype = ctypes.c_int

@@ -68,7 +68,7 @@ def _run(cmd, verbose):
 def build(verbose: bool = False, jobs: int | None = None) -> Path:
     """Compile the library; raises CalledProcessError on failure."""
     LIBDIR.mkdir(exist_ok=True)
-    sources = [s for s in SOURCES if (CSRC / s).exists()]
+    sources = list(SOURCES)
     jobs = jobs or max(1, min(len(sources), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
     with tempfile.TemporaryDirectory(prefix="mog_build_") as tmpdir:
         objs = [str(Path(tmpdir) / (Path(s).stem + ".o")) for s in sources]

@@ -10,6 +10,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <complex>
 #include <string>
 #include <vector>
@@ -20,6 +22,7 @@
 using namespace mog;
 
 struct mog_ctx {
+  unsigned long long uid;  // distinct for every context of the process
   mog_config cfg;
   Params P;
   int device;
@@ -530,10 +533,10 @@ void drop_graphs(mog_ctx* ctx) {
 // then makes dep the current density and old the zero spare (swap_grids)
 // once the step is launched, not when it is only captured.
 hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const int* order,
-                        long long* dep, long long* old, hipStream_t s) {
+                        long long* dep, long long* old, hipStream_t s, bool allow_sorted = true) {
   const Params& P = ctx->P;
   const long long cells = (long long)P.k * P.cells;
-  if (order && ctx->sorted_deposit && ctx->d_spos && ctx->d_cells) {
+  if (allow_sorted && order && ctx->sorted_deposit && ctx->d_spos && ctx->d_cells) {
     // sorted step: positions in visiting order, deposit with aggregated
     // atomics (and a cell cache), gather from the cache or recomputed cells
     const bool cache = ctx->cell_cache;
@@ -563,6 +566,19 @@ hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const i
   return launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
                               ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s, old, cells);
 }
+
+// Which visiting order the no-allocation scratch of each device holds
+// (mog_step_noalloc): the context, particle array and count it was built
+// for, the steps since, and the index buffer holding it.
+struct NoAllocOrder {
+  unsigned long long owner;
+  const void* pos;
+  long long n;
+  int steps;
+  const int* order;
+};
+std::mutex g_noalloc_mu;
+NoAllocOrder g_noalloc_order[64] = {};
 
 void swap_grids(mog_ctx* ctx) {
   long long* t = ctx->d_grids;
@@ -772,7 +788,9 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
   *out = nullptr;
   int rc = validate(cfg);
   if (rc != MOG_OK) return rc;
+  static std::atomic<unsigned long long> next_uid{1};
   mog_ctx* c = new mog_ctx();
+  c->uid = next_uid.fetch_add(1);
   c->cfg = *cfg;
   c->P = make_params(cfg);
   c->need_x = !(cfg->compat == MOG_COMPAT_REFERENCE && aliases_x_with_z(cfg->grid_size));
@@ -981,6 +999,66 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
     }
     swap_grids(ctx);
   }
+  return MOG_OK;
+}
+
+int mog_step_noalloc(mog_ctx* ctx, float* pos, float* vel, int64_t n, void* stream) {
+  int rc = check_particles(ctx, pos, vel, n, true);
+  if (rc != MOG_OK) return rc;
+  if (!ctx->have_transforms) return fail(ctx, MOG_ENOTRANSFORMS, "mog_set_transforms not called");
+  if (n > kNoAllocMaxParticles)
+    return fail(ctx, MOG_EINVAL,
+                "mog_step_noalloc: n = %lld exceeds its maximum of %lld particles (use mog_step)",
+                (long long)n, kNoAllocMaxParticles);
+  if (n == 0) return MOG_OK;
+  hipStream_t s = (hipStream_t)stream;
+  void* scr = nullptr;
+  hipError_t e = noalloc_scratch(&scr);
+  if (e != hipSuccess) return hip_fail(ctx, e, "mog_step_noalloc: scratch lookup");
+  const int* order = nullptr;
+  if (n >= kOrderMin && ctx->resort_every > 0) {
+    // four n-element arrays of the visiting order, then the sort's
+    // temporaries; the order kept there is reused while it belongs to this
+    // context, particle array and count and is fresh enough
+    const long long cap = kNoAllocMaxParticles;
+    unsigned* keys_a = (unsigned*)scr;
+    unsigned* keys_b = keys_a + cap;
+    int* idx_a = (int*)(keys_b + cap);
+    int* idx_b = idx_a + cap;
+    void* tmp = (void*)(idx_b + cap);
+    size_t need = 0;
+    if ((e = order_temp_bytes_db(n, &need)) != hipSuccess)
+      return hip_fail(ctx, e, "mog_step_noalloc: sort size");
+    if (need > kNoAllocSortBytes)
+      return fail(ctx, MOG_EUNSUPPORTED,
+                  "mog_step_noalloc: the radix sort needs %zu bytes of temporaries, over the %zu "
+                  "the scratch holds",
+                  need, kNoAllocSortBytes);
+    int dev = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return hip_fail(ctx, e, "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoAllocOrder& rec = g_noalloc_order[dev & 63];
+    if (rec.owner != ctx->uid || rec.pos != pos || rec.n != n || !rec.order ||
+        rec.steps >= ctx->resort_every) {
+      rec.owner = 0;
+      if ((e = build_order_db((const float4*)pos, n, ctx->P.W, keys_a, keys_b, idx_a, idx_b, tmp,
+                              kNoAllocSortBytes, &rec.order, s)) != hipSuccess)
+        return hip_fail(ctx, e, "mog_step_noalloc: visiting order");
+      rec.owner = ctx->uid;
+      rec.pos = pos;
+      rec.n = n;
+      rec.steps = 0;
+    }
+    rec.steps++;
+    order = rec.order;
+  }
+  if ((e = clean_spare(ctx, s)) == hipSuccess)
+    e = enqueue_step(ctx, pos, vel, n, order, ctx->d_grids_next, ctx->d_grids, s, false);
+  if (e != hipSuccess) {
+    ctx->spare_dirty = true;
+    return hip_fail(ctx, e, "mog_step_noalloc");
+  }
+  swap_grids(ctx);
   return MOG_OK;
 }
 

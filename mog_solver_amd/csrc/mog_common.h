@@ -162,6 +162,20 @@ hipError_t order_temp_bytes(long long n, size_t* bytes);
 hipError_t build_order(const float4* pos, long long n, float W, unsigned* keys_a,
                        unsigned* keys_b, int* idx_a, int* order, void* tmp, size_t tmp_bytes,
                        hipStream_t s);
+hipError_t order_temp_bytes_db(long long n, size_t* bytes);
+hipError_t build_order_db(const float4* pos, long long n, float W, unsigned* keys_a,
+                          unsigned* keys_b, int* idx_a, int* idx_b, void* tmp, size_t tmp_bytes,
+                          const int** order, hipStream_t s);
+// Device-resident scratch of the no-allocation step (noalloc.hip): a
+// file-scope __device__ array of kNoAllocBytes per GPU, holding the visiting
+// order of up to kNoAllocMaxParticles particles (four n-element 32-bit
+// arrays) followed by at most kNoAllocSortBytes of sort temporaries.
+// noalloc_scratch returns its device address on the current device (looked
+// up once per device with hipGetSymbolAddress).
+constexpr long long kNoAllocMaxParticles = 1ll << 20;
+constexpr size_t kNoAllocSortBytes = (size_t)4 << 20;
+constexpr size_t kNoAllocBytes = (size_t)kNoAllocMaxParticles * 16 + kNoAllocSortBytes;
+hipError_t noalloc_scratch(void** ptr);
 hipError_t launch_init_perlin(float4* pos, float4* vel, long long n, float W, float noise_scale,
                               float contrast, unsigned seed, float base_mass, float spawn_buffer,
                               int sphere, hipStream_t s);

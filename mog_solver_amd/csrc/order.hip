@@ -65,6 +65,37 @@ hipError_t build_order(const float4* pos, long long n, float W, unsigned* keys_a
   return rocprim::radix_sort_pairs(tmp, bytes, keys_a, keys_b, idx_a, order, (unsigned)n, 0, 30, s);
 }
 
+// The same order sorted between two caller-provided buffer pairs
+// (rocPRIM double buffers: the sort ping-pongs between keys_a/keys_b and
+// idx_a/idx_b, so its temporary storage holds only its histograms and
+// partition state).  *order receives whichever index buffer holds the
+// result.
+hipError_t order_temp_bytes_db(long long n, size_t* bytes) {
+  *bytes = 0;
+  rocprim::double_buffer<unsigned> keys(nullptr, nullptr);
+  rocprim::double_buffer<int> vals(nullptr, nullptr);
+  return rocprim::radix_sort_pairs((void*)nullptr, *bytes, keys, vals, (unsigned)n, 0, 30,
+                                   (hipStream_t)0);
+}
+
+hipError_t build_order_db(const float4* pos, long long n, float W, unsigned* keys_a,
+                          unsigned* keys_b, int* idx_a, int* idx_b, void* tmp, size_t tmp_bytes,
+                          const int** order, hipStream_t s) {
+  *order = nullptr;
+  if (n <= 0) return hipSuccess;
+  const long long blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(morton_keys_kernel, dim3((unsigned)blocks), dim3(256), 0, s, pos, n,
+                     1.0f / W, keys_a, idx_a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  rocprim::double_buffer<unsigned> keys(keys_a, keys_b);
+  rocprim::double_buffer<int> vals(idx_a, idx_b);
+  size_t bytes = tmp_bytes;
+  e = rocprim::radix_sort_pairs(tmp, bytes, keys, vals, (unsigned)n, 0, 30, s);
+  if (e == hipSuccess) *order = vals.current();
+  return e;
+}
+
 // Batch keys: (sim << 30) | Morton key in the simulation's own box, over
 // the gather blocks (simulation uniform per workgroup).
 __global__ __launch_bounds__(kBatchGatherBlock) void batch_morton_keys_kernel(

@@ -164,6 +164,15 @@ class MogSolver:
             raise ValueError("pos and vel must have the same length")
         self._check(self.lib.mog_step(self._ctx, _ptr(pos), _ptr(vel), pos.shape[0], _stream_ptr(stream)))
 
+    def step_noalloc(self, pos, vel, stream=None):
+        """One step that allocates no device memory (``mog_step_noalloc``:
+        n <= 2^20, bit-identical to ``step``)."""
+        _check_particles(pos, "pos")
+        _check_particles(vel, "vel")
+        if pos.shape[0] != vel.shape[0]:
+            raise ValueError("pos and vel must have the same length")
+        self._check(self.lib.mog_step_noalloc(self._ctx, _ptr(pos), _ptr(vel), pos.shape[0], _stream_ptr(stream)))
+
     def run(self, pos, vel, nsteps, use_graph=True, stream=None):
         _check_particles(pos, "pos")
         _check_particles(vel, "vel")
