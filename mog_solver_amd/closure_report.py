@@ -10,10 +10,17 @@
    the closure force is compared with the k_full force at the same positions
    (relative RMS); every ``--direct-every`` steps the k_full, k_small and
    closure forces are compared with FP64 direct summation (softening one
-   cell) at a ``--subsample`` of particles.
+   cell) at a ``--subsample`` of particles, and with the estimator floor:
+   a second, independent k_full ensemble (other rotations and offsets) and
+   a ``--k-ref``-grid ensemble standing in for the k -> infinity limit.  A
+   k_full ensemble is a random draw, so no predictor can be closer to one
+   draw than the draw's own scatter about the ensemble mean.
 4. Autotune both solvers' tilings (unless ``--no-autotune``), then time a
-   plain k_full step (hipGraph replay) and a closure step (features, MLP,
-   update), both with inputs resident on the device.
+   plain k_full step (hipGraph replay), a closure step (features, MLP,
+   update) and the ``ClosureSolver`` as shipped (default audit every 10
+   steps against k_full and FP64 direct summation, default threshold and
+   fallback), reporting the share of its steps that used the closure; all
+   with inputs resident on the device.
 
 Prints one JSON object.
 """
@@ -49,6 +56,11 @@ def main(argv=None):
     ap.add_argument("--subsample", type=int, default=4096)
     ap.add_argument("--time-steps", type=int, default=100)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--k-ref", type=int, default=2400,
+                    help="grids of the reference ensemble of the floor measurement (0 = off)")
+    ap.add_argument("--target-k", type=int, default=0,
+                    help="train on the forces of a k-grid ensemble (independent draw) instead of k_full's")
+    ap.add_argument("--compat", default="reference", choices=["reference", "physical"])
     ap.add_argument("--no-autotune", action="store_true",
                     help="time both solvers with the library's default tuning")
     args = ap.parse_args(argv)
@@ -67,7 +79,7 @@ def main(argv=None):
 
     def make(k, seed):
         cfg = SolverConfig.from_dict(dict(n_grids=k, grid_size=args.g, world_size=W, G=G, dt=dt, damping=damping,
-                                          solver="fft"))
+                                          solver="fft", compat=args.compat))
         off, rot = compute_transforms(k, args.g, W, seed=seed)
         return MogSolver(cfg, rot=rot, off=off)
 
@@ -82,10 +94,14 @@ def main(argv=None):
     v = torch.from_numpy(vel).cuda()
     rho0 = CL.default_rho0(n, 1.0, args.g)
     out = dict(workload=f"two_disks n={n}", k_full=args.k_full, k_small=args.k_small, g=args.g,
-               snapshots=args.snapshots, every=args.every, epochs=args.epochs)
+               snapshots=args.snapshots, every=args.every, epochs=args.epochs, compat=args.compat,
+               target_k=args.target_k or args.k_full)
 
     t0 = time.perf_counter()
-    snaps = CL.make_snapshots(full, small, p, v, args.snapshots, every=args.every, rho0=rho0)
+    target = make(args.target_k, args.seed + 5) if args.target_k else None
+    snaps = CL.make_snapshots(full, small, p, v, args.snapshots, every=args.every, rho0=rho0, target=target)
+    if target is not None:
+        target.close()
     torch.cuda.synchronize()
     out["snapshot_s"] = time.perf_counter() - t0
     log(f"{len(snaps)} snapshots in {out['snapshot_s']:.1f}s")
@@ -110,6 +126,11 @@ def main(argv=None):
     sub_t = torch.from_numpy(sub.astype(np.int64)).cuda()
     vs_full = {"closure": [], "k_small_ngp": [], "k_small_cic": []}
     direct = {"k_full": [], "k_small_ngp": [], "k_small_cic": [], "closure": []}
+    full_b = make(args.k_full, args.seed + 3)
+    kref = make(args.k_ref, args.seed + 4) if args.k_ref else None
+    floor = {"k_full_vs_independent_k_full": []}
+    if kref is not None:
+        floor.update({f"{name}_vs_k_ref": [] for name in ("k_full", "closure", "k_small_cic", "k_small_ngp")})
     for step in range(args.eval_steps):
         feat, mean, pred = cs._buffers(p)
         CL.closure_features(small, p, rho0, feat=feat, mean=mean)
@@ -123,11 +144,23 @@ def main(argv=None):
             nd = CL.direct_sum(p, sub, W, G_eff, W / args.g).cpu().numpy()
             for name, t in (("k_full", ref), ("k_small_ngp", ngp), ("k_small_cic", mean), ("closure", pred)):
                 direct[name].append(_rel(t[sub_t, :3].double().cpu().numpy(), nd))
+            floor["k_full_vs_independent_k_full"].append(CL.rel_rms(ref, full_b.forces(p)))
+            if kref is not None:
+                fr = kref.forces(p)
+                for name, t in (("k_full", ref), ("closure", pred), ("k_small_cic", mean), ("k_small_ngp", ngp)):
+                    floor[f"{name}_vs_k_ref"].append(CL.rel_rms(t, fr))
         small.update(p, v, pred)
     out["eval_steps"] = args.eval_steps
     out["vs_k_full_rel_rms"] = {k2: dict(mean=float(np.mean(v2)), max=float(np.max(v2))) for k2, v2 in vs_full.items()}
     out["vs_fp64_direct_sum_rel_rms"] = {k2: float(np.mean(v2)) for k2, v2 in direct.items()}
     out["direct_sum"] = dict(subsample=int(sub.size), softening=W / args.g, G_eff=G_eff)
+    out["floor_rel_rms"] = {k2: float(np.mean(v2)) for k2, v2 in floor.items() if v2}
+    out["floor"] = dict(k_ref=args.k_ref, note="k_full vs an independent k_full draw; every ensemble and the "
+                        "closure vs the k_ref-grid ensemble (the k -> infinity stand-in)")
+    full_b.close()
+    if kref is not None:
+        kref.close()
+    log(f"floor: {out['floor_rel_rms']}")
     log(f"eval vs k_full: {out['vs_k_full_rel_rms']}")
 
     # timing, inputs resident on the device
@@ -160,15 +193,32 @@ def main(argv=None):
             runner.step(p, v)
 
     t_closure = clock(closure_steps, K)
-    audited = CL.ClosureSolver(full, small, mlp, rho0, audit_every=10, threshold=math.inf)
+    audited = CL.ClosureSolver(full, small, mlp, rho0, audit_every=10, threshold=math.inf, direct_subsample=0)
 
     def audited_steps(s):
         for _ in range(s):
             audited.step(p, v)
 
     t_audit = clock(audited_steps, K)
-    out["ms_per_step"] = dict(k_full=t_full * 1e3, closure=t_closure * 1e3, closure_audit_every_10=t_audit * 1e3)
-    out["speedup_vs_k_full"] = dict(closure=t_full / t_closure, closure_audit_every_10=t_full / t_audit)
+    shipped = CL.ClosureSolver(full, small, mlp, rho0)  # defaults: audits, threshold, fallback, direct sum
+
+    def shipped_steps(s):
+        for _ in range(s):
+            shipped.step(p, v)
+
+    t_shipped = clock(shipped_steps, K)
+    out["ms_per_step"] = dict(k_full=t_full * 1e3, closure=t_closure * 1e3, closure_audit_every_10=t_audit * 1e3,
+                              closure_solver_defaults=t_shipped * 1e3)
+    out["speedup_vs_k_full"] = dict(closure=t_full / t_closure, closure_audit_every_10=t_full / t_audit,
+                                    closure_solver_defaults=t_full / t_shipped)
+    out["closure_solver_defaults"] = dict(
+        threshold=shipped.threshold, audit_every=shipped.audit_every, fallback_steps=shipped.fallback_steps,
+        direct_subsample=shipped.direct_subsample, direct_ratio=shipped.direct_ratio,
+        closure_step_share=shipped.closure_steps / max(shipped.steps, 1),
+        audits=[dict(step=a.step, vs_full=round(a.error_vs_full, 4),
+                     vs_direct=None if a.error_vs_direct is None else round(a.error_vs_direct, 4),
+                     full_vs_direct=None if a.full_vs_direct is None else round(a.full_vs_direct, 4),
+                     fallback=a.fallback, reason=a.reason) for a in shipped.audits])
     print(json.dumps(out))
 
 

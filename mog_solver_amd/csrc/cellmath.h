@@ -9,7 +9,13 @@
 //
 // Fast path: w_m = l_m / h - 1/2 evaluated as three FMAs with per-grid
 // coefficients F[4m+j] = RN(R[4j+m] / h), F[4m+3] = RN((W/2 - off_m)/h - 1/2);
-// the cell is floor(v) mod g = rint(w) mod g.  Both paths differ from the real-number value v* by at
+// the cell is floor(v) mod g = rint(w) mod g.  rint is taken by adding and
+// subtracting 1.5 * 2^23: for |w| < 2^22 the sum t = w + 1.5 * 2^23 lies in
+// [2^23, 2^24), where float32 spacing is 1, so t is w rounded to the
+// nearest integer (ties to even, like rint), and the low bits of t's
+// encoding are that integer modulo 2^23 (two's complement), from which the
+// cell's coordinate is masked without a float-to-integer conversion; the
+// face distance uses rint(w) itself.  Admitted particles give |w| < 8 g.  Both paths differ from the real-number value v* by at
 // most a bound E (derived in fast_margin below); when v lies farther than
 // M >= E_exact + E_fast from every integer, floor(v) mod g equals the exact
 // path's cell, so the fast result is used; otherwise the exact sequence runs.
@@ -18,6 +24,7 @@
 #pragma once
 
 #include <math.h>
+#include <string.h>
 
 #if defined(__HIPCC__)
 #define MOG_HD __host__ __device__ __forceinline__
@@ -66,35 +73,44 @@ MOG_HD int cell_exact(float c0, float c1, float c2, const float* R, const float*
   return (((z << lg) | y) << lg) | x;
 }
 
-// Fast cell; returns 0 when the exact path must decide.  F is the 12-float
-// fast row of the grid (make_fast_row), lim = 0.5 - M.  With w = v - 0.5
-// (the -0.5 is folded into F[4m+3]) and r = rint(w): floor(v) = r and the
-// distance of v to the nearest integer is 0.5 - |w - r|.  Flags combine with
-// bitwise & so that no branch is taken per axis.
-MOG_HD int cell_fast_a(float c0, float c1, float c2, const float* F, float a0, float a1, float a2,
-                       float lim, int lg, int* cell);
-
-MOG_HD int cell_fast(float c0, float c1, float c2, const float* F, float lim, int lg, int* cell) {
-  return cell_fast_a(c0, c1, c2, F, F[3], F[7], F[11], lim, lg, cell);
-}
-
-// cell_fast with the row's three addends F[3], F[7], F[11] passed as values
-// (a kernel can then keep them in vector registers).
-MOG_HD int cell_fast_a(float c0, float c1, float c2, const float* F, float a0, float a1, float a2,
-                       float lim, int lg, int* cell) {
+// Fast cell and its face distance: returns d = max_m |w_m - rint(w_m)|
+// (the distance of v to the nearest integer is 0.5 - d on the worst axis)
+// and writes the cell floor(v) mod g.  F is the 12-float fast row of the
+// grid (make_fast_row) with its three addends F[3], F[7], F[11] passed as
+// values (a kernel can then keep them in vector registers).  The cell is in
+// range for every input; d is meaningful for admitted particles only.
+MOG_HD float cell_fast_d(float c0, float c1, float c2, const float* F, float a0, float a1, float a2,
+                         int lg, int* cell) {
   const float w0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, a0)));
   const float w1 = fmaf(F[6], c2, fmaf(F[5], c1, fmaf(F[4], c0, a1)));
   const float w2 = fmaf(F[10], c2, fmaf(F[9], c1, fmaf(F[8], c0, a2)));
+  const float M = 12582912.0f;  // 1.5 * 2^23
+  const float t0 = w0 + M, t1 = w1 + M, t2 = w2 + M;
+  // rint(w) == t - M for |w| < 2^22; rintf keeps the two results independent
   const float r0 = rintf(w0), r1 = rintf(w1), r2 = rintf(w2);
-  // one max3 and one compare (callers admit only finite coordinates, so no
-  // NaN reaches fmaxf)
-  const int ok = (int)(fmaxf(fmaxf(fabsf(w0 - r0), fabsf(w1 - r1)), fabsf(w2 - r2)) < lim);
-  const int gm = (1 << lg) - 1;
-  int c = (int)r2;
-  c = (c << lg) | ((int)r1 & gm);
-  c = (c << lg) | ((int)r0 & gm);
-  *cell = c & ((1 << (3 * lg)) - 1);
-  return ok;
+  unsigned b0, b1, b2;
+  memcpy(&b0, &t0, 4);
+  memcpy(&b1, &t1, 4);
+  memcpy(&b2, &t2, 4);
+  // bit-field inserts: x from b0, y from b1 << lg, z from b2 << 2 lg
+  const unsigned gm = (1u << lg) - 1u, gm2 = (1u << (2 * lg)) - 1u;
+  unsigned c = (b0 & gm) | ((b1 << lg) & ~gm);
+  c = (c & gm2) | ((b2 << (2 * lg)) & ~gm2);
+  *cell = (int)(c & ((1u << (3 * lg)) - 1u));
+  return fmaxf(fmaxf(fabsf(w0 - r0), fabsf(w1 - r1)), fabsf(w2 - r2));
+}
+
+// Fast cell; returns 0 when the exact path must decide (lim = 0.5 - M).
+// With w = v - 0.5 (the -0.5 is folded into F[4m+3]) and r = rint(w):
+// floor(v) = r.  Callers admit only finite coordinates, so no NaN reaches
+// the comparison.
+MOG_HD int cell_fast_a(float c0, float c1, float c2, const float* F, float a0, float a1, float a2,
+                       float lim, int lg, int* cell) {
+  return (int)(cell_fast_d(c0, c1, c2, F, a0, a1, a2, lg, cell) < lim);
+}
+
+MOG_HD int cell_fast(float c0, float c1, float c2, const float* F, float lim, int lg, int* cell) {
+  return cell_fast_a(c0, c1, c2, F, F[3], F[7], F[11], lim, lg, cell);
 }
 
 // The 12-float fast row of one grid: F[4m+j] = RN(R[4j+m] / h),

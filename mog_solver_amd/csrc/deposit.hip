@@ -14,6 +14,8 @@
 // 32-bit LDS partials are exact while a chunk's sum stays below 2^32: a
 // particle whose milli-mass exceeds (2^32-1)/chunk goes straight to the
 // 64-bit global grid instead, so no LDS cell can overflow.
+#include <type_traits>
+
 #include "mog_common.h"
 
 namespace mog {
@@ -32,39 +34,45 @@ hipError_t launch_fill_zero(long long* p, long long count, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Deferred (particle, grid) pairs per workgroup: pairs that the fast cell
-// test cannot decide (near a face, or a particle outside the fast bound).
-constexpr int kDeferCap = 512;
-
-// Number of set bits of `mask` below this lane.
-__device__ __forceinline__ unsigned int lanes_below(unsigned long long mask) {
-  return __builtin_amdgcn_mbcnt_hi((unsigned int)(mask >> 32),
-                                   __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
-}
+// Deferred-pair records per lane of the LDS deposit (see deposit_lds_kernel).
+constexpr int kLaneSlots = 4;
+// UP particles per lane and register set in the LDS deposit.
+constexpr int kDepositUP = 2;
 
 // LDS-privatised deposit; grid (chunks, ceil(k / G)).  Dynamic LDS holds
-// G * cells uint32 counters.  The G fast rows are wave-uniform and live in
-// scalar registers for the whole chunk.
+// G * cells uint32 counters followed by kLaneSlots records per lane.  The G
+// fast rows are wave-uniform and live in scalar registers for the whole
+// chunk.
 //
-// Every lane adds `decided ? milli : 0` to its fast cell (always in range);
-// the pairs the fast test cannot decide (about 1e-3 of them) are appended to
-// an LDS queue, and after the particle loop whole waves run the exact
-// float32 sequence for the queued pairs.  The loop itself then makes no
-// function call, so the next iteration's particle loads stay in flight
-// across it (a call would wait for them), and no wave runs the exact
-// sequence for a single lane.  A full queue falls back to deciding in place.
-// Particles whose milli-mass does not fit the 32-bit LDS partials take the
-// 64-bit global path (rare: huge masses).
+// Every (particle, grid) pair adds the particle's 32-bit milli-mass at its
+// fast cell (cellmath.h), unconditionally: no per-pair compare or select.
+// A lane whose UP x G pairs of one iteration include one the fast path
+// cannot decide (a face within the error margin, a particle outside the
+// admitted bound or non-finite, a milli-mass that does not fit the 32-bit
+// partials) appends one record, the offset of its first particle, to its
+// own LDS slots; after the loop each lane repairs its recorded pairs: the
+// exact float32 sequence gives the cell, and if it differs the mass moves
+// (added at the exact cell, subtracted at the fast one: 32-bit modular
+// arithmetic, exact since the chunk's final sums fit), and huge masses are
+// taken out of the partials and added to the 64-bit grids in HBM.  A lane
+// with more records than slots repairs on the spot.  The counters are exact
+// while a chunk's sum stays below 2^32: a particle whose milli-mass exceeds
+// (2^32-1)/chunk takes the 64-bit path.
+//
+// Two loops feed the same per-particle work.  Workgroups whose chunk is a
+// whole number of (2 x UP x threads)-particle spans and whose group holds G
+// live grids (all but the last chunk of a launch sized by launch_deposit)
+// run a double-buffered loop: two register sets of UP particles each, one
+// processed while the other loads, with unguarded loads at 32-bit byte
+// offsets from the chunk's base.  The rest run the guarded loop.
 template <int G>
 __global__ __launch_bounds__(1024) void deposit_lds_kernel(
     const float4* __restrict__ pos, long long n, long long chunk,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
     Params P, unsigned long long lds_limit, unsigned long long* __restrict__ grids) {
   extern __shared__ unsigned int hist[];
-  __shared__ unsigned int defer_p[kDeferCap];  // particle offset within the chunk
-  __shared__ unsigned char defer_g[kDeferCap];  // grid within the group
-  __shared__ unsigned int defer_n;
   const int cells = P.cells;
+  unsigned int* slots = hist + G * cells;  // slot r of lane t: slots[r * blockDim.x + t]
   // tiles in chunk-major order: the groups of one chunk share an XCD's L2
   const int groups = (P.k + G - 1) / G;
   const unsigned int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -73,7 +81,6 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
   const int ng = min(G, P.k - grid0);
   const int total = ng * cells;
   for (int i = threadIdx.x; i < total; i += blockDim.x) hist[i] = 0u;
-  if (threadIdx.x == 0) defer_n = 0u;
 
   float F[G][12];
 #pragma unroll
@@ -90,121 +97,167 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
   const long long p1 = min(n, p0 + chunk);
   const unsigned int lds_limit32 =
       (unsigned int)(lds_limit < 0xFFFFFFFFull ? lds_limit : 0xFFFFFFFFull);
-  // UP particles per thread per iteration: their loads are issued together.
-  constexpr int UP = 2;
-  const long long step = (long long)UP * blockDim.x;
-  // the next iteration's particles are loaded while this one computes
-  float4 qn[UP];
-#pragma unroll
-  for (int u = 0; u < UP; ++u) {
-    const long long p = p0 + threadIdx.x + (long long)u * blockDim.x;
-    qn[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);  // mass 0 is skipped
-  }
-  for (long long pb = p0 + threadIdx.x; pb < p1; pb += step) {
-    float4 q[UP];
+  constexpr int UP = kDepositUP;
+
+  // The 32-bit milli-mass the loop adds for a particle: uint(m * 1000) when
+  // the product is in [1, 2^32), clamped below 2^32 above, 0 below 1 and for
+  // NaN; `small` when it is the true milli-mass and fits the partials.
+  // (lds_limit32 <= (2^32 - 1) / 256, below the clamp 4294967040, so a
+  // clamped value is never small; fmaxf takes NaN to 0)
+  auto milli32 = [&](float w, bool& small) -> unsigned int {
+    const float mm = w * 1000.0f;
+    const unsigned int m = (unsigned int)fminf(fmaxf(mm, 0.0f), 4294967040.0f);
+    small = m <= lds_limit32;
+    return m;
+  };
+
+  // Repair the pairs of the UP particles of one record (first particle
+  // pu0): exact cell where the fast one was not decided, 64-bit path for
+  // huge masses.  Runs per lane.
+  auto repair = [&](long long pu0) {
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
-      q[u] = qn[u];
-      const long long p = pb + step + (long long)u * blockDim.x;
-      qn[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const long long p = pu0 + (long long)u * blockDim.x;
+      if (p >= p1) continue;
+      const float4 q = pos[p];
+      bool small;
+      const unsigned int m32 = milli32(q.w, small);
+      const float c0 = q.x - P.half, c1 = q.y - P.half, c2 = q.z - P.half;
+      const int pf = particle_fast(c0, c1, c2, P.CB);
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi) {
+        if (gi >= ng) continue;
+        const int gr = grid0 + gi;
+        int fc;
+        const float d = cell_fast_d(c0, c1, c2, F[gi], F[gi][3], F[gi][7], F[gi][11], P.lg, &fc);
+        const bool ok = pf && d < P.lim;
+        if (!small) {  // huge mass: out of the partials, into the 64-bit grids
+          if (m32) atomicAdd(&hist[gi * cells + fc], 0u - m32);
+          const unsigned long long milli = milli_of(q.w);
+          if (milli) {
+            const int ex = ok ? fc : cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W,
+                                                     P.half, P.h, P.lg);
+            atomicAdd(&grids[(long long)gr * cells + ex], milli);
+          }
+        } else if (!ok && m32) {
+          const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half, P.h,
+                                         P.lg);
+          if (ex != fc) {
+            atomicAdd(&hist[gi * cells + ex], m32);
+            atomicAdd(&hist[gi * cells + fc], 0u - m32);
+          }
+        }
+      }
     }
-    // all UP x G cells first, with no branch between them (independent
-    // work the scheduler can interleave), then the atomics, then one
-    // wave-uniform test for the rare paths
-    unsigned int m32[UP];
-    bool small[UP], big[UP];
+  };
+
+  // The UP x G pairs of UP particles: their milli-masses and fast cells,
+  // every cell computed before any is used (no branch between them); true
+  // when every pair is decided.
+  auto cells_of = [&](const float4* q, unsigned int (&m32)[UP], int (&cell)[UP][G]) -> bool {
     float cc[UP][3];
-    int pf[UP];
-    int cell[UP][G];
-    bool ok[UP][G];
+    bool sure = true;  // flags combine with bitwise &: no short-circuit branches
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
-      // uint(m * 1000) (milli_of): 0 exactly when the float product is below
-      // 1 (or negative, or NaN); while it is below 2^32 a 32-bit truncation
-      // gives it, which spares the 64-bit conversion for every real mass
-      const float mm = q[u].w * 1000.0f;
-      const bool live = mm >= 1.0f;
-      m32[u] = live && mm < 4294967296.0f ? (unsigned int)mm : 0u;
-      small[u] = live && mm < 4294967296.0f && m32[u] <= lds_limit32;
-      big[u] = live && !small[u];
+      bool small;
+      m32[u] = milli32(q[u].w, small);
       cc[u][0] = q[u].x - P.half;
       cc[u][1] = q[u].y - P.half;
       cc[u][2] = q[u].z - P.half;
-      pf[u] = particle_fast(cc[u][0], cc[u][1], cc[u][2], P.CB);
+      sure = sure & small & (fabsf(cc[u][0]) <= P.CB) & (fabsf(cc[u][1]) <= P.CB) &
+             (fabsf(cc[u][2]) <= P.CB);
+    }
+    float dmax = 0.0f;
+#pragma unroll
+    for (int u = 0; u < UP; ++u)
 #pragma unroll
       for (int gi = 0; gi < G; ++gi)
-        ok[u][gi] = cell_fast_a(cc[u][0], cc[u][1], cc[u][2], F[gi], F[gi][3], F[gi][7],
-                                F[gi][11], P.lim, P.lg, &cell[u][gi]) & pf[u];
+        dmax = fmaxf(dmax, cell_fast_d(cc[u][0], cc[u][1], cc[u][2], F[gi], F[gi][3], F[gi][7],
+                                       F[gi][11], P.lg, &cell[u][gi]));
+    return sure & (dmax < P.lim);
+  };
+
+  // A lane's iterations start at chunk offsets threadIdx.x + j * UP *
+  // threads in both loops.  Records: the first kLaneSlots undecided
+  // iterations; from the next one on (ovf) the lane rescans after the loop.
+  int nrec = 0;
+  long long ovf = -1;
+  auto pairs = [&](const float4* q, long long pl) {
+    unsigned int m32[UP];
+    int cell[UP][G];
+    const bool sure = cells_of(q, m32, cell);
+    // grids gi >= ng (last group) add to allocated, unflushed counters
+#pragma unroll
+    for (int u = 0; u < UP; ++u)
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi) atomicAdd(&hist[gi * cells + cell[u][gi]], m32[u]);
+    if (!sure) {  // rare
+      if (nrec < kLaneSlots)
+        slots[nrec * blockDim.x + threadIdx.x] = (unsigned int)pl;
+      else if (nrec == kLaneSlots)
+        ovf = pl;
+      ++nrec;
     }
-    // grids gi >= ng (last group) add 0 to allocated, unflushed counters
-    bool rare = false;
+  };
+
+  const long long span = 2ll * UP * blockDim.x;
+  const bool fast = ng == G && p1 - p0 == chunk && chunk % span == 0 && chunk < (1ll << 27);
+  if (fast) {
+    const char* base = reinterpret_cast<const char*>(pos + p0);
+    const unsigned int lane_step = 16u * blockDim.x;  // bytes between a lane's particles
+    auto load = [&](float4* q, unsigned int boff) {
+#pragma unroll
+      for (int u = 0; u < UP; ++u)
+        q[u] = *reinterpret_cast<const float4*>(base + boff + (unsigned int)u * lane_step);
+    };
+    const int iters = (int)(chunk / span);
+    unsigned int boff = 16u * threadIdx.x;
+    float4 a[UP], b[UP];
+    load(a, boff);
+    for (int it = 0; it < iters; ++it) {
+      const unsigned int boff_b = boff + UP * lane_step;
+      load(b, boff_b);
+      pairs(a, boff >> 4);
+      boff = boff_b + UP * lane_step;
+      if (it + 1 < iters) load(a, boff);
+      pairs(b, boff_b >> 4);
+    }
+  } else {
+    const long long step = (long long)UP * blockDim.x;
+    // the next iteration's particles are loaded while this one computes;
+    // rows past the chunk read as mass 0 (they add 0 and are never repaired)
+    float4 qn[UP];
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
-#pragma unroll
-      for (int gi = 0; gi < G; ++gi) {
-        const bool take = gi < ng && small[u];
-        // the addend is hidden from the optimiser: a known-zero add would be
-        // turned into an atomic load and a wait
-        unsigned int v = take && ok[u][gi] ? m32[u] : 0u;
-        asm volatile("" : "+v"(v));
-        atomicAdd(&hist[gi * cells + cell[u][gi]], v);
-        rare |= take && !ok[u][gi];
-      }
-      rare |= big[u];
+      const long long p = p0 + threadIdx.x + (long long)u * blockDim.x;
+      qn[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    if (__ballot(rare)) {  // wave-uniform, rare
+    for (long long pb = p0 + threadIdx.x; pb < p1; pb += step) {
+      float4 q[UP];
 #pragma unroll
       for (int u = 0; u < UP; ++u) {
-        const float c0 = cc[u][0], c1 = cc[u][1], c2 = cc[u][2];
-        const long long p = pb + (long long)u * blockDim.x;
-        if (big[u]) {  // huge mass: the 64-bit global grids
-          const unsigned long long milli = milli_of(q[u].w);
-#pragma unroll
-          for (int gi = 0; gi < G; ++gi) {
-            if (gi < ng) {
-              const int gr = grid0 + gi;
-              const int c = cell_index(c0, c1, c2, pf[u], F[gi], rot + 12 * gr, off + 4 * gr, P);
-              atomicAdd(&grids[(long long)gr * cells + c], milli);
-            }
-          }
-        }
-#pragma unroll
-        for (int gi = 0; gi < G; ++gi) {
-          const bool defer = gi < ng && small[u] && !ok[u][gi];
-          const unsigned long long need = __ballot(defer);
-          if (need) {  // queue the undecided pairs: one reservation per wave
-            const int lane = threadIdx.x & 63;
-            const int leader = __ffsll((long long)need) - 1;
-            unsigned int base = 0u;
-            if (lane == leader) base = atomicAdd(&defer_n, (unsigned int)__popcll(need));
-            base = __builtin_amdgcn_readlane(base, leader);
-            if (defer) {
-              const unsigned int slot = base + lanes_below(need);
-              if (slot < (unsigned int)kDeferCap) {
-                defer_p[slot] = (unsigned int)(p - p0);
-                defer_g[slot] = (unsigned char)gi;
-              } else {
-                const int gr = grid0 + gi;
-                const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W,
-                                               P.half, P.h, P.lg);
-                atomicAdd(&hist[gi * cells + ex], m32[u]);
-              }
-            }
-          }
-        }
+        q[u] = qn[u];
+        const long long p = pb + step + (long long)u * blockDim.x;
+        qn[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
+      pairs(q, pb - p0);
     }
   }
-  __syncthreads();
-  // the deferred pairs, one per lane
-  const unsigned int nd = min(defer_n, (unsigned int)kDeferCap);
-  for (unsigned int d = threadIdx.x; d < nd; d += blockDim.x) {
-    const int gi = defer_g[d];
-    const int gr = grid0 + gi;
-    const float4 q = pos[p0 + defer_p[d]];
-    const int cell = cell_exact_call(q.x - P.half, q.y - P.half, q.z - P.half, rot + 12 * gr,
-                                     off + 4 * gr, P.W, P.half, P.h, P.lg);
-    atomicAdd(&hist[gi * cells + cell], (unsigned int)(q.w * 1000.0f));
+  // the recorded pairs, per lane
+  const int mine = min(nrec, kLaneSlots);
+  for (int r = 0; r < mine; ++r) repair(p0 + slots[r * blockDim.x + threadIdx.x]);
+  if (ovf >= 0) {  // more undecided iterations than slots: find the rest again
+    for (long long pl = ovf; p0 + pl < p1; pl += (long long)UP * blockDim.x) {
+      float4 q[UP];
+#pragma unroll
+      for (int u = 0; u < UP; ++u) {
+        const long long p = p0 + pl + (long long)u * blockDim.x;
+        q[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      unsigned int m32[UP];
+      int cell[UP][G];
+      if (!cells_of(q, m32, cell)) repair(p0 + pl);
+    }
   }
   __syncthreads();
   unsigned long long* dst = grids + (long long)grid0 * cells;
@@ -351,7 +404,7 @@ static hipError_t launch_lds(const Params& P, const float4* pos, long long n, lo
                              unsigned short* cells_out, bool sorted, hipStream_t s) {
   const int groups = (P.k + G - 1) / G;
   const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)chunk;
-  const size_t lds = (size_t)G * P.cells * 4;
+  const size_t lds = (size_t)G * P.cells * 4 + (sorted ? 0 : (size_t)kLaneSlots * threads * 4);
   const void* fn = !sorted ? (const void*)deposit_lds_kernel<G>
                    : cells_out ? (const void*)deposit_sorted_kernel<G, true>
                                : (const void*)deposit_sorted_kernel<G, false>;
@@ -383,7 +436,7 @@ hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, co
                              hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const size_t grid_bytes = (size_t)P.cells * 4;
-  if (grid_bytes > kMaxLds) {
+  if (grid_bytes + (size_t)kLaneSlots * 1024 * 4 > kMaxLds) {
     const int G = 8;
     const int groups = (P.k + G - 1) / G;
     const long long blocks = (n + 255) / 256;
@@ -393,7 +446,9 @@ hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, co
   }
   // G in {1, 2, 4, 8}, within the LDS and at most k
   int G = 1;
-  while (G * 2 <= t.grids_per_wg && G * 2 <= 8 && (size_t)(G * 2) * grid_bytes <= kMaxLds &&
+  // (the LDS also holds the per-lane repair records: at most 16 KB)
+  while (G * 2 <= t.grids_per_wg && G * 2 <= 8 &&
+         (size_t)(G * 2) * grid_bytes + (size_t)kLaneSlots * 1024 * 4 <= kMaxLds &&
          G * 2 <= P.k)
     G *= 2;
   const int groups = (P.k + G - 1) / G;
@@ -403,8 +458,11 @@ hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, co
   const long long max_chunks = 0xFFFFFFFFll / ((long long)threads * groups);
   if (chunks > max_chunks) chunks = max_chunks;
   if (chunks < 1) chunks = 1;
+  // whole spans of 2 x 2 x threads particles: every chunk but the last then
+  // takes the double-buffered, unguarded loop of deposit_lds_kernel
+  const long long span = 4ll * threads;
   long long chunk = (n + chunks - 1) / chunks;
-  if (chunk < threads) chunk = threads;
+  chunk = (chunk + span - 1) / span * span;
   chunks = (n + chunk - 1) / chunk;
   switch (G) {
     case 8:

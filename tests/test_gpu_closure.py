@@ -258,13 +258,70 @@ def test_closure_solver_audit_and_fallback(torch_cuda):
     mlp = CL.ClosureMLP(seed=0)
     rho0 = CL.default_rho0(8192, 1.0, g)
     p, v = dev(torch, pos), dev(torch, vel)
-    never = CL.ClosureSolver(full, small, mlp, rho0, audit_every=5, threshold=1e9, fallback_steps=3)
+    never = CL.ClosureSolver(full, small, mlp, rho0, audit_every=5, threshold=1e9, fallback_steps=3,
+                             direct_ratio=1e9, direct_every=5)
     modes = [never.step(p, v) for _ in range(12)]
-    assert modes == ["closure"] * 12
-    assert [a[0] for a in never.audits] == [0, 5, 10] and not any(a[2] for a in never.audits)
-    always = CL.ClosureSolver(full, small, mlp, rho0, audit_every=4, threshold=0.0, fallback_steps=3)
+    assert modes == ["closure"] * 12 and never.closure_steps == 12
+    assert [a.step for a in never.audits] == [0, 5, 10] and not any(a.fallback for a in never.audits)
+    assert all(a.error_vs_direct is not None and a.full_vs_direct > 0 for a in never.audits)
+    always = CL.ClosureSolver(full, small, mlp, rho0, audit_every=4, threshold=0.0, fallback_steps=3,
+                              direct_ratio=1e9)
     modes = [always.step(p, v) for _ in range(9)]
     # every audit fails: audit at step 0, 3 fallback steps, audit at 4, ...
     assert modes == ["full"] * 9
     assert [a[0] for a in always.audits] == [0, 4, 8] and all(a[2] for a in always.audits)
+    assert all(a.reason == "full" for a in always.audits)
+    # the FP64 direct-summation trigger alone: a ratio of 0 fails every audit
+    # on the direct-sum leg while the full-k threshold passes
+    direct = CL.ClosureSolver(full, small, mlp, rho0, audit_every=4, threshold=1e9, fallback_steps=3,
+                              direct_subsample=512, direct_ratio=0.0, direct_every=4)
+    modes = [direct.step(p, v) for _ in range(5)]
+    assert modes == ["full"] * 4 + ["full"]
+    assert [a.reason for a in direct.audits] == ["direct", "direct"]
+    assert all(a.error_vs_direct > 0 and a.fallback for a in direct.audits)
+    # without the subsample there is no direct leg
+    off = CL.ClosureSolver(full, small, mlp, rho0, audit_every=4, threshold=1e9, direct_subsample=0,
+                           direct_ratio=0.0)
+    assert [off.step(p, v) for _ in range(5)] == ["closure"] * 5
+    assert off.audits[0].error_vs_direct is None
+    # the direct leg every 8 steps, the full-k leg every 4
+    sparse = CL.ClosureSolver(full, small, mlp, rho0, audit_every=4, threshold=1e9, direct_subsample=256,
+                              direct_every=8, direct_ratio=1e9)
+    [sparse.step(p, v) for _ in range(9)]
+    assert [a.error_vs_direct is not None for a in sparse.audits] == [True, False, True]
+    assert bool(torch.isfinite(p).all()) and bool(torch.isfinite(v).all())
+
+
+def test_closure_at_benchmark_shapes(torch_cuda):
+    """The closure at its benchmark shapes: k = 24 features and MLP vs the
+    k = 240 force, g = 16, two colliding disks with n = 1e6 (40 training
+    snapshots every 5 steps and 10 epochs here; the report trains on 200).
+    On 4 held-out later snapshots the closure beats the k = 24 deconvolved
+    cloud-in-cell mean and the k = 24 NGP mean against k = 240, and stays
+    under the ClosureSolver's default audit threshold, so a default solver
+    keeps the closure; 20 default steps (audits every 10 steps against
+    k = 240 and FP64 direct summation) run without a fallback."""
+    from mog_solver_amd import closure as CL
+
+    torch = torch_cuda
+    n, g, W, G = 1_000_000, 16, 100.0, 100.0
+    pos, vel = workloads.two_disks(n, W, G, g, seed=2)
+    full, _, _ = solver(240, g, W, G, seed=1)
+    small, _, _ = solver(24, g, W, G, seed=2)
+    p, v = dev(torch, pos), dev(torch, vel)
+    rho0 = CL.default_rho0(n, 1.0, g)
+    train_set = CL.make_snapshots(full, small, p, v, 40, every=5, rho0=rho0)
+    test_set = CL.make_snapshots(full, small, p, v, 4, every=10, rho0=rho0)
+    mlp = CL.ClosureMLP(seed=0, bf16=True)
+    CL.train(mlp, train_set, epochs=10, lr=2e-3, lr_final=2e-4)
+    r = CL.evaluate_all(mlp, test_set)
+    print(f"k=24 vs k=240 (n=1e6, g=16): NGP {r['ngp']:.4f}, cloud-in-cell {r['cic']:.4f}, closure {r['closure']:.4f}")
+    assert r["closure"] < 0.7 * r["cic"] and r["cic"] < r["ngp"]
+    assert r["closure"] < CL.DEFAULT_THRESHOLD
+    cs = CL.ClosureSolver(full, small, mlp, rho0, direct_every=10)
+    modes = [cs.step(p, v) for _ in range(20)]
+    print("audits", [(a.step, round(a.error_vs_full, 4), a.error_vs_direct and round(a.error_vs_direct, 4),
+                      a.full_vs_direct and round(a.full_vs_direct, 4), a.reason) for a in cs.audits])
+    assert len(cs.audits) == 2 and all(a.error_vs_direct is not None for a in cs.audits)
+    assert modes == ["closure"] * 20
     assert bool(torch.isfinite(p).all()) and bool(torch.isfinite(v).all())
