@@ -63,26 +63,27 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
     // one wave-uniform test: only a wave with an undecided pair re-fetches
     // those pairs from the exact cells (the exact sequence is a call, whose
     // entry waits for every load in flight)
+    // one compare per group: the largest face distance of its U pairs
     int cell[U];
-    bool ok[U];
-    bool rare = false;
+    float d[U];
+    float dmax = 0.0f;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int gr = grid + u;
       const float* Fr = frot + 12 * gr;
-      ok[u] = (LDS3 ? cell_fast_a(c0, c1, c2, Fr, adds[3 * gr], adds[3 * gr + 1],
-                                  adds[3 * gr + 2], P.lim, P.lg, &cell[u])
-                    : cell_fast(c0, c1, c2, Fr, P.lim, P.lg, &cell[u])) &
-              pfast;
-      rare |= !ok[u];
+      d[u] = LDS3 ? cell_fast_d(c0, c1, c2, Fr, adds[3 * gr], adds[3 * gr + 1], adds[3 * gr + 2],
+                                P.lg, &cell[u])
+                  : cell_fast_d(c0, c1, c2, Fr, Fr[3], Fr[7], Fr[11], P.lg, &cell[u]);
+      dmax = fmaxf(dmax, d[u]);
     }
+    const bool rare = !(pfast & (dmax < P.lim));
     float4 F[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) F[u] = cell_load(force + (long long)(grid + u) * P.cells, cell[u]);
     if (__ballot(rare)) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        if (!ok[u]) {
+        if (!(pfast & (d[u] < P.lim))) {
           const int gr = grid + u;
           const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half,
                                          P.h, P.lg);
