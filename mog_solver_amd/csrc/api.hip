@@ -602,6 +602,9 @@ hipError_t clean_spare(mog_ctx* ctx, hipStream_t s) {
 // ---------------------------------------------------------------------------
 }  // namespace
 
+// grid sizes 2, 4, 8, 16, 32 each form a solve class of the batch
+constexpr int kBatchClasses = 5;
+
 struct mog_batch {
   int nsims;
   std::vector<mog_config> cfg;
@@ -609,13 +612,23 @@ struct mog_batch {
   std::vector<char> have_transforms;
   long long ntotal, cells_total, gather_blocks;
   int grids_total;
-  // solve classes by grid size (g <= 16, g = 32): simulations, slabs, LDS
+  // solve classes, one per grid size g = 2 .. 32 (class log2 g - 1):
+  // simulations, slabs, LDS
   struct SolveClass {
     int start, count, gmax;
     long long slabs;
-  } cls[2];
+  } cls[kBatchClasses];
   int* d_cls_sim;
   long long* d_cls_off;
+  // solve windows (kBatchWindowCells): class, first slab and slab count,
+  // and per window a row of n_sims scratch offsets (class slot j -> offset
+  // of the simulation's grid 0 in the window's scratch)
+  struct SolveWindow {
+    int cls;
+    long long slab0, slabs;
+  };
+  std::vector<SolveWindow> windows;
+  long long* d_win_soff;
   BatchSim* d_sims;
   float* d_rot;
   float* d_off;
@@ -680,7 +693,7 @@ void batch_release(mog_batch* b) {
   void* ptrs[] = {b->d_sims, b->d_rot,  b->d_off, b->d_frot, b->d_grids, b->d_S1,
                   b->d_S2,   b->d_force, b->d_tw, b->d_gtab, b->d_kd,    b->d_conv,
                   b->d_tasks, b->d_counter, b->d_order, b->d_idx, b->d_keys_a, b->d_keys_b,
-                  b->d_sort_tmp, b->d_cls_sim, b->d_cls_off};
+                  b->d_sort_tmp, b->d_cls_sim, b->d_cls_off, b->d_win_soff};
   for (void* q : ptrs) (void)hipFree(q);
 }
 
@@ -755,11 +768,13 @@ hipError_t batch_enqueue(mog_batch* b, float* pos, float* vel, hipStream_t s) {
     e = launch_batch_deposit((const float4*)pos, b->d_sims, b->d_tasks, b->ntasks, b->d_rot,
                              b->d_off, b->d_frot, (unsigned long long*)b->d_grids, b->d_counter,
                              b->deposit_threads, s);
-  for (int c = 0; c < 2 && e == hipSuccess; ++c) {
-    const mog_batch::SolveClass& sc = b->cls[c];
+  for (size_t w = 0; w < b->windows.size() && e == hipSuccess; ++w) {
+    const mog_batch::SolveWindow& win = b->windows[w];
+    const mog_batch::SolveClass& sc = b->cls[win.cls];
     e = launch_batch_solve(b->d_sims, b->d_cls_sim + sc.start, b->d_cls_off + sc.start, sc.count,
-                           sc.slabs, sc.gmax, b->d_grids, b->d_S1, b->d_S2, b->d_force, b->d_tw,
-                           b->d_gtab, b->d_kd, b->d_conv, b->d_rot, s);
+                           win.slab0, win.slabs, b->d_win_soff + w * (size_t)b->nsims, sc.gmax,
+                           b->d_grids, b->d_S1, b->d_S2, b->d_force, b->d_tw, b->d_gtab, b->d_kd,
+                           b->d_conv, b->d_rot, s);
   }
   if (e == hipSuccess)
     e = launch_batch_gather((float4*)pos, (float4*)vel, b->gather_blocks, b->d_sims, b->nsims,
@@ -1436,11 +1451,11 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
   b->cfg.assign(cfgs, cfgs + n_sims);
   b->sims.resize(n_sims);
   b->have_transforms.assign(n_sims, 0);
-  long long p_off = 0, cell_off = 0, conv_off = 0, blk_off = 0, cls_slabs[2] = {0, 0};
+  long long p_off = 0, cell_off = 0, conv_off = 0, blk_off = 0, cls_slabs[kBatchClasses] = {};
   int grid_off = 0, tab_off = 0, line_off = 0;
-  std::vector<int> cls_sim[2];
-  std::vector<long long> cls_off[2];
-  int cls_gmax[2] = {0, 0};
+  std::vector<int> cls_sim[kBatchClasses];
+  std::vector<long long> cls_off[kBatchClasses];
+  int cls_gmax[kBatchClasses] = {};
   for (int i = 0; i < n_sims; ++i) {
     const mog_config* c = &cfgs[i];
     int rc = validate(c);
@@ -1460,7 +1475,7 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
     sm.P = make_params(c);
     sm.p_off = p_off;
     sm.n = n[i];
-    const int ci = c->grid_size <= 16 ? 0 : 1;
+    const int ci = ilog2(c->grid_size) - 1;
     sm.cell_off = cell_off;
     sm.slab_off = cls_slabs[ci];
     cls_sim[ci].push_back(i);
@@ -1492,7 +1507,7 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
   b->gather_blocks = blk_off;
   std::vector<int> all_sim;
   std::vector<long long> all_off;
-  for (int ci = 0; ci < 2; ++ci) {
+  for (int ci = 0; ci < kBatchClasses; ++ci) {
     b->cls[ci].start = (int)all_sim.size();
     b->cls[ci].count = (int)cls_sim[ci].size();
     b->cls[ci].gmax = cls_gmax[ci];
@@ -1501,6 +1516,32 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
     all_off.insert(all_off.end(), cls_off[ci].begin(), cls_off[ci].end());
   }
   b->grids_total = grid_off;
+  // solve windows: whole grids of one class, at most kBatchWindowCells
+  // cells each, laid out in the window's scratch in slab order
+  std::vector<long long> win_soff;
+  for (int ci = 0; ci < kBatchClasses; ++ci) {
+    long long used = 0;
+    for (size_t jj = 0; jj < cls_sim[ci].size(); ++jj) {
+      const mog_config* c = &cfgs[cls_sim[ci][jj]];
+      const long long g = c->grid_size, g3 = g * g * g;
+      long long placed_in = -1;  // window that holds this simulation's offset
+      for (int grid = 0; grid < c->n_grids; ++grid) {
+        const long long slab = cls_off[ci][jj] + (long long)grid * g;
+        if (b->windows.empty() || b->windows.back().cls != ci || used + g3 > kBatchWindowCells) {
+          b->windows.push_back({ci, slab, 0});
+          win_soff.resize(b->windows.size() * (size_t)n_sims, 0);
+          used = 0;
+        }
+        const long long w = (long long)b->windows.size() - 1;
+        if (placed_in != w) {  // the simulation's first grid in this window
+          win_soff[(size_t)w * n_sims + jj] = used - (long long)grid * g3;
+          placed_in = w;
+        }
+        b->windows.back().slabs += g;
+        used += g3;
+      }
+    }
+  }
   b->deposit_chunk = 16384;
   b->deposit_threads = 1024;
   hipError_t e = hipSuccess;
@@ -1512,8 +1553,10 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
   A(dalloc(&b->d_off, (size_t)grid_off * 4));
   A(dalloc(&b->d_frot, (size_t)grid_off * 12));
   A(dalloc(&b->d_grids, (size_t)cell_off));
-  A(dalloc(&b->d_S1, (size_t)cell_off));
-  A(dalloc(&b->d_S2, (size_t)cell_off));
+  const long long win_cells = std::min(cell_off, kBatchWindowCells);
+  A(dalloc(&b->d_S1, (size_t)win_cells));
+  A(dalloc(&b->d_S2, (size_t)win_cells));
+  A(dalloc(&b->d_win_soff, win_soff.size() > 0 ? win_soff.size() : 1));
   A(dalloc(&b->d_force, (size_t)cell_off));
   A(dalloc(&b->d_tw, (size_t)line_off));
   A(dalloc(&b->d_gtab, (size_t)tab_off));
@@ -1551,6 +1594,9 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
   A(hipMemcpy(b->d_cls_sim, all_sim.data(), all_sim.size() * sizeof(int), hipMemcpyHostToDevice));
   A(hipMemcpy(b->d_cls_off, all_off.data(), all_off.size() * sizeof(long long),
               hipMemcpyHostToDevice));
+  if (!win_soff.empty())
+    A(hipMemcpy(b->d_win_soff, win_soff.data(), win_soff.size() * sizeof(long long),
+                hipMemcpyHostToDevice));
   A(batch_build_tasks(b));
   if (e != hipSuccess) {
     batch_release(b);

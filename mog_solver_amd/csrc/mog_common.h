@@ -281,14 +281,22 @@ hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const D
                                 int ntasks, const float* rot, const float* off, const float* frot,
                                 unsigned long long* grids, unsigned long long* counter,
                                 int threads, hipStream_t s);
-// Solve passes over one grid-size class: simulations cls_sim[0..ncls) whose
-// slabs start at cls_off[j] in the class's range of `slabs` slabs; LDS is
-// sized for the class's largest grid `gmax`.
+// Solve passes over one window of a grid-size class: simulations
+// cls_sim[0..ncls) whose slabs start at cls_off[j] in the class's slab
+// range; the window is slabs [slab0, slab0 + slabs) (whole grids) and the
+// spectral scratch S1/S2 of class slot j starts at offset soff[j] (complex
+// elements; see kBatchWindowCells).  LDS is sized for the class's largest
+// grid `gmax`.
 hipError_t launch_batch_solve(const BatchSim* sims, const int* cls_sim, const long long* cls_off,
-                              int ncls, long long slabs, int gmax, const long long* grids,
-                              float2* S1, float2* S2, float4* force, const float2* tw,
-                              const float* gtab, const float* kd, const float2* conv,
-                              const float* rot, hipStream_t s);
+                              int ncls, long long slab0, long long slabs, const long long* soff,
+                              int gmax, const long long* grids, float2* S1, float2* S2,
+                              float4* force, const float2* tw, const float* gtab, const float* kd,
+                              const float2* conv, const float* rot, hipStream_t s);
+// Cells of the batch's spectral scratch window: the solve runs window by
+// window through one reused pair of complex buffers (2 x 32 MiB), which so
+// stay in the Infinity Cache instead of streaming every simulation's
+// scratch through HBM.
+constexpr long long kBatchWindowCells = 1ll << 22;
 hipError_t launch_batch_gather(float4* pos, float4* vel, long long blocks, const BatchSim* sims,
                                int nsims, const float* rot, const float* off, const float* frot,
                                const float4* force, const int* order, hipStream_t s);

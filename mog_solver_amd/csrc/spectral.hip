@@ -21,6 +21,7 @@
 // -2*pi*G/k^2 of greens.glsl, 1/g^3 of pack.glsl) are folded into one FP64
 // table `gtab`.
 #include "mog_common.h"
+#include "regfft.h"
 
 namespace mog {
 
@@ -474,6 +475,264 @@ __global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Register-FFT passes for g = 16 and 32: every length-g line is transformed
+// by one thread in registers (regfft.h), so an axis costs one LDS (or
+// global) read and one write per element and no barrier inside it.  The
+// same three passes as above, with the same data layout in S1/S2:
+//   A  per (grid, z) slab: x- then y-transform                  -> S1
+//   B  per (grid, ky, kx) line along z: forward z, Green x i*k, inverse z of
+//      the y+iz spectrum (and of the x spectrum)                -> S1, S2
+//   C  per (grid, z) slab: inverse y, inverse x, pack float4
+// Pass B needs no LDS: a wave's lanes take consecutive kx, so its loads and
+// stores of one z plane are contiguous.  Passes A and C stage rows in LDS
+// with a padded row stride of g + 1 (lanes of a row read hit distinct
+// banks) and take 256 / g (A) or 128 / g (C, two fields) slabs per
+// workgroup of 256 threads.
+// ---------------------------------------------------------------------------
+// Where slab t of a launch lives: its simulation's arrays and tables and
+// the slab's index inside that simulation (slab = grid * g + z in passes A
+// and C, grid * g + ky in pass B).
+struct RfSlab {
+  const long long* mass;
+  float2* S;
+  float2* S2;
+  float4* force;
+  const float* gtab;
+  const float* kd;
+  const float2* conv;
+  const float* rot;
+  long long local;
+  int need_x;
+};
+
+// One simulation: slab t is its slab t.
+struct RfSingle {
+  const long long* mass;
+  float2* S;
+  float2* S2;
+  float4* force;
+  const float* gtab;
+  const float* kd;
+  const float2* conv;
+  const float* rot;
+  int need_x;
+  __device__ RfSlab operator()(long long t) const {
+    return {mass, S, S2, force, gtab, kd, conv, rot, t, need_x};
+  }
+};
+
+// A window of a batch grid-size class: slab t of the class belongs to class
+// slot j; its spectral scratch starts at S + soff[j] in the window buffers.
+struct RfBatch {
+  const BatchSim* sims;
+  const int* cls_sim;
+  const long long* cls_off;
+  int ncls;
+  const long long* soff;
+  const long long* mass;
+  float2* S;
+  float2* S2;
+  float4* force;
+  const float* gtab;
+  const float* kd;
+  const float2* conv;
+  const float* rot;
+  __device__ RfSlab operator()(long long t) const;
+};
+
+template <int G, typename R>
+__device__ __forceinline__ void rf_fwd_xy(const R& res, long long t0, long long nslabs,
+                                          long long blk) {
+  constexpr int SL = 256 / G, GG = G * G, RS = G + 1;
+  extern __shared__ float2 sm[];
+  __shared__ const long long* s_mass[SL];
+  __shared__ float2* s_S[SL];
+  const long long first = blk * SL;
+  if (threadIdx.x < SL && first + threadIdx.x < nslabs) {
+    const RfSlab a = res(t0 + first + threadIdx.x);
+    s_mass[threadIdx.x] = a.mass + a.local * GG;
+    s_S[threadIdx.x] = a.S + a.local * GG;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < SL * GG; i += blockDim.x) {
+    const int sl = i / GG, r = (i / G) % G, c = i % G;
+    const float v = first + sl < nslabs ? (float)s_mass[sl][r * G + c] : 0.f;
+    sm[(sl * G + r) * RS + c] = make_float2(v, 0.f);
+  }
+  __syncthreads();
+  const int sl = threadIdx.x / G, line = threadIdx.x % G;
+  float2 v[G];
+  float2* row = sm + (sl * G + line) * RS;  // x line: row `line` of slab sl
+#pragma unroll
+  for (int x = 0; x < G; ++x) v[x] = row[x];
+  reg_fft<G, false>(v);
+#pragma unroll
+  for (int x = 0; x < G; ++x) row[x] = v[x];
+  __syncthreads();
+  float2* col = sm + sl * G * RS + line;  // y line: column kx = `line`
+#pragma unroll
+  for (int y = 0; y < G; ++y) v[y] = col[y * RS];
+  reg_fft<G, false>(v);
+  if (first + sl < nslabs) {
+    float2* dst = s_S[sl];
+#pragma unroll
+    for (int ky = 0; ky < G; ++ky) dst[ky * G + line] = v[ky];
+  }
+}
+
+// Pass B: line `line` of the launch = (slab t0 + line / G, kx = line % G).
+template <int G, typename R>
+__device__ __forceinline__ void rf_z_green(const R& res, long long t0, long long nlines,
+                                           long long line) {
+  constexpr int GG = G * G;
+  if (line >= nlines) return;
+  const RfSlab a0 = res(t0 + line / G);
+  const long long grid = a0.local / G;
+  const int ky = (int)(a0.local % G), kx = (int)(line % G);
+  const long long at0 = grid * G * GG + ky * G + kx;
+  float2 a[G], xs[G];
+#pragma unroll
+  for (int z = 0; z < G; ++z) a[z] = a0.S[at0 + (long long)z * GG];
+  reg_fft<G, false>(a);
+  const float kyv = a0.kd[ky], kxv = a0.kd[kx];
+#pragma unroll
+  for (int kz = 0; kz < G; ++kz) {
+    const int fi = (kz * G + ky) * G + kx;
+    const float2 acc = a[kz];
+    if (a0.conv) {
+      const float2 t = a0.conv[fi], u = a0.conv[G * GG + fi];
+      a[kz] = make_float2(fmaf(acc.x, t.x, -(acc.y * t.y)), fmaf(acc.x, t.y, acc.y * t.x));
+      xs[kz] = make_float2(fmaf(acc.x, u.x, -(acc.y * u.y)), fmaf(acc.x, u.y, acc.y * u.x));
+    } else {
+      const float gf = a0.gtab[fi];
+      const float pr = acc.x * gf, pi = acc.y * gf;
+      const float kzv = a0.kd[kz];
+      a[kz] = make_float2(fmaf(-pr, kzv, -(pi * kyv)), fmaf(pr, kyv, -(pi * kzv)));
+      xs[kz] = make_float2(-(pi * kxv), pr * kxv);
+    }
+  }
+  reg_fft<G, true>(a);
+#pragma unroll
+  for (int z = 0; z < G; ++z) a0.S[at0 + (long long)z * GG] = a[z];
+  if (a0.need_x) {
+    reg_fft<G, true>(xs);
+#pragma unroll
+    for (int z = 0; z < G; ++z) a0.S2[at0 + (long long)z * GG] = xs[z];
+  }
+}
+
+// Pass C; world != 0 stores M F with the simulation's rotation rows.
+template <int G, typename R>
+__device__ __forceinline__ void rf_inv_xy(const R& res, long long t0, long long nslabs, int world,
+                                          long long blk) {
+  constexpr int SL = 128 / G, GG = G * G, RS = G + 1;
+  extern __shared__ float2 sm[];  // [sl][field][y][x], rows padded to RS
+  __shared__ float4* s_force[SL];
+  __shared__ const float* s_rot[SL];
+  __shared__ int s_need_x[SL];
+  const long long first = blk * SL;
+  const int f = (threadIdx.x / G) & 1, sl = threadIdx.x / (2 * G), line = threadIdx.x % G;
+  const bool valid = first + sl < nslabs;
+  RfSlab a = {};
+  if (valid) a = res(t0 + first + sl);
+  if (valid && f == 0 && line == 0) {
+    s_force[sl] = a.force + a.local * GG;
+    s_rot[sl] = a.rot + 12 * (a.local / G);
+    s_need_x[sl] = a.need_x;
+  }
+  float2* rows = sm + (sl * 2 + f) * G * RS;
+  float2 v[G];
+  const bool busy = valid && (f == 0 || a.need_x);
+  // inverse y of column kx = `line` of field f (f = 1: the x spectrum)
+  if (busy) {
+    const float2* src = (f == 0 ? a.S : a.S2) + a.local * GG;
+#pragma unroll
+    for (int ky = 0; ky < G; ++ky) v[ky] = src[ky * G + line];
+    reg_fft<G, true>(v);
+#pragma unroll
+    for (int y = 0; y < G; ++y) rows[y * RS + line] = v[y];
+  }
+  __syncthreads();
+  // inverse x of row y = `line`
+  if (busy) {
+    float2* row = rows + line * RS;
+#pragma unroll
+    for (int x = 0; x < G; ++x) v[x] = row[x];
+    reg_fft<G, true>(v);
+#pragma unroll
+    for (int x = 0; x < G; ++x) row[x] = v[x];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < SL * GG; i += blockDim.x) {
+    const int s2 = i / GG, y = (i / G) % G, x = i % G;
+    if (first + s2 >= nslabs) break;
+    const float2 tz = sm[((s2 * 2) * G + y) * RS + x];
+    const float fy = tz.x, fz = tz.y;
+    const float fx = s_need_x[s2] ? sm[((s2 * 2 + 1) * G + y) * RS + x].x : fz;
+    const float mag = sqrtf(fmaf(fx, fx, fmaf(fy, fy, fz * fz)));
+    float4 out;
+    if (world) {
+      const float* Rr = s_rot[s2];
+      out = make_float4(fmaf(Rr[0], fx, fmaf(Rr[1], fy, Rr[2] * fz)),
+                        fmaf(Rr[4], fx, fmaf(Rr[5], fy, Rr[6] * fz)),
+                        fmaf(Rr[8], fx, fmaf(Rr[9], fy, Rr[10] * fz)), mag);
+    } else {
+      out = make_float4(fx, fy, fz, mag);
+    }
+    s_force[s2][y * G + x] = out;
+  }
+}
+
+template <int G, typename R>
+__global__ __launch_bounds__(256) void rf_fwd_xy_kernel(R res, long long t0, long long nslabs) {
+  rf_fwd_xy<G>(res, t0, nslabs, blockIdx.x);
+}
+template <int G, typename R>
+__global__ __launch_bounds__(256) void rf_z_green_kernel(R res, long long t0, long long nlines) {
+  rf_z_green<G>(res, t0, nlines, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+template <int G, typename R>
+__global__ __launch_bounds__(256) void rf_inv_xy_kernel(R res, long long t0, long long nslabs,
+                                                        int world) {
+  rf_inv_xy<G>(res, t0, nslabs, world, blockIdx.x);
+}
+
+// The three register-FFT passes over slabs [t0, t0 + nslabs) of `res`.
+template <int G, typename R>
+static hipError_t launch_rf_passes(const R& res, long long t0, long long nslabs, int world,
+                                   hipStream_t s) {
+  if (nslabs <= 0) return hipSuccess;
+  const long long lines = nslabs * G;
+  const size_t ldsA = (size_t)(256 / G) * G * (G + 1) * sizeof(float2);
+  const size_t ldsC = (size_t)(128 / G) * 2 * G * (G + 1) * sizeof(float2);
+  hipError_t e;
+  if ((e = hipFuncSetAttribute((const void*)rf_fwd_xy_kernel<G, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsA)) != hipSuccess)
+    return e;
+  if ((e = hipFuncSetAttribute((const void*)rf_inv_xy_kernel<G, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsC)) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL((rf_fwd_xy_kernel<G, R>), dim3((unsigned)((nslabs + 256 / G - 1) / (256 / G))),
+                     dim3(256), ldsA, s, res, t0, nslabs);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL((rf_z_green_kernel<G, R>), dim3((unsigned)((lines + 255) / 256)), dim3(256), 0,
+                     s, res, t0, lines);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL((rf_inv_xy_kernel<G, R>), dim3((unsigned)((nslabs + 128 / G - 1) / (128 / G))),
+                     dim3(256), ldsC, s, res, t0, nslabs, world);
+  return hipGetLastError();
+}
+
+template <int G>
+static hipError_t launch_rf_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
+                                  float4* force, const float* gtab, const float* kd,
+                                  const float2* conv, int need_x, const float* rot, int world,
+                                  hipStream_t s) {
+  const RfSingle res = {grids, S1, S2, force, gtab, kd, conv, rot, need_x};
+  return launch_rf_passes<G>(res, 0, (long long)P.k * G, world, s);
+}
+
+// ---------------------------------------------------------------------------
 // Batched ensemble: the three passes, one launch each per grid-size class
 // (g <= 16, g = 32), one workgroup per slab of any simulation of the class.
 // A class lists its simulations (cls_sim) and their first slab in the
@@ -495,33 +754,52 @@ __device__ __forceinline__ int class_slot(const long long* __restrict__ cls_off,
   return lo;
 }
 
+// The spectral scratch of the batch covers one window of whole grids at a
+// time: slab t of a window (slabs slab0 .. slab0 + nslabs of its class)
+// belongs to class slot j, whose scratch starts at S + soff[j] (the window's
+// offset of the simulation's grid 0, so that the slab bodies index the
+// scratch as they index the simulation's own arrays).
+__device__ RfSlab RfBatch::operator()(long long t) const {
+  const int j = class_slot(cls_off, ncls, t);
+  const BatchSim& b = sims[cls_sim[j]];
+  return {mass + b.cell_off,          S + soff[j],
+          S2 + soff[j],               force + b.cell_off,
+          gtab + b.tab_off,           kd + b.line_off,
+          b.conv_off >= 0 ? conv + b.conv_off : nullptr,
+          rot + 12 * b.grid_off,      t - cls_off[j],
+          b.need_x};
+}
+
 __global__ __launch_bounds__(256) void batch_fwd_xy_kernel(const BatchSim* __restrict__ sims,
                                                            const int* __restrict__ cls_sim,
                                                            const long long* __restrict__ cls_off,
-                                                           int ncls,
+                                                           int ncls, long long slab0,
+                                                           const long long* __restrict__ soff,
                                                            const long long* __restrict__ mass,
                                                            float2* __restrict__ S,
                                                            const float2* __restrict__ tw) {
-  const long long t = blockIdx.x;
+  const long long t = slab0 + blockIdx.x;
   const int j = class_slot(cls_off, ncls, t);
   const BatchSim& b = sims[cls_sim[j]];
-  spec_fwd_xy_slab(mass + b.cell_off, S + b.cell_off, tw + b.line_off, b.P.g, b.P.lg,
+  spec_fwd_xy_slab(mass + b.cell_off, S + soff[j], tw + b.line_off, b.P.g, b.P.lg,
                    t - cls_off[j]);
 }
 
 __global__ __launch_bounds__(256) void batch_z_green_kernel(const BatchSim* __restrict__ sims,
                                                             const int* __restrict__ cls_sim,
                                                             const long long* __restrict__ cls_off,
-                                                            int ncls, float2* __restrict__ S,
+                                                            int ncls, long long slab0,
+                                                            const long long* __restrict__ soff,
+                                                            float2* __restrict__ S,
                                                             float2* __restrict__ S2,
                                                             const float2* __restrict__ tw,
                                                             const float* __restrict__ gtab,
                                                             const float* __restrict__ kd,
                                                             const float2* __restrict__ conv) {
-  const long long t = blockIdx.x;
+  const long long t = slab0 + blockIdx.x;
   const int j = class_slot(cls_off, ncls, t);
   const BatchSim& b = sims[cls_sim[j]];
-  spec_z_green_slab(S + b.cell_off, S2 + b.cell_off, tw + b.line_off, gtab + b.tab_off,
+  spec_z_green_slab(S + soff[j], S2 + soff[j], tw + b.line_off, gtab + b.tab_off,
                     kd + b.line_off, b.conv_off >= 0 ? conv + b.conv_off : nullptr, b.P.g, b.P.lg,
                     b.need_x, t - cls_off[j]);
 }
@@ -529,15 +807,17 @@ __global__ __launch_bounds__(256) void batch_z_green_kernel(const BatchSim* __re
 __global__ __launch_bounds__(256) void batch_inv_xy_kernel(const BatchSim* __restrict__ sims,
                                                            const int* __restrict__ cls_sim,
                                                            const long long* __restrict__ cls_off,
-                                                           int ncls, const float2* __restrict__ S,
+                                                           int ncls, long long slab0,
+                                                           const long long* __restrict__ soff,
+                                                           const float2* __restrict__ S,
                                                            const float2* __restrict__ S2,
                                                            float4* __restrict__ force,
                                                            const float2* __restrict__ tw,
                                                            const float* __restrict__ rot) {
-  const long long t = blockIdx.x;
+  const long long t = slab0 + blockIdx.x;
   const int j = class_slot(cls_off, ncls, t);
   const BatchSim& b = sims[cls_sim[j]];
-  spec_inv_xy_slab(S + b.cell_off, S2 + b.cell_off, force + b.cell_off, tw + b.line_off, b.P.g,
+  spec_inv_xy_slab(S + soff[j], S2 + soff[j], force + b.cell_off, tw + b.line_off, b.P.g,
                    b.P.lg, b.need_x, rot + 12 * b.grid_off, 1, t - cls_off[j]);
 }
 
@@ -554,11 +834,18 @@ static size_t solve_lds(int g, int pass) {
 }
 
 hipError_t launch_batch_solve(const BatchSim* sims, const int* cls_sim, const long long* cls_off,
-                              int ncls, long long slabs, int gmax, const long long* grids,
-                              float2* S1, float2* S2, float4* force, const float2* tw,
-                              const float* gtab, const float* kd, const float2* conv,
-                              const float* rot, hipStream_t s) {
+                              int ncls, long long slab0, long long slabs, const long long* soff,
+                              int gmax, const long long* grids, float2* S1, float2* S2,
+                              float4* force, const float2* tw, const float* gtab, const float* kd,
+                              const float2* conv, const float* rot, hipStream_t s) {
   if (slabs <= 0 || ncls <= 0) return hipSuccess;
+  // a class of one grid size 16 or 32 runs the register-FFT passes, the
+  // same arithmetic as the single-simulation solve
+  if (gmax == 16 || gmax == 32) {
+    const RfBatch res = {sims, cls_sim, cls_off, ncls, soff, grids, S1, S2, force, gtab, kd, conv, rot};
+    return gmax == 16 ? launch_rf_passes<16>(res, slab0, slabs, 1, s)
+                      : launch_rf_passes<32>(res, slab0, slabs, 1, s);
+  }
   const size_t ldsA = solve_lds(gmax, 0), ldsB = solve_lds(gmax, 1), ldsC = solve_lds(gmax, 2);
   hipError_t e;
   if ((e = hipFuncSetAttribute((const void*)batch_fwd_xy_kernel,
@@ -572,13 +859,13 @@ hipError_t launch_batch_solve(const BatchSim* sims, const int* cls_sim, const lo
     return e;
   const dim3 grid((unsigned)slabs);
   hipLaunchKernelGGL(batch_fwd_xy_kernel, grid, dim3(256), ldsA, s, sims, cls_sim, cls_off, ncls,
-                     grids, S1, tw);
+                     slab0, soff, grids, S1, tw);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(batch_z_green_kernel, grid, dim3(256), ldsB, s, sims, cls_sim, cls_off, ncls,
-                     S1, S2, tw, gtab, kd, conv);
+                     slab0, soff, S1, S2, tw, gtab, kd, conv);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(batch_inv_xy_kernel, grid, dim3(256), ldsC, s, sims, cls_sim, cls_off, ncls,
-                     S1, S2, force, tw, rot);
+                     slab0, soff, S1, S2, force, tw, rot);
   return hipGetLastError();
 }
 
@@ -587,6 +874,10 @@ hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, flo
                         const float2* conv, int need_x, const float* rot, int world,
                         hipStream_t s) {
   const int g = P.g, lg = P.lg;
+  if (g == 16)
+    return launch_rf_solve<16>(P, grids, S1, S2, force, gtab, kd, conv, need_x, rot, world, s);
+  if (g == 32)
+    return launch_rf_solve<32>(P, grids, S1, S2, force, gtab, kd, conv, need_x, rot, world, s);
   const unsigned slabs = (unsigned)(P.k * g);
   const size_t ldsA = solve_lds(g, 0), ldsB = solve_lds(g, 1), ldsC = solve_lds(g, 2);
   hipError_t e;
