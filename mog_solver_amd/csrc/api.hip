@@ -1542,7 +1542,7 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
       }
     }
   }
-  b->deposit_chunk = 16384;
+  b->deposit_chunk = 131072;
   b->deposit_threads = 1024;
   hipError_t e = hipSuccess;
   auto A = [&](hipError_t r) {

@@ -15,131 +15,49 @@
 // over the block offsets and then runs the single-simulation world-frame
 // gather (same summation order, same integrator), so the new state is
 // bit-identical to stepping each simulation alone.
+#include <type_traits>
+
 #include "mog_common.h"
 
 namespace mog {
 
-// Deferred (particle, grid) pairs per task: pairs that the fast cell test
-// cannot decide, settled by whole waves at the end of the task.
-constexpr int kBatchDeferCap = 512;
+// Repair records per lane and task (see batch_deposit_kernel).
+constexpr int kBatchLaneSlots = 4;
 
-// One particle's inputs to a deposit group.
-struct DepositParticle {
-  float c0, c1, c2;    // centred position
-  int pfast;           // fast-path admission
-  bool small, big;     // 32-bit LDS partial / 64-bit global add
-  unsigned int m32;    // milli-mass (small)
-  float mass;
-  unsigned int pofs;   // offset within the task
-};
-
-// A group of UP particles x GU grids (gi0 .. gi0+GU-1, all < ng): the UP x GU
-// fast cells with no branch between them, their atomics, then one
-// wave-uniform test for the rare paths (undecided pairs go to the task's
-// queue; huge masses to the 64-bit global grids).
-template <int UP, int GU>
-__device__ __forceinline__ void deposit_group(
-    int gi0, int gbase, int cells, const Params& P, const DepositParticle (&q)[UP],
-    const float* __restrict__ frot, const float* __restrict__ rot, const float* __restrict__ off,
-    unsigned int* hist, unsigned long long* __restrict__ dst, unsigned int* defer_p,
-    unsigned short* defer_g, unsigned int* defer_n) {
-  int cell[UP][GU];
-  bool ok[UP][GU];
-  bool rare = false;
-#pragma unroll
-  for (int a = 0; a < UP; ++a) {
-    rare |= q[a].big;
-#pragma unroll
-    for (int u = 0; u < GU; ++u)
-      ok[a][u] = cell_fast(q[a].c0, q[a].c1, q[a].c2, frot + 12 * (gbase + gi0 + u), P.lim, P.lg,
-                           &cell[a][u]) &
-                 q[a].pfast;
-  }
-#pragma unroll
-  for (int a = 0; a < UP; ++a)
-#pragma unroll
-    for (int u = 0; u < GU; ++u) {
-      // hidden from the optimiser: a known-zero add would become an atomic
-      // load and a wait
-      unsigned int v = q[a].small && ok[a][u] ? q[a].m32 : 0u;
-      asm volatile("" : "+v"(v));
-      atomicAdd(&hist[(gi0 + u) * cells + cell[a][u]], v);
-      rare |= q[a].small && !ok[a][u];
-    }
-  if (!__ballot(rare)) return;  // wave-uniform
-#pragma unroll
-  for (int a = 0; a < UP; ++a)
-#pragma unroll
-    for (int u = 0; u < GU; ++u) {
-      const int gi = gi0 + u;
-      const int gr = gbase + gi;
-      if (q[a].big) {
-        const int c = ok[a][u] ? cell[a][u]
-                               : cell_exact_call(q[a].c0, q[a].c1, q[a].c2, rot + 12 * gr,
-                                                 off + 4 * gr, P.W, P.half, P.h, P.lg);
-        atomicAdd(&dst[(long long)gi * cells + c], milli_of(q[a].mass));
-      }
-      const bool defer = q[a].small && !ok[a][u];
-      const unsigned long long need = __ballot(defer);
-      if (need) {  // queue: one reservation per wave
-        const int lane = threadIdx.x & 63;
-        const int leader = __ffsll((long long)need) - 1;
-        unsigned int base = 0u;
-        if (lane == leader) base = atomicAdd(defer_n, (unsigned int)__popcll(need));
-        base = __builtin_amdgcn_readlane(base, leader);
-        if (defer) {
-          const unsigned int slot =
-              base + __builtin_amdgcn_mbcnt_hi((unsigned int)(need >> 32),
-                                               __builtin_amdgcn_mbcnt_lo((unsigned int)need, 0u));
-          if (slot < (unsigned int)kBatchDeferCap) {
-            defer_p[slot] = q[a].pofs;
-            defer_g[slot] = (unsigned short)gi;
-          } else {
-            const int ex = cell_exact_call(q[a].c0, q[a].c1, q[a].c2, rot + 12 * gr, off + 4 * gr,
-                                           P.W, P.half, P.h, P.lg);
-            atomicAdd(&hist[gi * cells + ex], q[a].m32);
-          }
-        }
-      }
-    }
+// The 32-bit milli-mass a deposit adds for a particle of mass w (the
+// single-simulation deposit's rule): uint(w * 1000) in [1, 2^32), clamped
+// below 2^32 above, 0 below 1 and for NaN; `small` when it is the true
+// milli-mass and fits the task's 32-bit partials.
+__device__ __forceinline__ unsigned int batch_milli32(float w, unsigned int lds_limit32,
+                                                      bool& small) {
+  const float mm = w * 1000.0f;
+  const unsigned int m = (unsigned int)fminf(fmaxf(mm, 0.0f), 4294967040.0f);
+  small = m <= lds_limit32;
+  return m;
 }
 
-// Inputs of the particle at `p` (mass 0 when dead).
-__device__ __forceinline__ DepositParticle deposit_particle(const float4 qv, bool in, long long p,
-                                                            long long p0, const Params& P,
-                                                            unsigned int lds_limit32) {
-  // milli-mass as in the single-simulation deposit: 32-bit LDS partials
-  // while they cannot overflow, the 64-bit global grids otherwise
-  DepositParticle d;
-  const float mm = in ? qv.w * 1000.0f : 0.0f;
-  const bool live = mm >= 1.0f;
-  d.m32 = live && mm < 4294967296.0f ? (unsigned int)mm : 0u;
-  d.small = live && mm < 4294967296.0f && d.m32 <= lds_limit32;
-  d.big = live && !d.small;
-  d.mass = qv.w;
-  d.c0 = qv.x - P.half;
-  d.c1 = qv.y - P.half;
-  d.c2 = qv.z - P.half;
-  d.pfast = particle_fast(d.c0, d.c1, d.c2, P.CB);
-  d.pofs = (unsigned int)(p - p0);
-  return d;
-}
-
+// Persistent deposit over the task list.  A workgroup takes the next task
+// from a device counter and deposits like the single-simulation kernel:
+// every (particle, grid) pair adds its 32-bit milli-mass at the fast cell
+// with no per-pair compare or select; a lane whose iteration holds an
+// undecided pair (a face within the fast path's margin, a particle outside
+// the admitted bound, a milli-mass that does not fit the 32-bit partials)
+// records the iteration, and after the task's loop repairs its pairs: the
+// exact cell takes the mass from the fast one (32-bit modular arithmetic),
+// huge masses leave the partials for the 64-bit grids.  The flush adds the
+// non-zero counters to the grids and zeroes them, so the next task starts on
+// a clean histogram (one clear per workgroup, not one per task).
 __global__ __launch_bounds__(1024) void batch_deposit_kernel(
     const float4* __restrict__ pos, const BatchSim* __restrict__ sims,
     const DepositTask* __restrict__ tasks, int ntasks, const float* __restrict__ rot,
     const float* __restrict__ off, const float* __restrict__ frot,
     unsigned long long* __restrict__ grids, unsigned long long* __restrict__ counter) {
-  extern __shared__ unsigned int hist[];
+  extern __shared__ unsigned int hist[];  // kBatchLdsCells counters, then the records
+  unsigned int* slots = hist + kBatchLdsCells;  // record r of lane t: slots[r * blockDim.x + t]
   __shared__ int cur;
-  __shared__ unsigned int defer_p[kBatchDeferCap];  // particle offset within the task
-  __shared__ unsigned short defer_g[kBatchDeferCap];  // grid within the task
-  __shared__ unsigned int defer_n;
+  for (int i = threadIdx.x; i < kBatchLdsCells; i += blockDim.x) hist[i] = 0u;
   for (;;) {
-    if (threadIdx.x == 0) {
-      cur = (int)atomicAdd(counter, 1ull);
-      defer_n = 0u;
-    }
+    if (threadIdx.x == 0) cur = (int)atomicAdd(counter, 1ull);
     __syncthreads();
     const int ti = cur;
     if (ti >= ntasks) break;  // uniform across the workgroup
@@ -149,64 +67,129 @@ __global__ __launch_bounds__(1024) void batch_deposit_kernel(
     const int cells = P.cells;
     const int ng = task.ng;
     const int total = ng * cells;
-    for (int i = threadIdx.x; i < total; i += blockDim.x) hist[i] = 0u;
-    __syncthreads();
     const long long p0 = b.p_off + task.p0, p1 = b.p_off + task.p1;
     const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)(task.p1 - task.p0);
     const unsigned int lds_limit32 =
         (unsigned int)(lds_limit < 0xFFFFFFFFull ? lds_limit : 0xFFFFFFFFull);
     const int gbase = b.grid_off + task.grid0;
     unsigned long long* dst = grids + b.cell_off + (long long)task.grid0 * cells;
+    // one grid: two particles per lane and iteration; more: one particle
+    const int UPn = ng == 1 ? 2 : 1;  // particles per lane and iteration
+    const long long step = (long long)UPn * blockDim.x;
+
+    // the pairs of the iteration whose first particle is p (particles p and,
+    // with one grid, p + blockDim.x): add every fast cell when `add`, and
+    // return whether every pair is decided
+    auto iteration = [&](long long p, bool add, auto up) -> bool {
+      constexpr int UP = decltype(up)::value;
+      bool sure = true;
+      float dmax = 0.0f;
+      float4 qs[UP];
+#pragma unroll
+      for (int u = 0; u < UP; ++u) {
+        const long long pu = p + (long long)u * blockDim.x;
+        qs[u] = pu < p1 ? pos[pu] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < UP; ++u) {
+        const float4 q = qs[u];
+        bool small;
+        const unsigned int m32 = batch_milli32(q.w, lds_limit32, small);
+        const float c0 = q.x - P.half, c1 = q.y - P.half, c2 = q.z - P.half;
+        sure = sure & small & (fabsf(c0) <= P.CB) & (fabsf(c1) <= P.CB) & (fabsf(c2) <= P.CB);
+        int gi = 0;
+        for (; gi + 4 <= ng; gi += 4) {
+          int cell[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float* F = frot + 12 * (gbase + gi + v);
+            dmax = fmaxf(dmax, cell_fast_d(c0, c1, c2, F, F[3], F[7], F[11], P.lg, &cell[v]));
+          }
+          if (add) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) atomicAdd(&hist[(gi + v) * cells + cell[v]], m32);
+          }
+        }
+        for (; gi < ng; ++gi) {
+          int cell;
+          const float* F = frot + 12 * (gbase + gi);
+          dmax = fmaxf(dmax, cell_fast_d(c0, c1, c2, F, F[3], F[7], F[11], P.lg, &cell));
+          if (add) atomicAdd(&hist[gi * cells + cell], m32);
+        }
+      }
+      return sure & (dmax < P.lim);
+    };
+
+    // repair the pairs of the iteration whose first particle is p
+    auto repair = [&](long long p) {
+      for (int u = 0; u < UPn; ++u) {
+        const long long pu = p + (long long)u * blockDim.x;
+        if (pu >= p1) continue;
+        const float4 q = pos[pu];
+        bool small;
+        const unsigned int m32 = batch_milli32(q.w, lds_limit32, small);
+        const float c0 = q.x - P.half, c1 = q.y - P.half, c2 = q.z - P.half;
+        const int pf = particle_fast(c0, c1, c2, P.CB);
+        for (int gi = 0; gi < ng; ++gi) {
+          const int gr = gbase + gi;
+          const float* F = frot + 12 * gr;
+          int fc;
+          const bool ok = pf && cell_fast_d(c0, c1, c2, F, F[3], F[7], F[11], P.lg, &fc) < P.lim;
+          if (!small) {  // huge mass: out of the partials, into the 64-bit grids
+            if (m32) atomicAdd(&hist[gi * cells + fc], 0u - m32);
+            const unsigned long long milli = milli_of(q.w);
+            if (milli) {
+              const int ex = ok ? fc : cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr,
+                                                       P.W, P.half, P.h, P.lg);
+              atomicAdd(&dst[(long long)gi * cells + ex], milli);
+            }
+          } else if (!ok && m32) {
+            const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half,
+                                           P.h, P.lg);
+            if (ex != fc) {
+              atomicAdd(&hist[gi * cells + ex], m32);
+              atomicAdd(&hist[gi * cells + fc], 0u - m32);
+            }
+          }
+        }
+      }
+    };
+
+    int nrec = 0;
+    long long ovf = -1;
+    auto record = [&](long long p) {
+      if (nrec < kBatchLaneSlots)
+        slots[nrec * blockDim.x + threadIdx.x] = (unsigned int)(p - p0);
+      else if (nrec == kBatchLaneSlots)
+        ovf = p;
+      ++nrec;
+    };
+    // lane-varying trip counts: no barrier inside
     if (ng == 1) {
-      // one grid: two particles per lane and iteration, the next two loaded
-      // while these are deposited
-      const long long stride = 2LL * blockDim.x;
-      long long p = p0 + threadIdx.x;
-      float4 n0 = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
-      float4 n1 = p + blockDim.x < p1 ? pos[p + blockDim.x] : make_float4(0.f, 0.f, 0.f, 0.f);
-      for (; p < p1; p += stride) {  // lane-varying trip count: no barrier inside
-        const float4 a = n0, b2 = n1;
-        const long long pb = p + blockDim.x;
-        n0 = p + stride < p1 ? pos[p + stride] : make_float4(0.f, 0.f, 0.f, 0.f);
-        n1 = pb + stride < p1 ? pos[pb + stride] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const DepositParticle q[2] = {deposit_particle(a, true, p, p0, P, lds_limit32),
-                                      deposit_particle(b2, pb < p1, pb, p0, P, lds_limit32)};
-        deposit_group<2, 1>(0, gbase, cells, P, q, frot, rot, off, hist, dst, defer_p, defer_g,
-                            &defer_n);
-      }
+      for (long long p = p0 + threadIdx.x; p < p1; p += step)
+        if (!iteration(p, true, std::integral_constant<int, 2>{})) record(p);
     } else {
-      // the next particle is loaded while this one is deposited
-      float4 qn = p0 + threadIdx.x < p1 ? pos[p0 + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
-      for (long long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        const float4 qv = qn;
-        qn = p + blockDim.x < p1 ? pos[p + blockDim.x] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const DepositParticle q[1] = {deposit_particle(qv, true, p, p0, P, lds_limit32)};
-        int gi0 = 0;
-        for (; gi0 + 4 <= ng; gi0 += 4)
-          deposit_group<1, 4>(gi0, gbase, cells, P, q, frot, rot, off, hist, dst, defer_p, defer_g,
-                              &defer_n);
-        for (; gi0 < ng; ++gi0)
-          deposit_group<1, 1>(gi0, gbase, cells, P, q, frot, rot, off, hist, dst, defer_p, defer_g,
-                              &defer_n);
-      }
+      for (long long p = p0 + threadIdx.x; p < p1; p += step)
+        if (!iteration(p, true, std::integral_constant<int, 1>{})) record(p);
     }
-    __syncthreads();
-    // the deferred pairs, one per lane
-    const unsigned int nd = min(defer_n, (unsigned int)kBatchDeferCap);
-    for (unsigned int d = threadIdx.x; d < nd; d += blockDim.x) {
-      const int gi = defer_g[d];
-      const int gr = gbase + gi;
-      const float4 q = pos[p0 + defer_p[d]];
-      const int cell = cell_exact_call(q.x - P.half, q.y - P.half, q.z - P.half, rot + 12 * gr,
-                                       off + 4 * gr, P.W, P.half, P.h, P.lg);
-      atomicAdd(&hist[gi * cells + cell], (unsigned int)(q.w * 1000.0f));
+    const int mine = min(nrec, kBatchLaneSlots);
+    for (int r = 0; r < mine; ++r) repair(p0 + slots[r * blockDim.x + threadIdx.x]);
+    if (ovf >= 0) {  // more undecided iterations than records: find the rest again
+      for (long long p = ovf; p < p1; p += step) {
+        const bool sure = ng == 1 ? iteration(p, false, std::integral_constant<int, 2>{})
+                                  : iteration(p, false, std::integral_constant<int, 1>{});
+        if (!sure) repair(p);
+      }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
       const unsigned int v = hist[i];
-      if (v) atomicAdd(&dst[i], (unsigned long long)v);
+      if (v) {
+        atomicAdd(&dst[i], (unsigned long long)v);
+        hist[i] = 0u;
+      }
     }
-    __syncthreads();  // LDS, `cur` and the queue are reused by the next task
+    __syncthreads();  // LDS and `cur` are reused by the next task
   }
 }
 
@@ -215,7 +198,8 @@ hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const D
                                 unsigned long long* grids, unsigned long long* counter,
                                 int threads, hipStream_t s) {
   if (ntasks <= 0) return hipSuccess;
-  const size_t lds = (size_t)kBatchLdsCells * sizeof(unsigned int);
+  const size_t lds = (size_t)kBatchLdsCells * sizeof(unsigned int) +
+                     (size_t)kBatchLaneSlots * threads * sizeof(unsigned int);
   // the request is the dynamic size itself: the kernel also has static LDS,
   // and static + dynamic may not exceed the 160 KB of a CU
   hipError_t e = hipFuncSetAttribute((const void*)batch_deposit_kernel,
@@ -258,21 +242,24 @@ __global__ __launch_bounds__(kBatchGatherBlock) void batch_gather_kernel(
     constexpr int U = 4;
     int gr = 0;
     for (; gr + U <= P.k; gr += U) {
+      // one compare per group: the largest face distance of its U pairs
       int cell[U];
-      bool ok[U];
-      bool rare = false;
+      float d[U];
+      float dmax = 0.0f;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        ok[u] = cell_fast(c0, c1, c2, FR + 12 * (gr + u), P.lim, P.lg, &cell[u]) & pfast;
-        rare |= !ok[u];
+        const float* Fr = FR + 12 * (gr + u);
+        d[u] = cell_fast_d(c0, c1, c2, Fr, Fr[3], Fr[7], Fr[11], P.lg, &cell[u]);
+        dmax = fmaxf(dmax, d[u]);
       }
+      const bool rare = !(pfast & (dmax < P.lim));
       float4 f[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) f[u] = cell_load(F + (long long)(gr + u) * P.cells, cell[u]);
       if (__ballot(rare)) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          if (!ok[u]) {
+          if (!(pfast & (d[u] < P.lim))) {
             const int ex = cell_exact_call(c0, c1, c2, R + 12 * (gr + u), O + 4 * (gr + u), P.W,
                                            P.half, P.h, P.lg);
             f[u] = cell_load(F + (long long)(gr + u) * P.cells, ex);
