@@ -40,10 +40,23 @@ constexpr int oW1 = 0, ob1 = oW1 + kIn * kH, oW2 = ob1 + kH, ob2 = oW2 + kH * kH
               kParams = ob4 + kOut;
 constexpr int TILE = 64;  // particles per tile
 
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + expf(-z)); }
-__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+// sigmoid with the precise exp and IEEE division (FP32 operands), or with
+// the hardware exp2 and reciprocal, about 1 ulp each (BF16 operands: the
+// activation is rounded to bf16 next).  The builtins are written out because
+// this library is compiled with correctly rounded division, which would
+// expand a quotient into the full IEEE sequence.
+template <bool FAST>
+__device__ __forceinline__ float sigm(float z) {
+  if (FAST) return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * z));
+  return 1.0f / (1.0f + expf(-z));
+}
+template <bool FAST>
+__device__ __forceinline__ float silu(float z) {
+  return z * sigm<FAST>(z);
+}
+template <bool FAST>
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = sigm(z);
+  const float s = sigm<FAST>(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -53,10 +66,11 @@ struct Mat {
   const float* p;
   int rs, cs, nr, nc;
   bool act;
+  template <bool FAST>
   __device__ float at(int r, int c) const {
     if (r >= nr || c >= nc) return 0.0f;
     const float v = p[r * rs + c * cs];
-    return act ? silu(v) : v;
+    return act ? silu<FAST>(v) : v;
   }
 };
 
@@ -78,15 +92,16 @@ __device__ void wg_gemm(const Mat A, const Mat B, float* C, int ldc, int M, int 
         const int kb = k0 + 8 * (lane >> 4);
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          a[j] = (__bf16)A.at(ar, kb + j);
-          b[j] = (__bf16)B.at(kb + j, bc);
+          a[j] = (__bf16)A.template at<true>(ar, kb + j);
+          b[j] = (__bf16)B.template at<true>(kb + j, bc);
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
       }
     } else {
       for (int k0 = 0; k0 < K; k0 += 4) {
         const int k = k0 + (lane >> 4);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A.at(ar, k), B.at(k, bc), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A.template at<false>(ar, k),
+                                                  B.template at<false>(k, bc), acc, 0, 0, 0);
       }
     }
 #pragma unroll
@@ -478,7 +493,7 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const float* __restrict_
       for (int r = 0; r < TILE; ++r) s += DY[r * 16 + i];
       G[ob4 + i] += s;
     }
-    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) D1[i] *= dsilu(Z3[i]);
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) D1[i] *= dsilu<BF16>(Z3[i]);
     __syncthreads();
     // layer 3: dW3 += A2^T D1, db3 += sum D1, D2 = (D1 W3^T) * silu'(Z2)
     wg_gemm<BF16>(Mat{Z2, 1, kH, kH, TILE, true}, Mat{D1, kH, 1, TILE, kH, false}, G + oW3, kH, kH,
@@ -491,7 +506,7 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const float* __restrict_
       for (int r = 0; r < TILE; ++r) s += D1[r * kH + i];
       G[ob3 + i] += s;
     }
-    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) D2[i] *= dsilu(Z2[i]);
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) D2[i] *= dsilu<BF16>(Z2[i]);
     __syncthreads();
     // layer 2: dW2 += A1^T D2, db2 += sum D2, D1 = (D2 W2^T) * silu'(Z1)
     wg_gemm<BF16>(Mat{Z1, 1, kH, kH, TILE, true}, Mat{D2, kH, 1, TILE, kH, false}, G + oW2, kH, kH,
@@ -504,7 +519,7 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const float* __restrict_
       for (int r = 0; r < TILE; ++r) s += D2[r * kH + i];
       G[ob2 + i] += s;
     }
-    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) D1[i] *= dsilu(Z1[i]);
+    for (int i = threadIdx.x; i < TILE * kH; i += blockDim.x) D1[i] *= dsilu<BF16>(Z1[i]);
     __syncthreads();
     // layer 1: dW1 += X^T D1, db1 += sum D1
     wg_gemm<BF16>(Mat{X, 1, kXS, kIn, TILE, false}, Mat{D1, kH, 1, TILE, kH, false}, G + oW1, kH, 16,
