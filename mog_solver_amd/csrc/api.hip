@@ -366,6 +366,7 @@ Params make_params(const mog_config* c) {
   P.fk = (float)c->n_grids;
   P.CB = (float)(1.5 * c->world_size);
   P.lim = -1.0f;  // set with the transforms
+  P.shift_ok = 0;
   return P;
 }
 
@@ -913,6 +914,7 @@ int mog_set_transforms(mog_ctx* ctx, const float* rot_k12, const float* off_k4) 
   if (e != hipSuccess) return hip_fail(ctx, e, "mog_set_transforms");
   // The fast path is only admitted with a small margin (sane transforms).
   P.lim = margin < 0.05 ? (float)(0.5 - margin) : -1.0f;
+  P.shift_ok = shift_flags(ctx->cfg.world_size, P.g, P.lg, rmax, omax);
   drop_graphs(ctx);  // kernel arguments of a captured step changed
   ctx->have_transforms = true;
   return MOG_OK;
@@ -1636,6 +1638,7 @@ int mog_batch_set_transforms(mog_batch* b, int32_t sim, const float* rot_k12, co
     make_fast_row(rot_k12 + 12 * i, off_k4 + 4 * i, sm.P.half, sm.P.h, frot.data() + 12 * i);
   const double margin = fast_margin(b->cfg[sim].world_size, sm.P.g, rmax, omax);
   sm.P.lim = margin < 0.05 ? (float)(0.5 - margin) : -1.0f;
+  sm.P.shift_ok = shift_flags(b->cfg[sim].world_size, sm.P.g, sm.P.lg, rmax, omax);
   const size_t g0 = (size_t)sm.grid_off;
   hipError_t e = hipMemcpy(b->d_rot + 12 * g0, rot_k12, (size_t)k * 12 * sizeof(float),
                            hipMemcpyHostToDevice);

@@ -100,6 +100,94 @@ MOG_HD float cell_fast_d(float c0, float c1, float c2, const float* F, float a0,
   return fmaxf(fmaxf(fabsf(w0 - r0), fabsf(w1 - r1)), fabsf(w2 - r2));
 }
 
+// Shifted cell fields.  For an integer r with |r| < 2^(22 - s), the float32
+// sum u = r + 1.5 * 2^(23 - s) is exact (u lies in [2^(23-s), 2^(24-s)),
+// where the spacing is 2^-s), and the low 23 bits of its encoding are
+// r * 2^s + 2^22: the integer sits at bit s (two's complement, zeros
+// below).  With s_m = s0 + m lg for the three axes the coordinates of the
+// cell land in disjoint fields, and two bit selects and a mask give
+// cell << s0: the byte offset of the cell in an array of 2^s0-byte
+// elements.  The adds and bit selects issue at the full vector rate on
+// gfx950, where shifts and shift-adds take twice as long.
+struct FieldShift {
+  float k0, k1, k2;         // 1.5 * 2^(23 - s_m)
+  unsigned m0, m01, m012;   // bits below s0 + lg, s0 + 2 lg, s0 + 3 lg
+};
+
+MOG_HD float pow2_1p5(int e) {  // 1.5 * 2^e, |e| < 127
+  const unsigned b = ((unsigned)(127 + e) << 23) | 0x400000u;
+  float f;
+  memcpy(&f, &b, 4);
+  return f;
+}
+
+MOG_HD FieldShift field_shift(int lg, int s0) {
+  FieldShift S;
+  S.k0 = pow2_1p5(23 - s0);
+  S.k1 = pow2_1p5(23 - s0 - lg);
+  S.k2 = pow2_1p5(23 - s0 - 2 * lg);
+  S.m0 = (1u << (s0 + lg)) - 1u;
+  S.m01 = (1u << (s0 + 2 * lg)) - 1u;
+  S.m012 = (1u << (s0 + 3 * lg)) - 1u;
+  return S;
+}
+
+// Bound on |w_m| of the fast path for admitted particles (|c_j| <= 1.5 W)
+// of transforms with |R| <= rmax, |off| <= omax: |F| <= rmax / h and
+// |F[4m+3]| <= (W/2 + omax) / h + 1/2, each up to one rounding, and three
+// FMAs of relative error 2^-24 each; 1.0 covers the 1/2 and the slack.
+inline double fast_w_bound(double W, int g, double rmax, double omax) {
+  const double h = W / g;
+  return (3.0 * rmax * 1.5 * W + 0.5 * W + omax) / h * (1.0 + 1e-6) + 1.0;
+}
+
+// Whether the shifted fields hold for every admitted particle: |rint(w)| <=
+// wbound + 1/2 must stay below 2^(22 - s0 - 2 lg), and the three fields
+// below bit 22.
+inline bool field_shift_ok(int lg, int s0, double wbound) {
+  return s0 + 3 * lg <= 22 && wbound + 0.5 < ldexp(1.0, 22 - s0 - 2 * lg);
+}
+
+// Bit operations with three operands, one v_bitop3_b32 each on the device
+// (truth-table immediates: operands a, b, c select table bits 0xf0, 0xcc,
+// 0xaa).  bit_select(a, b, m) = (a & m) | (b & ~m); and_or(a, m, c) =
+// (a & m) | c.
+MOG_HD unsigned bit_select(unsigned a, unsigned b, unsigned m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, m, 0xE4);
+#else
+  return (a & m) | (b & ~m);
+#endif
+}
+
+MOG_HD unsigned and_or(unsigned a, unsigned m, unsigned c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, m, c, 0xEA);
+#else
+  return (a & m) | c;
+#endif
+}
+
+// cell_fast_d with the cell returned as the byte offset (cell << s0) | base
+// (S from field_shift(lg, s0), valid under field_shift_ok; base has no bits
+// below s0 + 3 lg).  Same cell, same d; (cell << s0) < g^3 << s0 for every
+// input.
+MOG_HD float cell_fast_o(float c0, float c1, float c2, const float* F, float a0, float a1, float a2,
+                         const FieldShift& S, unsigned base, unsigned* off) {
+  const float w0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, a0)));
+  const float w1 = fmaf(F[6], c2, fmaf(F[5], c1, fmaf(F[4], c0, a1)));
+  const float w2 = fmaf(F[10], c2, fmaf(F[9], c1, fmaf(F[8], c0, a2)));
+  const float r0 = rintf(w0), r1 = rintf(w1), r2 = rintf(w2);
+  const float u0 = r0 + S.k0, u1 = r1 + S.k1, u2 = r2 + S.k2;
+  unsigned b0, b1, b2;
+  memcpy(&b0, &u0, 4);
+  memcpy(&b1, &u1, 4);
+  memcpy(&b2, &u2, 4);
+  const unsigned x = bit_select(bit_select(b0, b1, S.m0), b2, S.m01);
+  *off = and_or(x, S.m012, base);
+  return fmaxf(fmaxf(fabsf(w0 - r0), fabsf(w1 - r1)), fabsf(w2 - r2));
+}
+
 // Fast cell; returns 0 when the exact path must decide (lim = 0.5 - M).
 // With w = v - 0.5 (the -0.5 is folded into F[4m+3]) and r = rint(w):
 // floor(v) = r.  Callers admit only finite coordinates, so no NaN reaches

@@ -34,6 +34,8 @@ hipError_t launch_fill_zero(long long* p, long long count, hipStream_t s) {
   return hipGetLastError();
 }
 
+typedef __attribute__((address_space(3))) unsigned int lds_u32;
+
 // Deferred-pair records per lane of the LDS deposit (see deposit_lds_kernel).
 constexpr int kLaneSlots = 4;
 // UP particles per lane and register set in the LDS deposit.
@@ -65,7 +67,11 @@ constexpr int kDepositUP = 2;
 // run a double-buffered loop: two register sets of UP particles each, one
 // processed while the other loads, with unguarded loads at 32-bit byte
 // offsets from the chunk's base.  The rest run the guarded loop.
-template <int G>
+//
+// SH: the fast cells come as LDS byte offsets through the shifted fields of
+// cellmath.h (cell_fast_o), for launches whose transforms satisfy
+// field_shift_ok(lg, 2, ...); otherwise as indices (cell_fast_d).
+template <int G, bool SH>
 __global__ __launch_bounds__(1024) void deposit_lds_kernel(
     const float4* __restrict__ pos, long long n, long long chunk,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
@@ -88,8 +94,13 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
     const int gr = min(grid0 + gi, P.k - 1);
 #pragma unroll
     for (int j = 0; j < 12; ++j) F[gi][j] = frot[12 * gr + j];
+    if constexpr (SH && G <= 4) {  // all 12 in vector registers: full-rate FMAs
 #pragma unroll
-    for (int m = 0; m < 3; ++m) F[gi][4 * m + 3] = to_vgpr(F[gi][4 * m + 3]);
+      for (int j = 0; j < 12; ++j) F[gi][j] = to_vgpr(F[gi][j]);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) F[gi][4 * m + 3] = to_vgpr(F[gi][4 * m + 3]);
+    }
   }
   __syncthreads();
 
@@ -100,15 +111,35 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
   constexpr int UP = kDepositUP;
 
   // The 32-bit milli-mass the loop adds for a particle: uint(m * 1000) when
-  // the product is in [1, 2^32), clamped below 2^32 above, 0 below 1 and for
-  // NaN; `small` when it is the true milli-mass and fits the partials.
-  // (lds_limit32 <= (2^32 - 1) / 256, below the clamp 4294967040, so a
-  // clamped value is never small; fmaxf takes NaN to 0)
+  // the product is in [1, 2^32), 2^32 - 1 from 2^32 on, 0 below 1 and for
+  // NaN (the saturating conversion of v_cvt_u32_f32); `small` when it is the
+  // true milli-mass and fits the partials (lds_limit32 <= (2^32 - 1) / 256,
+  // so a saturated value is never small).
   auto milli32 = [&](float w, bool& small) -> unsigned int {
     const float mm = w * 1000.0f;
-    const unsigned int m = (unsigned int)fminf(fmaxf(mm, 0.0f), 4294967040.0f);
+    unsigned int m;
+    asm("v_cvt_u32_f32 %0, %1" : "=v"(m) : "v"(mm));
     small = m <= lds_limit32;
     return m;
+  };
+  FieldShift FS = field_shift(P.lg, 2);
+  float half = P.half;
+  if constexpr (SH) {  // vector-register operands issue at the full rate
+    FS.k1 = to_vgpr(FS.k1);
+    FS.k2 = to_vgpr(FS.k2);
+    FS.m0 = to_vgpr(FS.m0);
+    FS.m01 = to_vgpr(FS.m01);
+    FS.m012 = to_vgpr(FS.m012);
+    half = to_vgpr(half);
+  }
+  // pair adds: (grid gi, fast cell c) += v; under SH, c is the LDS address
+  // of the counter itself (hist starts at LDS address 0: the kernel holds
+  // no static LDS, which the launch checks)
+  auto add_at = [&](int gi, unsigned c, unsigned v) {
+    if constexpr (SH)
+      __hip_atomic_fetch_add((lds_u32*)(size_t)c, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+      atomicAdd(hist + gi * cells + c, v);
   };
 
   // Repair the pairs of the UP particles of one record (first particle
@@ -154,16 +185,16 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
   // The UP x G pairs of UP particles: their milli-masses and fast cells,
   // every cell computed before any is used (no branch between them); true
   // when every pair is decided.
-  auto cells_of = [&](const float4* q, unsigned int (&m32)[UP], int (&cell)[UP][G]) -> bool {
+  auto cells_of = [&](const float4* q, unsigned int (&m32)[UP], unsigned (&cell)[UP][G]) -> bool {
     float cc[UP][3];
     bool sure = true;  // flags combine with bitwise &: no short-circuit branches
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
       bool small;
       m32[u] = milli32(q[u].w, small);
-      cc[u][0] = q[u].x - P.half;
-      cc[u][1] = q[u].y - P.half;
-      cc[u][2] = q[u].z - P.half;
+      cc[u][0] = q[u].x - half;
+      cc[u][1] = q[u].y - half;
+      cc[u][2] = q[u].z - half;
       sure = sure & small & (fabsf(cc[u][0]) <= P.CB) & (fabsf(cc[u][1]) <= P.CB) &
              (fabsf(cc[u][2]) <= P.CB);
     }
@@ -171,9 +202,17 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
 #pragma unroll
     for (int u = 0; u < UP; ++u)
 #pragma unroll
-      for (int gi = 0; gi < G; ++gi)
-        dmax = fmaxf(dmax, cell_fast_d(cc[u][0], cc[u][1], cc[u][2], F[gi], F[gi][3], F[gi][7],
-                                       F[gi][11], P.lg, &cell[u][gi]));
+      for (int gi = 0; gi < G; ++gi) {
+        if constexpr (SH) {
+          dmax = fmaxf(dmax, cell_fast_o(cc[u][0], cc[u][1], cc[u][2], F[gi], F[gi][3], F[gi][7],
+                                         F[gi][11], FS, (unsigned)(gi * cells * 4), &cell[u][gi]));
+        } else {
+          int c;
+          dmax = fmaxf(dmax, cell_fast_d(cc[u][0], cc[u][1], cc[u][2], F[gi], F[gi][3], F[gi][7],
+                                         F[gi][11], P.lg, &c));
+          cell[u][gi] = (unsigned)c;
+        }
+      }
     return sure & (dmax < P.lim);
   };
 
@@ -184,13 +223,13 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
   long long ovf = -1;
   auto pairs = [&](const float4* q, long long pl) {
     unsigned int m32[UP];
-    int cell[UP][G];
+    unsigned cell[UP][G];
     const bool sure = cells_of(q, m32, cell);
     // grids gi >= ng (last group) add to allocated, unflushed counters
 #pragma unroll
     for (int u = 0; u < UP; ++u)
 #pragma unroll
-      for (int gi = 0; gi < G; ++gi) atomicAdd(&hist[gi * cells + cell[u][gi]], m32[u]);
+      for (int gi = 0; gi < G; ++gi) add_at(gi, cell[u][gi], m32[u]);
     if (!sure) {  // rare
       if (nrec < kLaneSlots)
         slots[nrec * blockDim.x + threadIdx.x] = (unsigned int)pl;
@@ -255,7 +294,7 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
         q[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
       unsigned int m32[UP];
-      int cell[UP][G];
+      unsigned cell[UP][G];
       if (!cells_of(q, m32, cell)) repair(p0 + pl);
     }
   }
@@ -405,16 +444,27 @@ static hipError_t launch_lds(const Params& P, const float4* pos, long long n, lo
   const int groups = (P.k + G - 1) / G;
   const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)chunk;
   const size_t lds = (size_t)G * P.cells * 4 + (sorted ? 0 : (size_t)kLaneSlots * threads * 4);
-  const void* fn = !sorted ? (const void*)deposit_lds_kernel<G>
-                   : cells_out ? (const void*)deposit_sorted_kernel<G, true>
-                               : (const void*)deposit_sorted_kernel<G, false>;
+  bool sh = P.shift_ok & kShiftLds;
+  if (sh && !sorted) {  // the shifted path addresses its counters from LDS address 0
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, (const void*)deposit_lds_kernel<G, true>);
+    if (e != hipSuccess) return e;
+    sh = fa.sharedSizeBytes == 0;
+  }
+  const void* fn = sorted ? (cells_out ? (const void*)deposit_sorted_kernel<G, true>
+                                       : (const void*)deposit_sorted_kernel<G, false>)
+                   : sh   ? (const void*)deposit_lds_kernel<G, true>
+                          : (const void*)deposit_lds_kernel<G, false>;
   // the exact dynamic size: the kernels also hold static LDS
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   const dim3 grid((unsigned)(chunks * groups));
-  if (!sorted)
-    hipLaunchKernelGGL(deposit_lds_kernel<G>, grid, dim3(threads), lds, s, pos, n, chunk, rot, off,
-                       frot, P, lds_limit, grids);
+  if (!sorted && sh)
+    hipLaunchKernelGGL((deposit_lds_kernel<G, true>), grid, dim3(threads), lds, s, pos, n, chunk,
+                       rot, off, frot, P, lds_limit, grids);
+  else if (!sorted)
+    hipLaunchKernelGGL((deposit_lds_kernel<G, false>), grid, dim3(threads), lds, s, pos, n, chunk,
+                       rot, off, frot, P, lds_limit, grids);
   else if (cells_out)
     hipLaunchKernelGGL((deposit_sorted_kernel<G, true>), grid, dim3(threads), lds, s, pos, n, chunk,
                        rot, off, frot, P, lds_limit, grids, cells_out);

@@ -31,7 +31,11 @@ __device__ __forceinline__ void accumulate_rotated(const float* __restrict__ R, 
 // chain reads them as vector operands (ds_read) instead of re-copying them
 // from scalar registers with three v_mov per pair; used when k * 12 bytes
 // fit the launch's LDS budget.
-template <bool WORLD, bool LDS3>
+//
+// SH: the fast cells come as byte offsets into the float4 grid through the
+// shifted fields of cellmath.h (cell_fast_o), for transforms that satisfy
+// field_shift_ok(lg, 4, ...); otherwise as indices (cell_fast_d).
+template <bool WORLD, bool LDS3, bool SH>
 __global__ __launch_bounds__(256) void gather_update_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, long long n,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
@@ -56,6 +60,15 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
   const float c1 = p.y - P.half;
   const float c2 = p.z - P.half;
   const int pfast = particle_fast(c0, c1, c2, P.CB);
+  FieldShift FS = field_shift(P.lg, 4);
+  if constexpr (SH) {  // vector-register operands issue at the full rate
+    FS.k0 = to_vgpr(FS.k0);
+    FS.k1 = to_vgpr(FS.k1);
+    FS.k2 = to_vgpr(FS.k2);
+    FS.m0 = to_vgpr(FS.m0);
+    FS.m01 = to_vgpr(FS.m01);
+    FS.m012 = to_vgpr(FS.m012);
+  }
   float tx = 0.f, ty = 0.f, tz = 0.f;
   int grid = 0;
   for (; grid + U <= P.k; grid += U) {
@@ -64,22 +77,29 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
     // those pairs from the exact cells (the exact sequence is a call, whose
     // entry waits for every load in flight)
     // one compare per group: the largest face distance of its U pairs
-    int cell[U];
+    unsigned cell[U];  // byte offsets under SH
     float d[U];
     float dmax = 0.0f;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int gr = grid + u;
       const float* Fr = frot + 12 * gr;
-      d[u] = LDS3 ? cell_fast_d(c0, c1, c2, Fr, adds[3 * gr], adds[3 * gr + 1], adds[3 * gr + 2],
-                                P.lg, &cell[u])
-                  : cell_fast_d(c0, c1, c2, Fr, Fr[3], Fr[7], Fr[11], P.lg, &cell[u]);
+      const float a0 = LDS3 ? adds[3 * gr] : Fr[3];
+      const float a1 = LDS3 ? adds[3 * gr + 1] : Fr[7];
+      const float a2 = LDS3 ? adds[3 * gr + 2] : Fr[11];
+      if constexpr (SH) {
+        d[u] = cell_fast_o(c0, c1, c2, Fr, a0, a1, a2, FS, 0u, &cell[u]);
+      } else {
+        int c;
+        d[u] = cell_fast_d(c0, c1, c2, Fr, a0, a1, a2, P.lg, &c);
+        cell[u] = (unsigned)c << 4;
+      }
       dmax = fmaxf(dmax, d[u]);
     }
     const bool rare = !(pfast & (dmax < P.lim));
     float4 F[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) F[u] = cell_load(force + (long long)(grid + u) * P.cells, cell[u]);
+    for (int u = 0; u < U; ++u) F[u] = cell_load_b(force + (long long)(grid + u) * P.cells, cell[u]);
     if (__ballot(rare)) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -196,22 +216,20 @@ hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long 
   const long long blocks = (n + 255) / 256;
   const size_t lds = (size_t)P.k * 3 * sizeof(float);
   const bool lds3 = lds <= 16 * 1024;  // k <= 1365: a few KB per workgroup
-  if (world && lds3)
-    hipLaunchKernelGGL((gather_update_kernel<true, true>), dim3((unsigned)blocks), dim3(256), lds, s,
-                       pos, vel, n, rot, off, frot, force, P, avg_out, mode, order, zero,
-                       zero_count);
-  else if (world)
-    hipLaunchKernelGGL((gather_update_kernel<true, false>), dim3((unsigned)blocks), dim3(256), 0, s,
-                       pos, vel, n, rot, off, frot, force, P, avg_out, mode, order, zero,
-                       zero_count);
-  else if (lds3)
-    hipLaunchKernelGGL((gather_update_kernel<false, true>), dim3((unsigned)blocks), dim3(256), lds,
-                       s, pos, vel, n, rot, off, frot, force, P, avg_out, mode, order, zero,
-                       zero_count);
-  else
-    hipLaunchKernelGGL((gather_update_kernel<false, false>), dim3((unsigned)blocks), dim3(256), 0,
-                       s, pos, vel, n, rot, off, frot, force, P, avg_out, mode, order, zero,
-                       zero_count);
+  const bool sh = P.shift_ok & kShiftF4;
+  const dim3 grid((unsigned)blocks);
+#define MOG_GATHER(W3, L3, S3)                                                                    \
+  hipLaunchKernelGGL((gather_update_kernel<W3, L3, S3>), grid, dim3(256), L3 ? lds : 0, s, pos, vel, \
+                     n, rot, off, frot, force, P, avg_out, mode, order, zero, zero_count)
+  if (world && lds3 && sh) MOG_GATHER(true, true, true);
+  else if (world && lds3) MOG_GATHER(true, true, false);
+  else if (world && sh) MOG_GATHER(true, false, true);
+  else if (world) MOG_GATHER(true, false, false);
+  else if (lds3 && sh) MOG_GATHER(false, true, true);
+  else if (lds3) MOG_GATHER(false, true, false);
+  else if (sh) MOG_GATHER(false, false, true);
+  else MOG_GATHER(false, false, false);
+#undef MOG_GATHER
   return hipGetLastError();
 }
 

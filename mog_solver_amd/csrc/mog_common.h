@@ -27,7 +27,19 @@ struct Params {
   float fk;    // float(k), the divisor of update.glsl:81
   float CB;    // fast-path admission bound on |c_j| (1.5 W)
   float lim;   // fast-path acceptance: |frac(v) - 0.5| < lim (0.5 - margin); < 0 disables
+  int shift_ok;  // kShiftLds | kShiftF4: shifted cell fields valid (cellmath.h field_shift_ok)
 };
+
+// Params::shift_ok bits: the shifted fields hold for byte offsets of 4-byte
+// (LDS counters, s0 = 2) and 16-byte (float4 forces, s0 = 4) elements.
+constexpr int kShiftLds = 1;
+constexpr int kShiftF4 = 2;
+
+// shift_ok for transforms with |R| <= rmax and |off| <= omax.
+inline int shift_flags(double W, int g, int lg, double rmax, double omax) {
+  const double wb = fast_w_bound(W, g, rmax, omax);
+  return (field_shift_ok(lg, 2, wb) ? kShiftLds : 0) | (field_shift_ok(lg, 4, wb) ? kShiftF4 : 0);
+}
 
 // The exact float32 sequence, kept out of line (one copy per translation
 // unit): it runs for a few percent of (particle, grid) pairs only.
@@ -42,10 +54,19 @@ static __device__ __attribute__((noinline)) int cell_exact_call(float c0, float 
 // A wave-uniform value copied into a vector register.  A VOP3 instruction
 // reads at most one scalar operand on this target, so an FMA whose
 // multiplier and addend are both wave-uniform would otherwise re-copy the
-// addend with a v_mov on every use; a kernel that keeps a fast row for a
-// whole particle loop copies its three addends once.
+// addend with a v_mov on every use.  And on gfx950 a vector instruction
+// with a scalar-register operand issues at half the rate of one whose
+// operands are vector registers or literals (about 4.3 against 2.7 cycles
+// per wave64 add, FMA or bit operation, measured), so the hot loops keep
+// their wave-uniform coefficients in vector registers.
 __device__ __forceinline__ float to_vgpr(float x) {
   float y;
+  asm("v_mov_b32 %0, %1" : "=v"(y) : "s"(x));
+  return y;
+}
+
+__device__ __forceinline__ unsigned to_vgpr(unsigned x) {
+  unsigned y;
   asm("v_mov_b32 %0, %1" : "=v"(y) : "s"(x));
   return y;
 }
@@ -69,6 +90,12 @@ __device__ __forceinline__ int cell_index(float c0, float c1, float c2, int pfas
 __device__ __forceinline__ float4 cell_load(const float4* __restrict__ grid, int cell) {
   return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(grid) +
                                           ((unsigned)cell << 4));
+}
+
+// The float4 at byte offset `off` (< 2^32) of a grid: wave-uniform base,
+// 32-bit vector offset.
+__device__ __forceinline__ float4 cell_load_b(const float4* __restrict__ grid, unsigned off) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(grid) + off);
 }
 
 // Position of the cached cell of (slot t, grid) in the uint16 cell cache:

@@ -3,8 +3,11 @@
 // For random rotations/offsets and for positions drawn both uniformly and
 // adversarially on cell faces (a local coordinate j*h + tiny, mapped back to
 // world space and rounded to float32), every position the fast path accepts
-// must give the exact path's cell.  Prints "mismatches fastrate" and exits
-// non-zero on any mismatch.  Usage: cellmath_check [samples] [margin_scale]
+// must give the exact path's cell.  Every admitted position must also give
+// the same cell through the shifted fields (cell_fast_o, byte offsets for
+// 4- and 16-byte elements) wherever field_shift_ok admits them.  Prints
+// "mismatches fastrate total shifted_checked" and exits non-zero on any
+// mismatch.  Usage: cellmath_check [samples] [margin_scale]
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -58,7 +61,7 @@ int main(int argc, char** argv) {
   const double mscale = argc > 2 ? atof(argv[2]) : 1.0;
   const double Ws[] = {100.0, 2000.0, 10.0, 1.0, 37.3};
   const int gs[] = {2, 4, 8, 16, 32, 64};
-  long long bad = 0, fast = 0, total = 0;
+  long long bad = 0, fast = 0, total = 0, shifted = 0;
   for (double W : Ws)
     for (int g : gs) {
       int lg = 0;
@@ -73,6 +76,9 @@ int main(int argc, char** argv) {
         for (int i = 0; i < 3; ++i) omax = fmax(omax, fabs((double)G.O[i]));
         const double M = mog::fast_margin(W, g, rmax, omax) * mscale;
         const float lim = (float)(0.5 - M);
+        const double wb = mog::fast_w_bound(W, g, rmax, omax);
+        const bool sh2 = mog::field_shift_ok(lg, 2, wb), sh4 = mog::field_shift_ok(lg, 4, wb);
+        const mog::FieldShift S2 = mog::field_shift(lg, 2), S4 = mog::field_shift(lg, 4);
         for (long long s = 0; s < samples / 240; ++s) {
           float p[3];
           if (s & 1) {
@@ -96,6 +102,23 @@ int main(int argc, char** argv) {
           int cf = -1;
           const int ok = mog::cell_fast(c0, c1, c2, G.F, lim, lg, &cf) & mog::particle_fast(c0, c1, c2, CB);
           ++total;
+          if (mog::particle_fast(c0, c1, c2, CB)) {
+            const float dref = mog::cell_fast_d(c0, c1, c2, G.F, G.F[3], G.F[7], G.F[11], lg, &cf);
+            for (int v = 0; v < 2; ++v) {
+              if (!(v ? sh4 : sh2)) continue;
+              const int s0 = v ? 4 : 2;
+              unsigned off = ~0u;
+              const unsigned base = (unsigned)(t + 1) << (s0 + 3 * lg);
+              const float d = mog::cell_fast_o(c0, c1, c2, G.F, G.F[3], G.F[7], G.F[11], v ? S4 : S2, base, &off);
+              ++shifted;
+              if (off != (((unsigned)cf << s0) | base) || d != dref) {
+                if (bad < 10)
+                  fprintf(stderr, "shifted mismatch W=%g g=%d s0=%d p=(%.9g,%.9g,%.9g) cell=%d off=%u\n", W, g,
+                          s0, p[0], p[1], p[2], cf, off);
+                ++bad;
+              }
+            }
+          }
           if (ok) {
             ++fast;
             if (cf != ce) {
@@ -108,6 +131,6 @@ int main(int argc, char** argv) {
         }
       }
     }
-  printf("%lld %.6f %lld\n", bad, (double)fast / (double)total, total);
+  printf("%lld %.6f %lld %lld\n", bad, (double)fast / (double)total, total, shifted);
   return bad ? 1 : 0;
 }

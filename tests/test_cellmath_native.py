@@ -3,8 +3,10 @@
 Compiles tests/native/cellmath_check.cpp against the kernels' own
 mog_solver_amd/csrc/cellmath.h with g++ (-ffp-contract=off) and runs it on
 uniform and adversarial on-face positions for W in {100, 2000, 10, 1, 37.3}
-and g in {2..64}.  A run with the margin removed must report mismatches,
-which shows the check can fail.
+and g in {2..64}, and checks that the shifted-field form of the fast cell
+(the byte offsets the LDS deposit and the gather use) gives the same cell
+wherever its range condition admits it.  A run with the margin removed must
+report mismatches, which shows the check can fail.
 """
 import subprocess
 from pathlib import Path
@@ -24,9 +26,10 @@ def checker(tmp_path_factory):
 
 def test_fast_cell_path_is_exact(checker):
     r = subprocess.run([str(checker), "20000000"], capture_output=True, text=True)
-    bad, rate, total = r.stdout.split()
+    bad, rate, total, shifted = r.stdout.split()
     assert r.returncode == 0 and int(bad) == 0, r.stderr
     assert int(total) > 19_000_000 and float(rate) > 0.5
+    assert int(shifted) > 20_000_000
 
 
 def test_check_detects_missing_margin(checker):

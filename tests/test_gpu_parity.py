@@ -214,6 +214,34 @@ def test_gather_matches_fp64(torch_cuda):
     assert O.rel_rms(got[:, :3], want) < 1e-6
 
 
+@pytest.mark.parametrize("k,g,shift", [(3, 32, 40.0), (5, 16, 300.0)])
+def test_deposit_and_gather_with_far_offsets(torch_cuda, k, g, shift):
+    """Offsets moved by a multiple of W (same cells up to rounding, which the
+    exact sequence reproduces): the fast-path bound |w| then exceeds the
+    range of the shifted cell fields (2^(22 - s0 - 2 lg) with s0 = 2 for the
+    LDS deposit, 4 for the gather), so both kernels take their index form."""
+    torch = torch_cuda
+    W = 100.0
+    s, rot, off = make_solver(k, g, W)
+    off = off.copy()
+    off[:, :3] += np.float32(shift * W)
+    s.set_transforms(rot, off)
+    lg = int(np.log2(g))
+    wbound = (4.5 * W + 0.5 * W + float(np.abs(off[:, :3]).max())) / (W / g)
+    assert wbound + 1.5 > 2.0 ** (22 - 4 - 2 * lg)  # gather index form
+    if shift > 100:
+        assert wbound + 1.5 > 2.0 ** (22 - 2 - 2 * lg)  # deposit index form
+    pos = edge_positions(W, g, 20000, seed=k + g)
+    got = s.deposit(dev(torch, pos)).cpu().numpy()
+    want = O.deposit(pos, rot, off, O.Params.make(k, g, W))
+    assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} cells differ"
+    rng = np.random.default_rng(4)
+    force = rng.normal(size=(k, g, g, g, 4)).astype(np.float32)
+    gf = s.gather(dev(torch, pos), dev(torch, force)).cpu().numpy()
+    wf = O.gather(pos, rot, off, force.astype(np.float64), O.Params.make(k, g, W))
+    assert O.rel_rms(gf[:, :3], wf) < 1e-6
+
+
 def test_update_bit_exact(torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(9)
