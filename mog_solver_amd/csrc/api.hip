@@ -364,7 +364,10 @@ Params make_params(const mog_config* c) {
   P.dtp = (float)(c->dt * c->timescale);
   P.damping = (float)c->damping;
   P.fk = (float)c->n_grids;
-  P.CB = (float)(1.5 * c->world_size);
+  // Admission bound of the fast path: |c_j| <= W / 2 holds for every
+  // position the integrator's wrap leaves in [0, W]; a particle outside takes
+  // the exact sequence.  The error margin scales with this bound.
+  P.CB = P.half;
   P.lim = -1.0f;  // set with the transforms
   P.shift_ok = 0;
   return P;
@@ -906,7 +909,7 @@ int mog_set_transforms(mog_ctx* ctx, const float* rot_k12, const float* off_k4) 
   float* frot = (float*)malloc(k * 12 * sizeof(float));
   if (!frot) return fail(ctx, MOG_ENOMEM, "host allocation failed");
   for (size_t i = 0; i < k; ++i) make_fast_row(rot_k12 + 12 * i, off_k4 + 4 * i, P.half, P.h, frot + 12 * i);
-  const double margin = fast_margin(ctx->cfg.world_size, P.g, rmax, omax);
+  const double margin = fast_margin(ctx->cfg.world_size, P.g, rmax, omax, P.CB);
   hipError_t e = hipMemcpy(ctx->d_rot, rot_k12, k * 12 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ctx->d_off, off_k4, k * 4 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ctx->d_frot, frot, k * 12 * sizeof(float), hipMemcpyHostToDevice);
@@ -914,7 +917,7 @@ int mog_set_transforms(mog_ctx* ctx, const float* rot_k12, const float* off_k4) 
   if (e != hipSuccess) return hip_fail(ctx, e, "mog_set_transforms");
   // The fast path is only admitted with a small margin (sane transforms).
   P.lim = margin < 0.05 ? (float)(0.5 - margin) : -1.0f;
-  P.shift_ok = shift_flags(ctx->cfg.world_size, P.g, P.lg, rmax, omax);
+  P.shift_ok = shift_flags(ctx->cfg.world_size, P.g, P.lg, rmax, omax, P.CB);
   drop_graphs(ctx);  // kernel arguments of a captured step changed
   ctx->have_transforms = true;
   return MOG_OK;
@@ -1636,9 +1639,9 @@ int mog_batch_set_transforms(mog_batch* b, int32_t sim, const float* rot_k12, co
   std::vector<float> frot((size_t)k * 12);
   for (int i = 0; i < k; ++i)
     make_fast_row(rot_k12 + 12 * i, off_k4 + 4 * i, sm.P.half, sm.P.h, frot.data() + 12 * i);
-  const double margin = fast_margin(b->cfg[sim].world_size, sm.P.g, rmax, omax);
+  const double margin = fast_margin(b->cfg[sim].world_size, sm.P.g, rmax, omax, sm.P.CB);
   sm.P.lim = margin < 0.05 ? (float)(0.5 - margin) : -1.0f;
-  sm.P.shift_ok = shift_flags(b->cfg[sim].world_size, sm.P.g, sm.P.lg, rmax, omax);
+  sm.P.shift_ok = shift_flags(b->cfg[sim].world_size, sm.P.g, sm.P.lg, rmax, omax, sm.P.CB);
   const size_t g0 = (size_t)sm.grid_off;
   hipError_t e = hipMemcpy(b->d_rot + 12 * g0, rot_k12, (size_t)k * 12 * sizeof(float),
                            hipMemcpyHostToDevice);
@@ -1798,7 +1801,7 @@ int mog2d_create(int32_t n_grids, int32_t grid_size, double world_size, double G
     for (int j = 0; j < 12; ++j) rmax = fmax(rmax, fabs((double)rot_k12[12 * i + j]));
     for (int m = 0; m < 2; ++m) omax = fmax(omax, fabs((double)off_k4[4 * i + m]));
   }
-  const double margin = fast_margin(world_size, grid_size, rmax, omax);
+  const double margin = fast_margin(world_size, grid_size, rmax, omax, 1.5 * world_size);
   c->lim = margin < 0.05 ? (float)(0.5 - margin) : -1.0f;
   A(hipMemcpy(c->d_rot, rot_k12, (size_t)n_grids * 12 * sizeof(float), hipMemcpyHostToDevice));
   A(hipMemcpy(c->d_off, off_k4, (size_t)n_grids * 4 * sizeof(float), hipMemcpyHostToDevice));

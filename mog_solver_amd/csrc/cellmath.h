@@ -132,13 +132,13 @@ MOG_HD FieldShift field_shift(int lg, int s0) {
   return S;
 }
 
-// Bound on |w_m| of the fast path for admitted particles (|c_j| <= 1.5 W)
-// of transforms with |R| <= rmax, |off| <= omax: |F| <= rmax / h and
+// Bound on |w_m| of the fast path for admitted particles (|c_j| <= CB) of
+// transforms with |R| <= rmax, |off| <= omax: |F| <= rmax / h and
 // |F[4m+3]| <= (W/2 + omax) / h + 1/2, each up to one rounding, and three
 // FMAs of relative error 2^-24 each; 1.0 covers the 1/2 and the slack.
-inline double fast_w_bound(double W, int g, double rmax, double omax) {
+inline double fast_w_bound(double W, int g, double rmax, double omax, double CB) {
   const double h = W / g;
-  return (3.0 * rmax * 1.5 * W + 0.5 * W + omax) / h * (1.0 + 1e-6) + 1.0;
+  return (3.0 * rmax * CB + 0.5 * W + omax) / h * (1.0 + 1e-6) + 1.0;
 }
 
 // Whether the shifted fields hold for every admitted particle: |rint(w)| <=
@@ -217,8 +217,8 @@ MOG_HD int particle_fast(float c0, float c1, float c2, float CB) {
 }
 
 // Error margin M of the fast path (in cell units) for transforms whose
-// entries satisfy |R| <= rmax and |off| <= omax, particles with
-// |c_j| <= CB = 1.5 W, cell size h = W / g.  With u = 2^-24 and T = rmax*CB:
+// entries satisfy |R| <= rmax and |off| <= omax, particles admitted with
+// |c_j| <= CB, cell size h = W / g.  With u = 2^-24 and T = rmax*CB:
 //   exact path, l: products, partial sums, +W/2, -off     u (14 T + W + omax)
 //   exact path, mod chain (|floor| <= L/W, L = 3T + W/2 + omax)  u (L + 4 W)
 //   exact path, l / h                                   u g      (cell units)
@@ -226,10 +226,9 @@ MOG_HD int particle_fast(float c0, float c1, float c2, float CB) {
 //   fast path, three FMAs                               u (6 T + 3 (W/2 + omax + h/2))
 // M = 3 x (sum / h + u g); returns a value >= 0.05 when the fast path should
 // not be used.
-inline double fast_margin(double W, int g, double rmax, double omax) {
+inline double fast_margin(double W, int g, double rmax, double omax, double CB) {
   const double u = 1.0 / 16777216.0;
   const double h = W / g;
-  const double CB = 1.5 * W;
   const double T = rmax * CB;
   const double L = 3.0 * T + 0.5 * W + omax;
   const double exact_l = 14.0 * T + W + omax;

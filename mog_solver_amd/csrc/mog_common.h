@@ -25,7 +25,7 @@ struct Params {
   float W, half, h;
   float dtp, damping;
   float fk;    // float(k), the divisor of update.glsl:81
-  float CB;    // fast-path admission bound on |c_j| (1.5 W)
+  float CB;    // fast-path admission bound on |c_j| (W / 2: every wrapped position)
   float lim;   // fast-path acceptance: |frac(v) - 0.5| < lim (0.5 - margin); < 0 disables
   int shift_ok;  // kShiftLds | kShiftF4: shifted cell fields valid (cellmath.h field_shift_ok)
 };
@@ -36,8 +36,8 @@ constexpr int kShiftLds = 1;
 constexpr int kShiftF4 = 2;
 
 // shift_ok for transforms with |R| <= rmax and |off| <= omax.
-inline int shift_flags(double W, int g, int lg, double rmax, double omax) {
-  const double wb = fast_w_bound(W, g, rmax, omax);
+inline int shift_flags(double W, int g, int lg, double rmax, double omax, double CB) {
+  const double wb = fast_w_bound(W, g, rmax, omax, CB);
   return (field_shift_ok(lg, 2, wb) ? kShiftLds : 0) | (field_shift_ok(lg, 4, wb) ? kShiftF4 : 0);
 }
 

@@ -1,9 +1,10 @@
 // Host check of the fast NGP cell path against the exact float32 sequence.
 //
-// For random rotations/offsets and for positions drawn both uniformly and
-// adversarially on cell faces (a local coordinate j*h + tiny, mapped back to
-// world space and rounded to float32), every position the fast path accepts
-// must give the exact path's cell.  Every admitted position must also give
+// For random rotations/offsets, both admission bounds the library uses
+// (|c_j| <= W/2 for the 3D step, 1.5 W for the planar path), and positions
+// drawn both uniformly and adversarially on cell faces (a local coordinate
+// j*h + tiny, mapped back to world space and rounded to float32), every
+// position the fast path accepts must give the exact path's cell.  Every admitted position must also give
 // the same cell through the shifted fields (cell_fast_o, byte offsets for
 // 4- and 16-byte elements) wherever field_shift_ok admits them.  Prints
 // "mismatches fastrate total shifted_checked" and exits non-zero on any
@@ -62,27 +63,29 @@ int main(int argc, char** argv) {
   const double Ws[] = {100.0, 2000.0, 10.0, 1.0, 37.3};
   const int gs[] = {2, 4, 8, 16, 32, 64};
   long long bad = 0, fast = 0, total = 0, shifted = 0;
+  const double cbws[] = {0.5, 1.5};
+  for (double cbw : cbws)
   for (double W : Ws)
     for (int g : gs) {
       int lg = 0;
       while ((1 << lg) < g) ++lg;
       const float Wf = (float)W, half = Wf * 0.5f, h = (float)(W / g);
-      const float CB = (float)(1.5 * W);
+      const float CB = (float)(cbw * W);
       for (int t = 0; t < 8; ++t) {
         Grid G;
         make_grid(G, W, g);
         double rmax = 0, omax = 0;
         for (int i = 0; i < 12; ++i) rmax = fmax(rmax, fabs((double)G.R[i]));
         for (int i = 0; i < 3; ++i) omax = fmax(omax, fabs((double)G.O[i]));
-        const double M = mog::fast_margin(W, g, rmax, omax) * mscale;
+        const double M = mog::fast_margin(W, g, rmax, omax, (double)CB) * mscale;
         const float lim = (float)(0.5 - M);
-        const double wb = mog::fast_w_bound(W, g, rmax, omax);
+        const double wb = mog::fast_w_bound(W, g, rmax, omax, (double)CB);
         const bool sh2 = mog::field_shift_ok(lg, 2, wb), sh4 = mog::field_shift_ok(lg, 4, wb);
         const mog::FieldShift S2 = mog::field_shift(lg, 2), S4 = mog::field_shift(lg, 4);
         for (long long s = 0; s < samples / 240; ++s) {
           float p[3];
           if (s & 1) {
-            for (int d = 0; d < 3; ++d) p[d] = (float)(uni() * 3.0 * W - W);
+            for (int d = 0; d < 3; ++d) p[d] = (float)((uni() * 2.0 * cbw + 0.5 - cbw) * W * 1.25);
           } else {
             // local coordinates on or next to cell faces, mapped to world
             double loc[3];

@@ -2,8 +2,9 @@
 
 Compiles tests/native/cellmath_check.cpp against the kernels' own
 mog_solver_amd/csrc/cellmath.h with g++ (-ffp-contract=off) and runs it on
-uniform and adversarial on-face positions for W in {100, 2000, 10, 1, 37.3}
-and g in {2..64}, and checks that the shifted-field form of the fast cell
+uniform and adversarial on-face positions for W in {100, 2000, 10, 1, 37.3},
+g in {2..64} and the admission bounds |c| <= W/2 (3D step) and 1.5 W (planar
+path), and checks that the shifted-field form of the fast cell
 (the byte offsets the LDS deposit and the gather use) gives the same cell
 wherever its range condition admits it.  A run with the margin removed must
 report mismatches, which shows the check can fail.
@@ -25,11 +26,12 @@ def checker(tmp_path_factory):
 
 
 def test_fast_cell_path_is_exact(checker):
-    r = subprocess.run([str(checker), "20000000"], capture_output=True, text=True)
+    r = subprocess.run([str(checker), "10000000"], capture_output=True, text=True)
     bad, rate, total, shifted = r.stdout.split()
     assert r.returncode == 0 and int(bad) == 0, r.stderr
-    assert int(total) > 19_000_000 and float(rate) > 0.5
-    assert int(shifted) > 20_000_000
+    # both admission bounds; a third of the samples lie outside W/2 on purpose
+    assert int(total) > 19_000_000 and float(rate) > 0.25
+    assert int(shifted) > 15_000_000
 
 
 def test_check_detects_missing_margin(checker):
