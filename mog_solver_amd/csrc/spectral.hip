@@ -580,20 +580,15 @@ __device__ __forceinline__ void rf_fwd_xy(const R& res, long long t0, long long 
   }
 }
 
-// Pass B: line `line` of the launch = (slab t0 + line / G, kx = line % G).
-template <int G, typename R>
-__device__ __forceinline__ void rf_z_green(const R& res, long long t0, long long nlines,
-                                           long long line) {
+// The spectral multiply of one z line (ky, kx) of a grid's spectrum:
+// a[kz] (in: the density spectrum) becomes the packed y + i z force
+// spectrum, xs[kz] the x force spectrum; Green's function times i k
+// (update.glsl's gradient of the potential) or the convolution solver's two
+// precomputed kernels.
+template <int G>
+__device__ __forceinline__ void rf_multiply(const RfSlab& a0, int ky, int kx, float2 (&a)[G],
+                                            float2 (&xs)[G]) {
   constexpr int GG = G * G;
-  if (line >= nlines) return;
-  const RfSlab a0 = res(t0 + line / G);
-  const long long grid = a0.local / G;
-  const int ky = (int)(a0.local % G), kx = (int)(line % G);
-  const long long at0 = grid * G * GG + ky * G + kx;
-  float2 a[G], xs[G];
-#pragma unroll
-  for (int z = 0; z < G; ++z) a[z] = a0.S[at0 + (long long)z * GG];
-  reg_fft<G, false>(a);
   const float kyv = a0.kd[ky], kxv = a0.kd[kx];
 #pragma unroll
   for (int kz = 0; kz < G; ++kz) {
@@ -611,6 +606,23 @@ __device__ __forceinline__ void rf_z_green(const R& res, long long t0, long long
       xs[kz] = make_float2(-(pi * kxv), pr * kxv);
     }
   }
+}
+
+// Pass B: line `line` of the launch = (slab t0 + line / G, kx = line % G).
+template <int G, typename R>
+__device__ __forceinline__ void rf_z_green(const R& res, long long t0, long long nlines,
+                                           long long line) {
+  constexpr int GG = G * G;
+  if (line >= nlines) return;
+  const RfSlab a0 = res(t0 + line / G);
+  const long long grid = a0.local / G;
+  const int ky = (int)(a0.local % G), kx = (int)(line % G);
+  const long long at0 = grid * G * GG + ky * G + kx;
+  float2 a[G], xs[G];
+#pragma unroll
+  for (int z = 0; z < G; ++z) a[z] = a0.S[at0 + (long long)z * GG];
+  reg_fft<G, false>(a);
+  rf_multiply<G>(a0, ky, kx, a, xs);
   reg_fft<G, true>(a);
 #pragma unroll
   for (int z = 0; z < G; ++z) a0.S[at0 + (long long)z * GG] = a[z];
@@ -697,6 +709,125 @@ __global__ __launch_bounds__(256) void rf_inv_xy_kernel(R res, long long t0, lon
   rf_inv_xy<G>(res, t0, nslabs, world, blockIdx.x);
 }
 
+// The whole solve of one g = 16 grid in one workgroup (512 threads, the two
+// force fields of 16^3 complex values in LDS, 69.6 KB): load and convert the
+// density, forward FFT along x, y and z, spectral multiply and inverse z
+// (pass A and B's work), inverse y and x of both fields and the packed
+// float4 store (pass C's).  Each line is transformed by the same register
+// FFT, in the same order, as in the three passes, so the result is the
+// same; the spectrum never leaves LDS.  Lines: 256 per direction (one per
+// thread of the first half); the z pass and the inverse passes give the
+// second half the x field.  Block b takes grid g0 + b of `res`.
+template <int G, typename R>
+__global__ __launch_bounds__(512) void rf_grid_kernel(R res, long long g0, int world) {
+  static_assert(G * G == 256, "one line per thread of a half");
+  constexpr int GG = G * G, RS = G + 1, PL = G * RS;  // row and plane strides (float2)
+  extern __shared__ float2 sm[];
+  float2* const A = sm;            // [z][y][x]: the spectrum, then the y + i z field
+  float2* const B = sm + G * PL;   // the x field
+  const RfSlab a = res((g0 + (long long)blockIdx.x) * G);
+  const long long cell0 = a.local * GG;  // the grid's first cell: local = grid * G
+  const long long grid = a.local / G;
+  const int t = threadIdx.x, half = t >> 8, hi = (t >> 4) & (G - 1), lo = t & (G - 1);
+  const bool need_x = a.need_x;
+  // density -> complex
+  for (int i = t; i < G * GG; i += 512)
+    A[(i / GG) * PL + ((i / G) % G) * RS + (i % G)] = make_float2((float)a.mass[cell0 + i], 0.f);
+  __syncthreads();
+  float2 v[G], xs[G];
+  if (half == 0) {  // x lines (z = hi, y = lo)
+    float2* row = A + hi * PL + lo * RS;
+#pragma unroll
+    for (int x = 0; x < G; ++x) v[x] = row[x];
+    reg_fft<G, false>(v);
+#pragma unroll
+    for (int x = 0; x < G; ++x) row[x] = v[x];
+  }
+  __syncthreads();
+  if (half == 0) {  // y lines (z = hi, kx = lo)
+    float2* col = A + hi * PL + lo;
+#pragma unroll
+    for (int y = 0; y < G; ++y) v[y] = col[y * RS];
+    reg_fft<G, false>(v);
+#pragma unroll
+    for (int y = 0; y < G; ++y) col[y * RS] = v[y];
+  }
+  __syncthreads();
+  // z lines (ky = hi, kx = lo): both halves transform the same line, the
+  // first keeps the y + i z field, the second the x field
+  float2* zl = A + hi * RS + lo;
+#pragma unroll
+  for (int z = 0; z < G; ++z) v[z] = zl[z * PL];
+  __syncthreads();
+  if (half == 0 || need_x) {
+    reg_fft<G, false>(v);
+    rf_multiply<G>(a, hi, lo, v, xs);
+    float2* dst = half == 0 ? zl : B + hi * RS + lo;
+    if (half == 0) {
+      reg_fft<G, true>(v);
+#pragma unroll
+      for (int z = 0; z < G; ++z) dst[z * PL] = v[z];
+    } else {
+      reg_fft<G, true>(xs);
+#pragma unroll
+      for (int z = 0; z < G; ++z) dst[z * PL] = xs[z];
+    }
+  }
+  __syncthreads();
+  float2* const F = half == 0 ? A : B;
+  const bool busy = half == 0 || need_x;
+  if (busy) {  // inverse y (z = hi, kx = lo) of the half's field
+    float2* col = F + hi * PL + lo;
+#pragma unroll
+    for (int ky = 0; ky < G; ++ky) v[ky] = col[ky * RS];
+    reg_fft<G, true>(v);
+#pragma unroll
+    for (int y = 0; y < G; ++y) col[y * RS] = v[y];
+  }
+  __syncthreads();
+  if (busy) {  // inverse x (z = hi, y = lo)
+    float2* row = F + hi * PL + lo * RS;
+#pragma unroll
+    for (int x = 0; x < G; ++x) v[x] = row[x];
+    reg_fft<G, true>(v);
+#pragma unroll
+    for (int x = 0; x < G; ++x) row[x] = v[x];
+  }
+  __syncthreads();
+  const float* Rr = a.rot + 12 * grid;
+  float4* out = a.force + cell0;
+  for (int i = t; i < G * GG; i += 512) {
+    const int at = (i / GG) * PL + ((i / G) % G) * RS + (i % G);
+    const float2 tz = A[at];
+    const float fy = tz.x, fz = tz.y;
+    const float fx = need_x ? B[at].x : fz;
+    const float mag = sqrtf(fmaf(fx, fx, fmaf(fy, fy, fz * fz)));
+    float4 o;
+    if (world) {
+      o = make_float4(fmaf(Rr[0], fx, fmaf(Rr[1], fy, Rr[2] * fz)),
+                      fmaf(Rr[4], fx, fmaf(Rr[5], fy, Rr[6] * fz)),
+                      fmaf(Rr[8], fx, fmaf(Rr[9], fy, Rr[10] * fz)), mag);
+    } else {
+      o = make_float4(fx, fy, fz, mag);
+    }
+    out[i] = o;
+  }
+}
+
+// One rf_grid_kernel workgroup per grid, grids [g0, g0 + ngrids) of `res`.
+template <int G, typename R>
+static hipError_t launch_rf_grids(const R& res, long long g0, long long ngrids, int world,
+                                  hipStream_t s) {
+  if (ngrids <= 0) return hipSuccess;
+  const size_t lds = (size_t)2 * G * G * (G + 1) * sizeof(float2);
+  hipError_t e = hipFuncSetAttribute((const void*)rf_grid_kernel<G, R>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((rf_grid_kernel<G, R>), dim3((unsigned)ngrids), dim3(512), lds, s, res, g0,
+                     world);
+  return hipGetLastError();
+}
+
 // The three register-FFT passes over slabs [t0, t0 + nslabs) of `res`.
 template <int G, typename R>
 static hipError_t launch_rf_passes(const R& res, long long t0, long long nslabs, int world,
@@ -729,6 +860,7 @@ static hipError_t launch_rf_solve(const Params& P, const long long* grids, float
                                   const float2* conv, int need_x, const float* rot, int world,
                                   hipStream_t s) {
   const RfSingle res = {grids, S1, S2, force, gtab, kd, conv, rot, need_x};
+  if (G == 16) return launch_rf_grids<16>(res, 0, P.k, world, s);
   return launch_rf_passes<G>(res, 0, (long long)P.k * G, world, s);
 }
 
@@ -843,7 +975,8 @@ hipError_t launch_batch_solve(const BatchSim* sims, const int* cls_sim, const lo
   // same arithmetic as the single-simulation solve
   if (gmax == 16 || gmax == 32) {
     const RfBatch res = {sims, cls_sim, cls_off, ncls, soff, grids, S1, S2, force, gtab, kd, conv, rot};
-    return gmax == 16 ? launch_rf_passes<16>(res, slab0, slabs, 1, s)
+    // (windows hold whole grids: slab0 and slabs are multiples of g)
+    return gmax == 16 ? launch_rf_grids<16>(res, slab0 / 16, slabs / 16, 1, s)
                       : launch_rf_passes<32>(res, slab0, slabs, 1, s);
   }
   const size_t ldsA = solve_lds(gmax, 0), ldsB = solve_lds(gmax, 1), ldsC = solve_lds(gmax, 2);
