@@ -78,6 +78,9 @@ struct mog_ctx {
   // uint16 cell of every (grid, slot) pair
   float4* d_spos;
   unsigned short* d_cells;
+  // prepared rows of the LDS deposit (centred coordinates, milli-mass)
+  float4* d_cpos;
+  long long cpos_cap;
 
   // graph cache of mog_run: one captured step per assignment of the two
   // density buffers (gexec deposits into g_dep, gexec_alt into the other)
@@ -89,6 +92,7 @@ struct mog_ctx {
   const void* g_vel;
   long long g_n;
   bool g_ordered;
+  const float4* g_cpos;
   std::string err;
 };
 
@@ -407,6 +411,7 @@ void release(mog_ctx* c) {
   (void)hipFree(c->d_sort_tmp);
   (void)hipFree(c->d_Wf);
   (void)hipFree(c->d_Wi);
+  (void)hipFree(c->d_cpos);
 
 }
 
@@ -496,6 +501,23 @@ bool ensure_order(mog_ctx* c, int64_t n) {
   return true;
 }
 
+// The deposit's prepared-row scratch for n particles (outside graph
+// capture); NULL when allocation fails (the deposit then reads the
+// positions directly).
+float4* ensure_cpos(mog_ctx* c, int64_t n) {
+  if (c->cpos_cap >= n && c->d_cpos) return c->d_cpos;
+  (void)hipFree(c->d_cpos);
+  c->d_cpos = nullptr;
+  c->cpos_cap = 0;
+  if (hipMalloc((void**)&c->d_cpos, (size_t)n * sizeof(float4)) != hipSuccess) {
+    (void)hipGetLastError();
+    c->d_cpos = nullptr;
+    return nullptr;
+  }
+  c->cpos_cap = n;
+  return c->d_cpos;
+}
+
 // Rebuild the order when it is stale for (pos, n); returns the order to use
 // (NULL: storage order).
 hipError_t refresh_order(mog_ctx* c, const float* pos, int64_t n, hipStream_t s, const int** out) {
@@ -537,7 +559,8 @@ void drop_graphs(mog_ctx* ctx) {
 // then makes dep the current density and old the zero spare (swap_grids)
 // once the step is launched, not when it is only captured.
 hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const int* order,
-                        long long* dep, long long* old, hipStream_t s, bool allow_sorted = true) {
+                        long long* dep, long long* old, hipStream_t s, bool allow_sorted = true,
+                        float4* cpos = nullptr) {
   const Params& P = ctx->P;
   const long long cells = (long long)P.k * P.cells;
   if (allow_sorted && order && ctx->sorted_deposit && ctx->d_spos && ctx->d_cells) {
@@ -563,7 +586,7 @@ hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const i
     return e;
   }
   hipError_t e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                                (unsigned long long*)dep, ctx->tune, s);
+                                (unsigned long long*)dep, ctx->tune, s, cpos);
   if (e != hipSuccess) return e;
   e = solve_dispatch(ctx, dep, ctx->d_force, 1, s);
   if (e != hipSuccess) return e;
@@ -931,8 +954,10 @@ int mog_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, void* stream) {
   const int* order = nullptr;
   hipError_t e = clean_spare(ctx, (hipStream_t)stream);
   if (e == hipSuccess) e = refresh_order(ctx, pos, n, (hipStream_t)stream, &order);
+  float4* const cpos = ensure_cpos(ctx, n);
   if (e == hipSuccess)
-    e = enqueue_step(ctx, pos, vel, n, order, ctx->d_grids_next, ctx->d_grids, (hipStream_t)stream);
+    e = enqueue_step(ctx, pos, vel, n, order, ctx->d_grids_next, ctx->d_grids, (hipStream_t)stream,
+                     true, cpos);
   if (e != hipSuccess) {
     ctx->spare_dirty = true;
     return hip_fail(ctx, e, "mog_step");
@@ -951,12 +976,13 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = clean_spare(ctx, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "mog_run");
+  float4* const cpos = ensure_cpos(ctx, n);
   if (!use_graph) {
     for (int i = 0; i < nsteps; ++i) {
       const int* order = nullptr;
       if ((e = refresh_order(ctx, pos, n, s, &order)) != hipSuccess ||
-          (e = enqueue_step(ctx, pos, vel, n, order, ctx->d_grids_next, ctx->d_grids, s)) !=
-              hipSuccess) {
+          (e = enqueue_step(ctx, pos, vel, n, order, ctx->d_grids_next, ctx->d_grids, s, true,
+                            cpos)) != hipSuccess) {
         ctx->spare_dirty = true;
         return hip_fail(ctx, e, "mog_run");
       }
@@ -968,7 +994,9 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
   // direct launches between graph replays (the sort is not captured).
   const int* const order_buf_before = ctx->d_order;
   const bool ordered = ensure_order(ctx, n);
-  if (ctx->gexec && (ctx->d_order != order_buf_before || ordered != ctx->g_ordered)) drop_graphs(ctx);
+  if (ctx->gexec && (ctx->d_order != order_buf_before || ordered != ctx->g_ordered ||
+                     ctx->g_cpos != cpos))
+    drop_graphs(ctx);
   if (!ctx->gexec || ctx->g_pos != pos || ctx->g_vel != vel || ctx->g_n != n) {
     drop_graphs(ctx);
     if (!ctx->cap_stream &&
@@ -985,7 +1013,7 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
         return hip_fail(ctx, e, "hipStreamBeginCapture");
       }
       hipError_t ee = enqueue_step(ctx, pos, vel, n, ordered ? ctx->d_order : nullptr, bufs[a],
-                                   bufs[1 - a], ctx->cap_stream);
+                                   bufs[1 - a], ctx->cap_stream, true, cpos);
       hipGraph_t graph = nullptr;
       e = hipStreamEndCapture(ctx->cap_stream, &graph);
       if (ee != hipSuccess || e != hipSuccess) {
@@ -1006,6 +1034,7 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
     ctx->g_vel = vel;
     ctx->g_n = n;
     ctx->g_ordered = ordered;
+    ctx->g_cpos = cpos;
   }
   for (int i = 0; i < nsteps; ++i) {
     if (ordered) {
@@ -1110,7 +1139,7 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
   if (e == hipSuccess)
     e = launch_deposit_ex(P, sorted ? ctx->d_spos : (const float4*)pos, n, ctx->d_rot, ctx->d_off,
                           ctx->d_frot, (unsigned long long*)dep, cache ? ctx->d_cells : nullptr,
-                          sorted, ctx->tune, s);
+                          sorted, ctx->tune, s, sorted ? nullptr : ensure_cpos(ctx, n));
   if (e == hipSuccess) e = hipEventRecord(ev[2], s);
   if (e == hipSuccess) e = solve_dispatch(ctx, dep, ctx->d_force, 1, s);
   if (e == hipSuccess) e = hipEventRecord(ev[3], s);
@@ -1142,10 +1171,11 @@ int mog_deposit(mog_ctx* ctx, const float* pos, int64_t n, int64_t* grids_out, v
   if (((uintptr_t)grids_out & 7u) != 0) return fail(ctx, MOG_EINVAL, "grids_out must be 8-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   const long long cells = (long long)ctx->P.k * ctx->P.cells;
+  float4* const cpos = ensure_cpos(ctx, n);
   hipError_t e = launch_fill_zero((long long*)grids_out, cells, s);
   if (e == hipSuccess)
     e = launch_deposit(ctx->P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                       (unsigned long long*)grids_out, ctx->tune, s);
+                       (unsigned long long*)grids_out, ctx->tune, s, cpos);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_deposit");
 }
 

@@ -111,7 +111,7 @@ MOG_HD float cell_fast_d(float c0, float c1, float c2, const float* F, float a0,
 // gfx950, where shifts and shift-adds take twice as long.
 struct FieldShift {
   float k0, k1, k2;         // 1.5 * 2^(23 - s_m)
-  unsigned m0, m01, m012;   // bits below s0 + lg, s0 + 2 lg, s0 + 3 lg
+  unsigned m0, m01, m012;   // bits below s0 + lg, s0 + 2 lg; bits [s0, s0 + 3 lg)
 };
 
 MOG_HD float pow2_1p5(int e) {  // 1.5 * 2^e, |e| < 127
@@ -128,7 +128,11 @@ MOG_HD FieldShift field_shift(int lg, int s0) {
   S.k2 = pow2_1p5(23 - s0 - 2 * lg);
   S.m0 = (1u << (s0 + lg)) - 1u;
   S.m01 = (1u << (s0 + 2 * lg)) - 1u;
-  S.m012 = (1u << (s0 + 3 * lg)) - 1u;
+  // the final mask also clears the bits below s0: zero for admitted
+  // particles, they hold rounding residue where |w| exceeds the fields'
+  // range (a particle outside the admitted bound), and the offset must stay
+  // a multiple of the element size for every input
+  S.m012 = ((1u << (s0 + 3 * lg)) - 1u) & ~((1u << s0) - 1u);
   return S;
 }
 
@@ -170,8 +174,9 @@ MOG_HD unsigned and_or(unsigned a, unsigned m, unsigned c) {
 
 // cell_fast_d with the cell returned as the byte offset (cell << s0) | base
 // (S from field_shift(lg, s0), valid under field_shift_ok; base has no bits
-// below s0 + 3 lg).  Same cell, same d; (cell << s0) < g^3 << s0 for every
-// input.
+// below s0 + 3 lg).  Same cell, same d, for admitted particles; for every
+// input the offset is a multiple of 2^s0 below g^3 << s0 (an aligned element
+// inside the grid), whatever the cell.
 MOG_HD float cell_fast_o(float c0, float c1, float c2, const float* F, float a0, float a1, float a2,
                          const FieldShift& S, unsigned base, unsigned* off) {
   const float w0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, a0)));

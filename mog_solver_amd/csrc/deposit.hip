@@ -71,9 +71,16 @@ constexpr int kDepositUP = 2;
 // SH: the fast cells come as LDS byte offsets through the shifted fields of
 // cellmath.h (cell_fast_o), for launches whose transforms satisfy
 // field_shift_ok(lg, 2, ...); otherwise as indices (cell_fast_d).
-template <int G, bool SH>
+//
+// PREP: the loops read the rows deposit_prep_kernel wrote (centred
+// coordinates and the 32-bit milli-mass, or kPrepMarker for a particle the
+// fast path cannot take: outside the admitted bound, non-finite or with a
+// milli-mass that does not fit the partials), so the per-particle work of a
+// pair is one compare; the repair adds the marker back and takes such
+// particles from `pos`.
+template <int G, bool SH, bool PREP>
 __global__ __launch_bounds__(1024) void deposit_lds_kernel(
-    const float4* __restrict__ pos, long long n, long long chunk,
+    const float4* __restrict__ pos, const float4* __restrict__ cpos, long long n, long long chunk,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
     Params P, unsigned long long lds_limit, unsigned long long* __restrict__ grids) {
   extern __shared__ unsigned int hist[];
@@ -142,6 +149,66 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
       atomicAdd(hist + gi * cells + c, v);
   };
 
+  // The fast cell (index) of a pair and its face distance, computed as the
+  // loop computed the cell it added to: the two forms agree for admitted
+  // particles only, and a repair takes back what was added.
+  auto fast_cell = [&](float c0, float c1, float c2, int gi, int* fc) -> float {
+    if constexpr (SH) {
+      unsigned o;
+      const float d = cell_fast_o(c0, c1, c2, F[gi], F[gi][3], F[gi][7], F[gi][11], FS, 0u, &o);
+      *fc = (int)(o >> 2);
+      return d;
+    } else {
+      return cell_fast_d(c0, c1, c2, F[gi], F[gi][3], F[gi][7], F[gi][11], P.lg, fc);
+    }
+  };
+
+  // PREP: the pairs of particle p.  A marked particle had the marker added
+  // at its fast cells (2^32 - 1, i.e. -1 modulo 2^32): it is taken back and
+  // the true milli-mass goes to the exact cell, in the partials when small,
+  // else into the 64-bit grids.
+  auto repair_prepped = [&](long long p) {
+    const float4 cp = cpos[p];
+    const unsigned int m_add = __float_as_uint(cp.w);
+    const float c0 = cp.x, c1 = cp.y, c2 = cp.z;
+    const int pf = particle_fast(c0, c1, c2, P.CB);
+    const bool marked = m_add > lds_limit32;
+    bool small = true;
+    unsigned int m32 = 0;
+    float w = 0.0f;
+    if (marked) {
+      w = pos[p].w;
+      m32 = milli32(w, small);
+    }
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+      if (gi >= ng) continue;
+      const int gr = grid0 + gi;
+      int fc;
+      const float d = fast_cell(c0, c1, c2, gi, &fc);
+      const bool ok = pf && d < P.lim;
+      if (marked) {
+        atomicAdd(&hist[gi * cells + fc], 0u - m_add);
+        const unsigned long long milli = small ? m32 : milli_of(w);
+        if (milli) {
+          const int ex = ok ? fc : cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W,
+                                                   P.half, P.h, P.lg);
+          if (small)
+            atomicAdd(&hist[gi * cells + ex], m32);
+          else
+            atomicAdd(&grids[(long long)gr * cells + ex], milli);
+        }
+      } else if (!ok && m_add) {
+        const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half, P.h,
+                                       P.lg);
+        if (ex != fc) {
+          atomicAdd(&hist[gi * cells + ex], m_add);
+          atomicAdd(&hist[gi * cells + fc], 0u - m_add);
+        }
+      }
+    }
+  };
+
   // Repair the pairs of the UP particles of one record (first particle
   // pu0): exact cell where the fast one was not decided, 64-bit path for
   // huge masses.  Runs per lane.
@@ -150,6 +217,10 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
     for (int u = 0; u < UP; ++u) {
       const long long p = pu0 + (long long)u * blockDim.x;
       if (p >= p1) continue;
+      if constexpr (PREP) {
+        repair_prepped(p);
+        continue;
+      }
       const float4 q = pos[p];
       bool small;
       const unsigned int m32 = milli32(q.w, small);
@@ -160,7 +231,7 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
         if (gi >= ng) continue;
         const int gr = grid0 + gi;
         int fc;
-        const float d = cell_fast_d(c0, c1, c2, F[gi], F[gi][3], F[gi][7], F[gi][11], P.lg, &fc);
+        const float d = fast_cell(c0, c1, c2, gi, &fc);
         const bool ok = pf && d < P.lim;
         if (!small) {  // huge mass: out of the partials, into the 64-bit grids
           if (m32) atomicAdd(&hist[gi * cells + fc], 0u - m32);
@@ -190,13 +261,21 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
     bool sure = true;  // flags combine with bitwise &: no short-circuit branches
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
-      bool small;
-      m32[u] = milli32(q[u].w, small);
-      cc[u][0] = q[u].x - half;
-      cc[u][1] = q[u].y - half;
-      cc[u][2] = q[u].z - half;
-      sure = sure & small & (fabsf(cc[u][0]) <= P.CB) & (fabsf(cc[u][1]) <= P.CB) &
-             (fabsf(cc[u][2]) <= P.CB);
+      if constexpr (PREP) {  // admission and mass folded into one compare
+        m32[u] = __float_as_uint(q[u].w);
+        cc[u][0] = q[u].x;
+        cc[u][1] = q[u].y;
+        cc[u][2] = q[u].z;
+        sure = sure & (m32[u] <= lds_limit32);
+      } else {
+        bool small;
+        m32[u] = milli32(q[u].w, small);
+        cc[u][0] = q[u].x - half;
+        cc[u][1] = q[u].y - half;
+        cc[u][2] = q[u].z - half;
+        sure = sure & small & (fabsf(cc[u][0]) <= P.CB) & (fabsf(cc[u][1]) <= P.CB) &
+               (fabsf(cc[u][2]) <= P.CB);
+      }
     }
     float dmax = 0.0f;
 #pragma unroll
@@ -241,8 +320,9 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
 
   const long long span = 2ll * UP * blockDim.x;
   const bool fast = ng == G && p1 - p0 == chunk && chunk % span == 0 && chunk < (1ll << 27);
+  const float4* const src = PREP ? cpos : pos;  // the rows the loops read
   if (fast) {
-    const char* base = reinterpret_cast<const char*>(pos + p0);
+    const char* base = reinterpret_cast<const char*>(src + p0);
     const unsigned int lane_step = 16u * blockDim.x;  // bytes between a lane's particles
     auto load = [&](float4* q, unsigned int boff) {
 #pragma unroll
@@ -269,7 +349,7 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
       const long long p = p0 + threadIdx.x + (long long)u * blockDim.x;
-      qn[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+      qn[u] = p < p1 ? src[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     for (long long pb = p0 + threadIdx.x; pb < p1; pb += step) {
       float4 q[UP];
@@ -277,7 +357,7 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
       for (int u = 0; u < UP; ++u) {
         q[u] = qn[u];
         const long long p = pb + step + (long long)u * blockDim.x;
-        qn[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+        qn[u] = p < p1 ? src[p] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
       pairs(q, pb - p0);
     }
@@ -291,7 +371,7 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
 #pragma unroll
       for (int u = 0; u < UP; ++u) {
         const long long p = p0 + pl + (long long)u * blockDim.x;
-        q[u] = p < p1 ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+        q[u] = p < p1 ? src[p] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
       unsigned int m32[UP];
       unsigned cell[UP][G];
@@ -436,54 +516,98 @@ hipError_t launch_permute(const float4* pos, const int* order, long long n, floa
 
 static const size_t kMaxLds = 160 * 1024;
 
+// Rows of the prepared deposit: (c0, c1, c2, milli) with c = p - W/2 and the
+// 32-bit milli-mass, or kPrepMarker where the fast path cannot take the
+// particle (outside |c_j| <= CB, non-finite, or a milli-mass above the
+// partials' limit).  The milli-mass and the admission test are those of the
+// unprepared loop (same float32 operations, saturating conversion).
+constexpr unsigned int kPrepMarker = 0xFFFFFFFFu;
+
+__global__ __launch_bounds__(256) void deposit_prep_kernel(const float4* __restrict__ pos,
+                                                           long long n, Params P,
+                                                           unsigned int lds_limit32,
+                                                           float4* __restrict__ cpos) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = pos[i];
+  const float c0 = q.x - P.half, c1 = q.y - P.half, c2 = q.z - P.half;
+  const float mm = q.w * 1000.0f;
+  unsigned int m;
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(m) : "v"(mm));
+  const bool take = (m <= lds_limit32) & (bool)particle_fast(c0, c1, c2, P.CB);
+  cpos[i] = make_float4(c0, c1, c2, __uint_as_float(take ? m : kPrepMarker));
+}
+
 template <int G>
 static hipError_t launch_lds(const Params& P, const float4* pos, long long n, long long chunks,
                              long long chunk, int threads, const float* rot, const float* off,
                              const float* frot, unsigned long long* grids,
-                             unsigned short* cells_out, bool sorted, hipStream_t s) {
+                             unsigned short* cells_out, bool sorted, float4* cpos, hipStream_t s) {
   const int groups = (P.k + G - 1) / G;
   const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)chunk;
   const size_t lds = (size_t)G * P.cells * 4 + (sorted ? 0 : (size_t)kLaneSlots * threads * 4);
   bool sh = P.shift_ok & kShiftLds;
   if (sh && !sorted) {  // the shifted path addresses its counters from LDS address 0
     hipFuncAttributes fa;
-    hipError_t e = hipFuncGetAttributes(&fa, (const void*)deposit_lds_kernel<G, true>);
+    hipError_t e = hipFuncGetAttributes(&fa, (const void*)deposit_lds_kernel<G, true, false>);
     if (e != hipSuccess) return e;
     sh = fa.sharedSizeBytes == 0;
+    if constexpr (G <= 4) {
+      if (sh && cpos) {
+        e = hipFuncGetAttributes(&fa, (const void*)deposit_lds_kernel<G, true, true>);
+        if (e != hipSuccess) return e;
+        if (fa.sharedSizeBytes != 0) cpos = nullptr;
+      }
+    }
   }
+  const bool prep = G <= 4 && sh && !sorted && cpos;
   const void* fn = sorted ? (cells_out ? (const void*)deposit_sorted_kernel<G, true>
                                        : (const void*)deposit_sorted_kernel<G, false>)
-                   : sh   ? (const void*)deposit_lds_kernel<G, true>
-                          : (const void*)deposit_lds_kernel<G, false>;
+                   : sh   ? (const void*)deposit_lds_kernel<G, true, false>
+                          : (const void*)deposit_lds_kernel<G, false, false>;
+  if constexpr (G <= 4)
+    if (prep) fn = (const void*)deposit_lds_kernel<G, true, true>;
   // the exact dynamic size: the kernels also hold static LDS
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   const dim3 grid((unsigned)(chunks * groups));
-  if (!sorted && sh)
-    hipLaunchKernelGGL((deposit_lds_kernel<G, true>), grid, dim3(threads), lds, s, pos, n, chunk,
-                       rot, off, frot, P, lds_limit, grids);
-  else if (!sorted)
-    hipLaunchKernelGGL((deposit_lds_kernel<G, false>), grid, dim3(threads), lds, s, pos, n, chunk,
-                       rot, off, frot, P, lds_limit, grids);
-  else if (cells_out)
+  if constexpr (G <= 4) {
+    if (prep) {
+      const unsigned int lds_limit32 = (unsigned int)lds_limit;  // (2^32 - 1) / chunk
+      hipLaunchKernelGGL(deposit_prep_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                         pos, n, P, lds_limit32, cpos);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      hipLaunchKernelGGL((deposit_lds_kernel<G, true, true>), grid, dim3(threads), lds, s, pos,
+                         (const float4*)cpos, n, chunk, rot, off, frot, P, lds_limit, grids);
+      return hipGetLastError();
+    }
+  }
+  if (!sorted && sh) {
+    hipLaunchKernelGGL((deposit_lds_kernel<G, true, false>), grid, dim3(threads), lds, s, pos,
+                       (const float4*)nullptr, n, chunk, rot, off, frot, P, lds_limit, grids);
+  } else if (!sorted) {
+    hipLaunchKernelGGL((deposit_lds_kernel<G, false, false>), grid, dim3(threads), lds, s, pos,
+                       (const float4*)nullptr, n, chunk, rot, off, frot, P, lds_limit, grids);
+  } else if (cells_out) {
     hipLaunchKernelGGL((deposit_sorted_kernel<G, true>), grid, dim3(threads), lds, s, pos, n, chunk,
                        rot, off, frot, P, lds_limit, grids, cells_out);
-  else
+  } else {
     hipLaunchKernelGGL((deposit_sorted_kernel<G, false>), grid, dim3(threads), lds, s, pos, n,
                        chunk, rot, off, frot, P, lds_limit, grids, cells_out);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const float* rot,
                           const float* off, const float* frot, unsigned long long* grids,
-                          const DepositTune& t, hipStream_t s) {
-  return launch_deposit_ex(P, pos, n, rot, off, frot, grids, nullptr, false, t, s);
+                          const DepositTune& t, hipStream_t s, float4* cpos) {
+  return launch_deposit_ex(P, pos, n, rot, off, frot, grids, nullptr, false, t, s, cpos);
 }
 
 hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, const float* rot,
                              const float* off, const float* frot, unsigned long long* grids,
                              unsigned short* cells_out, bool sorted, const DepositTune& t,
-                             hipStream_t s) {
+                             hipStream_t s, float4* cpos) {
   if (n <= 0) return hipSuccess;
   const size_t grid_bytes = (size_t)P.cells * 4;
   if (grid_bytes + (size_t)kLaneSlots * 1024 * 4 > kMaxLds) {
@@ -517,16 +641,16 @@ hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, co
   switch (G) {
     case 8:
       return launch_lds<8>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
-                           sorted, s);
+                           sorted, cpos, s);
     case 4:
       return launch_lds<4>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
-                           sorted, s);
+                           sorted, cpos, s);
     case 2:
       return launch_lds<2>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
-                           sorted, s);
+                           sorted, cpos, s);
     default:
       return launch_lds<1>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
-                           sorted, s);
+                           sorted, cpos, s);
   }
 }
 

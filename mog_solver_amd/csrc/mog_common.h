@@ -142,9 +142,11 @@ struct DepositTune {
 };
 
 hipError_t launch_fill_zero(long long* p, long long count, hipStream_t s);
+// cpos (optional, n rows): scratch for the prepared rows of the LDS
+// deposit (centred coordinates and milli-mass, written by a prepass).
 hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const float* rot,
                           const float* off, const float* frot, unsigned long long* grids,
-                          const DepositTune& t, hipStream_t s);
+                          const DepositTune& t, hipStream_t s, float4* cpos = nullptr);
 // world != 0: force grids hold M F (world frame) for the step's gather;
 // world == 0: grid-frame forces as pack.glsl writes them.
 // Deposit of positions already in Morton visiting order (sorted != 0; lanes
@@ -153,7 +155,7 @@ hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const
 hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, const float* rot,
                              const float* off, const float* frot, unsigned long long* grids,
                              unsigned short* cells_out, bool sorted, const DepositTune& t,
-                             hipStream_t s);
+                             hipStream_t s, float4* cpos = nullptr);
 // spos[t] = pos[order[t]]
 hipError_t launch_permute(const float4* pos, const int* order, long long n, float4* spos,
                           hipStream_t s);

@@ -84,7 +84,9 @@ int main(int argc, char** argv) {
         const mog::FieldShift S2 = mog::field_shift(lg, 2), S4 = mog::field_shift(lg, 4);
         for (long long s = 0; s < samples / 240; ++s) {
           float p[3];
-          if (s & 1) {
+          if (s % 16 == 3) {  // far outside the box, up to 1e10 W
+            for (int d = 0; d < 3; ++d) p[d] = (float)((uni() * 2.0 - 1.0) * W * pow(10.0, uni() * 10.0));
+          } else if (s & 1) {
             for (int d = 0; d < 3; ++d) p[d] = (float)((uni() * 2.0 * cbw + 0.5 - cbw) * W * 1.25);
           } else {
             // local coordinates on or next to cell faces, mapped to world
@@ -105,6 +107,18 @@ int main(int argc, char** argv) {
           int cf = -1;
           const int ok = mog::cell_fast(c0, c1, c2, G.F, lim, lg, &cf) & mog::particle_fast(c0, c1, c2, CB);
           ++total;
+          // every position, admitted or not: aligned offsets inside the grid
+          for (int v = 0; v < 2; ++v) {
+            const int s0 = v ? 4 : 2;
+            unsigned off = ~0u;
+            (void)mog::cell_fast_o(c0, c1, c2, G.F, G.F[3], G.F[7], G.F[11], v ? S4 : S2, 0u, &off);
+            if ((off & ((1u << s0) - 1u)) != 0u || off >= (1u << (s0 + 3 * lg))) {
+              if (bad < 10)
+                fprintf(stderr, "offset out of range W=%g g=%d s0=%d p=(%.9g,%.9g,%.9g) off=%u\n", W, g, s0, p[0], p[1],
+                        p[2], off);
+              ++bad;
+            }
+          }
           if (mog::particle_fast(c0, c1, c2, CB)) {
             const float dref = mog::cell_fast_d(c0, c1, c2, G.F, G.F[3], G.F[7], G.F[11], lg, &cf);
             for (int v = 0; v < 2; ++v) {

@@ -214,6 +214,28 @@ def test_gather_matches_fp64(torch_cuda):
     assert O.rel_rms(got[:, :3], want) < 1e-6
 
 
+@pytest.mark.parametrize("k,g", [(5, 16), (3, 32)])
+def test_deposit_far_positions(torch_cuda, k, g):
+    """Positions far outside the box (|w| beyond the range of the fast
+    cell's float fields) next to ordinary ones: the exact sequence takes them,
+    and the fast cell a pair was first added to is taken back where it was
+    added."""
+    torch = torch_cuda
+    W = 100.0
+    rng = np.random.default_rng(21)
+    pos = edge_positions(W, g, 20000, seed=g)
+    far = np.array([3e9, -2e8, 1e7, -5e5, 2.5e4, -333.25], np.float32)
+    idx = rng.integers(0, pos.shape[0], size=(3000, 3))
+    pos[idx[:, 0], 0] = far[rng.integers(0, far.size, 3000)]
+    pos[idx[:, 1], 1] = far[rng.integers(0, far.size, 3000)]
+    pos[idx[:, 2], 2] = far[rng.integers(0, far.size, 3000)]
+    pos[::7, 3] = np.float32(3e6)  # and some masses above the 32-bit partials
+    s, rot, off = make_solver(k, g, W)
+    got = s.deposit(dev(torch, pos)).cpu().numpy()
+    want = O.deposit(pos, rot, off, O.Params.make(k, g, W))
+    assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} cells differ"
+
+
 @pytest.mark.parametrize("k,g,shift", [(3, 32, 40.0), (5, 16, 300.0)])
 def test_deposit_and_gather_with_far_offsets(torch_cuda, k, g, shift):
     """Offsets moved by a multiple of W (same cells up to rounding, which the
