@@ -43,17 +43,21 @@ FP32_VECTOR_PEAK_TFS = 157.3
 # profiles/pmc_traffic_two_disks_n1e6_k240_g16.md.  Used only when the run
 # matches (workload, n, k, g, precision, grids per deposit workgroup,
 # deposit workgroup target): the deposit's re-reads of the positions depend
-# on its tiling.
+# on its tiling.  The deposit phase is its prepass plus the LDS kernel.
 PMC_TRAFFIC_KIB = {
-    ("two_disks", 1_000_000, 240, 16, "fp32", 2, 4096): {
-        "deposit": (9_585.8, 61_289.1),
-        "gather_update": (124_184.8, 73_676.0),
-    },
     ("two_disks", 1_000_000, 240, 16, "fp32", 2, 2048): {
-        "deposit": (10_627.5, 33_642.9),
-        "gather_update": (124_184.8, 73_672.3),
+        "deposit": (7_818.1 + 10_505.8, 15_625.0 + 35_177.7),
+        "gather_update": (123_321.4, 73_737.6),
+        "solve": (3_735.3, 14_242.9),
     },
 }
+# Vector instructions per wave-wide particle-grid pair (SQ_INSTS_VALU / 3.75e6
+# for the kernels of the same tiling, profiles/sq_counters_two_disks_n1e6_k240_g16.md):
+# the deposit and the gather are bound by vector-instruction issue.
+SQ_VALU_PER_WAVE_PAIR = {
+    ("two_disks", 1_000_000, 240, 16, "fp32", 2, 2048): {"deposit": 25.66, "gather_update": 31.10},
+}
+SIMDS, CLOCK_HZ = 1024, 2.4e9
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
 
 
@@ -265,12 +269,28 @@ def main():
     dom = max(("deposit", "gather_update", "solve"), key=lambda key: phases[key])
     achieved = alg_bytes[dom] / (phases[dom] / 1e3) / 1e9
     tuning = solver.get_tuning()
-    measured = PMC_TRAFFIC_KIB.get((args.workload, n, k, g, args.precision, tuning["deposit_grids_per_wg"],
-                                    tuning["deposit_wg_target"]), {})
+    key = (args.workload, n, k, g, args.precision, tuning["deposit_grids_per_wg"], tuning["deposit_wg_target"])
+    measured = PMC_TRAFFIC_KIB.get(key, {})
     traffic = None
     if dom in measured:
         fetch_kib, write_kib = measured[dom]
         traffic = round((fetch_kib * FETCH_FACTOR + write_kib * WRITE_FACTOR) * 1024.0)
+    # the limit the SQ counters show: vector-instruction issue.  Issue rate
+    # of the timed phase = profiled instructions per pair x pairs / (SIMDs x
+    # cycles); peak = one wave64 instruction per 2 SIMD-cycles
+    issue = None
+    per_pair = SQ_VALU_PER_WAVE_PAIR.get(key, {}).get(dom)
+    if per_pair is not None:
+        instr = per_pair * float(n) * k / 64.0
+        simd_cycles = SIMDS * CLOCK_HZ * phases[dom] / 1e3
+        issue = {
+            "limit": "vector-instruction issue",
+            "valu_instr_per_wave_pair": per_pair,
+            "simd_cycles_per_instr": round(simd_cycles / instr, 3),
+            "frac": round(2.0 * instr / simd_cycles, 4),
+            "peak": "1 wave64 instruction per 2 SIMD-cycles; about 2.7 for FP32 add/FMA and bit operations "
+                    "with vector operands, 4.3 for shifts, rint, max3, conversions and scalar operands",
+        }
     roofline = {
         "bound": "hbm",
         "kernel": dom,
@@ -281,6 +301,7 @@ def main():
         "traffic": traffic,
         "alg_bytes_per_launch": alg_bytes[dom],
         "avg_launch_ms": round(phases[dom], 4),
+        "issue": issue,
     }
 
     # achieved fractions per phase: HBM (algorithmic bytes), and the level

@@ -73,7 +73,10 @@ typedef struct mog_tuning {
   int32_t cell_cache;           /* 1: the sorted deposit stores cells for the gather (g <= 32) */
   int32_t spectral_mfma;        /* 1: matrix-core DFT for FP32 at g >= 16 (other precisions
                                    always use it) */
-  int32_t reserved[9];
+  int32_t deposit_prepass;      /* LDS deposit reads rows prepared by a prepass (centred
+                                   coordinates, milli-mass): 0 = when n*k >= 2^26 (default),
+                                   1 = always, -1 = never; results are identical */
+  int32_t reserved[8];
 } mog_tuning;
 
 typedef struct mog_ctx mog_ctx;

@@ -96,11 +96,12 @@ class MogTuning(ctypes.Structure):
         ("sorted_deposit", ctypes.c_int32),
         ("cell_cache", ctypes.c_int32),
         ("spectral_mfma", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 9),
+        ("deposit_prepass", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 8),
     ]
 
     FIELDS = ("deposit_threads", "deposit_grids_per_wg", "deposit_wg_target", "resort_every",
-              "sorted_deposit", "cell_cache", "spectral_mfma")
+              "sorted_deposit", "cell_cache", "spectral_mfma", "deposit_prepass")
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f in self.FIELDS}

@@ -1441,6 +1441,7 @@ int mog_get_tuning(mog_ctx* ctx, mog_tuning* out) {
   out->sorted_deposit = ctx->sorted_deposit ? 1 : 0;
   out->cell_cache = ctx->cell_cache ? 1 : 0;
   out->spectral_mfma = ctx->spectral_mfma ? 1 : 0;
+  out->deposit_prepass = ctx->tune.prepass;
   return MOG_OK;
 }
 
@@ -1456,11 +1457,14 @@ int mog_set_tuning(mog_ctx* ctx, const mog_tuning* t) {
   if (t->resort_every < 0) return fail(ctx, MOG_EINVAL, "resort_every must be >= 0");
   if ((t->sorted_deposit | t->cell_cache | t->spectral_mfma) & ~1)
     return fail(ctx, MOG_EINVAL, "flags must be 0 or 1");
+  if (t->deposit_prepass < -1 || t->deposit_prepass > 1)
+    return fail(ctx, MOG_EINVAL, "deposit_prepass must be -1, 0 or 1");
   if (t->cell_cache && !t->sorted_deposit)
     return fail(ctx, MOG_EINVAL, "cell_cache needs sorted_deposit");
   ctx->tune.threads = t->deposit_threads;
   ctx->tune.grids_per_wg = G;
   ctx->tune.wg_target = t->deposit_wg_target;
+  ctx->tune.prepass = t->deposit_prepass;
   const bool realloc = (t->sorted_deposit != 0) != ctx->sorted_deposit;
   ctx->resort_every = t->resort_every;
   ctx->sorted_deposit = t->sorted_deposit != 0;

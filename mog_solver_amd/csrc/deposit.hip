@@ -609,6 +609,11 @@ hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, co
                              unsigned short* cells_out, bool sorted, const DepositTune& t,
                              hipStream_t s, float4* cpos) {
   if (n <= 0) return hipSuccess;
+  // by default the prepass runs from 2^26 pairs, where it pays for itself
+  // (two disks, 2.4e8 pairs: deposit phase -1 %; 64 Plummer spheres,
+  // 9.6e8: -9 %; 65,536 particles x 16 grids: +8 %, the extra launch and
+  // pass cost more than the loop saves)
+  if (t.prepass < 0 || (t.prepass == 0 && (long long)n * P.k < (1ll << 26))) cpos = nullptr;
   const size_t grid_bytes = (size_t)P.cells * 4;
   if (grid_bytes + (size_t)kLaneSlots * 1024 * 4 > kMaxLds) {
     const int G = 8;

@@ -139,6 +139,7 @@ struct DepositTune {
   int threads;        // threads per workgroup
   int grids_per_wg;   // grids privatised in LDS per workgroup
   int wg_target;      // aim for about this many workgroups
+  int prepass = 0;    // prepared rows: 0 from 2^26 pairs, 1 always, -1 never
 };
 
 hipError_t launch_fill_zero(long long* p, long long count, hipStream_t s);

@@ -49,12 +49,16 @@ def edge_positions(W, g, n_rand, seed):
     return pos
 
 
+@pytest.mark.parametrize("prepass", [-1, 1])
 @pytest.mark.parametrize("k,g", [(16, 16), (5, 4), (7, 8), (3, 32), (2, 64), (2, 2)])
-def test_deposit_bit_exact(torch_cuda, k, g):
+def test_deposit_bit_exact(torch_cuda, k, g, prepass):
+    """Both forms of the LDS deposit: rows read from the positions, and
+    rows prepared by the prepass (centred coordinates, milli-mass)."""
     torch = torch_cuda
     W = 100.0
     pos = edge_positions(W, g, 20000, seed=k * 100 + g)
     s, rot, off = make_solver(k, g, W)
+    s.set_tuning(deposit_prepass=prepass)
     got = s.deposit(dev(torch, pos)).cpu().numpy()
     want = O.deposit(pos, rot, off, O.Params.make(k, g, W))
     assert got.dtype == np.int64 and got.shape == want.shape
@@ -74,8 +78,9 @@ def test_deposit_c1_and_permutation(torch_cuda):
     assert np.all(want.sum(axis=(1, 2, 3)) == 65536 * 1000)
 
 
+@pytest.mark.parametrize("prepass", [-1, 1])
 @pytest.mark.parametrize("k,g", [(12, 16), (3, 32)])
-def test_deposit_masses_and_large_values(torch_cuda, k, g):
+def test_deposit_masses_and_large_values(torch_cuda, k, g, prepass):
     """Fractional, zero, negative and very large masses (64-bit path), in
     the multi-grid (g = 16) and one-grid (g = 32) deposit tilings."""
     torch = torch_cuda
@@ -87,7 +92,9 @@ def test_deposit_masses_and_large_values(torch_cuda, k, g):
     pos[:100, 3] = 0.0
     pos[100:200, 3] = -1.0
     pos[200:300, 3] = np.float32(5e9)  # milli-mass 5e12: beyond 32 bits
+    pos[300:320, 3] = np.float32(np.nan)
     s, rot, off = make_solver(k, g)
+    s.set_tuning(deposit_prepass=prepass)
     want = O.deposit(pos, rot, off, O.Params.make(k, g, 100.0))
     got = s.deposit(dev(torch, pos)).cpu().numpy()
     assert np.array_equal(got, want)
@@ -214,8 +221,9 @@ def test_gather_matches_fp64(torch_cuda):
     assert O.rel_rms(got[:, :3], want) < 1e-6
 
 
+@pytest.mark.parametrize("prepass", [-1, 1])
 @pytest.mark.parametrize("k,g", [(5, 16), (3, 32)])
-def test_deposit_far_positions(torch_cuda, k, g):
+def test_deposit_far_positions(torch_cuda, k, g, prepass):
     """Positions far outside the box (|w| beyond the range of the fast
     cell's float fields) next to ordinary ones: the exact sequence takes them,
     and the fast cell a pair was first added to is taken back where it was
@@ -231,6 +239,7 @@ def test_deposit_far_positions(torch_cuda, k, g):
     pos[idx[:, 2], 2] = far[rng.integers(0, far.size, 3000)]
     pos[::7, 3] = np.float32(3e6)  # and some masses above the 32-bit partials
     s, rot, off = make_solver(k, g, W)
+    s.set_tuning(deposit_prepass=prepass)
     got = s.deposit(dev(torch, pos)).cpu().numpy()
     want = O.deposit(pos, rot, off, O.Params.make(k, g, W))
     assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} cells differ"
