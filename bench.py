@@ -41,12 +41,13 @@ FP32_VECTOR_PEAK_TFS = 157.3
 # 256 MiB copy reads as FETCH 128 MiB and WRITE 256 MiB, so FETCH_SIZE is
 # scaled by 2.0 and WRITE_SIZE by 1.0.  Figures and commands:
 # profiles/pmc_traffic_two_disks_n1e6_k240_g16.md.  Used only when the run
-# matches (workload, n, k, g, precision, grids per deposit workgroup,
-# deposit workgroup target): the deposit's re-reads of the positions depend
-# on its tiling.  The deposit phase is its prepass plus the LDS kernel.
+# matches (workload, n, k, g, precision); the deposit's figure also needs
+# the same (grids per workgroup, workgroup target), since its re-reads of the
+# positions depend on its tiling.  The deposit phase is its prepass plus
+# the LDS kernel.
 PMC_TRAFFIC_KIB = {
-    ("two_disks", 1_000_000, 240, 16, "fp32", 2, 2048): {
-        "deposit": (7_818.1 + 10_505.8, 15_625.0 + 35_177.7),
+    ("two_disks", 1_000_000, 240, 16, "fp32"): {
+        ("deposit", 2, 2048): (7_818.1 + 10_505.8, 15_625.0 + 35_177.7),
         "gather_update": (123_321.4, 73_737.6),
         "solve": (3_735.3, 14_242.9),
     },
@@ -55,7 +56,7 @@ PMC_TRAFFIC_KIB = {
 # for the kernels of the same tiling, profiles/sq_counters_two_disks_n1e6_k240_g16.md):
 # the deposit and the gather are bound by vector-instruction issue.
 SQ_VALU_PER_WAVE_PAIR = {
-    ("two_disks", 1_000_000, 240, 16, "fp32", 2, 2048): {"deposit": 25.66, "gather_update": 31.10},
+    ("two_disks", 1_000_000, 240, 16, "fp32"): {("deposit", 2, 2048): 25.66, "gather_update": 31.10},
 }
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
@@ -269,17 +270,20 @@ def main():
     dom = max(("deposit", "gather_update", "solve"), key=lambda key: phases[key])
     achieved = alg_bytes[dom] / (phases[dom] / 1e3) / 1e9
     tuning = solver.get_tuning()
-    key = (args.workload, n, k, g, args.precision, tuning["deposit_grids_per_wg"], tuning["deposit_wg_target"])
+    key = (args.workload, n, k, g, args.precision)
+    entry = dom
+    if dom == "deposit":
+        entry = ("deposit", tuning["deposit_grids_per_wg"], tuning["deposit_wg_target"])
     measured = PMC_TRAFFIC_KIB.get(key, {})
     traffic = None
-    if dom in measured:
-        fetch_kib, write_kib = measured[dom]
+    if entry in measured:
+        fetch_kib, write_kib = measured[entry]
         traffic = round((fetch_kib * FETCH_FACTOR + write_kib * WRITE_FACTOR) * 1024.0)
     # the limit the SQ counters show: vector-instruction issue.  Issue rate
     # of the timed phase = profiled instructions per pair x pairs / (SIMDs x
     # cycles); peak = one wave64 instruction per 2 SIMD-cycles
     issue = None
-    per_pair = SQ_VALU_PER_WAVE_PAIR.get(key, {}).get(dom)
+    per_pair = SQ_VALU_PER_WAVE_PAIR.get(key, {}).get(entry)
     if per_pair is not None:
         instr = per_pair * float(n) * k / 64.0
         simd_cycles = SIMDS * CLOCK_HZ * phases[dom] / 1e3
