@@ -215,12 +215,14 @@ hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const D
 // Blocks of kBatchGatherBlock particles, each inside one simulation, so the
 // simulation (and with it every transform row) is uniform across the
 // workgroup and comes through scalar loads, as in the single-simulation
-// gather.
+// gather, whose deferral of undecided pairs (and so its summation order) it
+// repeats.
 __global__ __launch_bounds__(kBatchGatherBlock) void batch_gather_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, long long blocks,
     const BatchSim* __restrict__ sims, int nsims, const float* __restrict__ rot,
     const float* __restrict__ off, const float* __restrict__ frot,
     const float4* __restrict__ force, const int* __restrict__ order) {
+  __shared__ unsigned recs[kGatherSlots * kBatchGatherBlock];
   for (long long blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
     const BatchSim& b = sims[sim_of_block(sims, nsims, blk)];
     const long long j = (blk - b.blk_off) * kBatchGatherBlock + threadIdx.x;
@@ -237,9 +239,10 @@ __global__ __launch_bounds__(kBatchGatherBlock) void batch_gather_kernel(
     const int pfast = particle_fast(c0, c1, c2, P.CB);
     float tx = 0.f, ty = 0.f, tz = 0.f;
     // groups of U grids: the fast cells and their fetches first, then one
-    // wave-uniform test re-fetches undecided pairs from the exact cells; the
-    // forces are summed in grid order, as in the single-simulation gather
+    // wave-uniform test records undecided pairs, whose exact cells follow the
+    // loop; the summation order is the single-simulation gather's
     constexpr int U = 4;
+    int nrec = 0;
     int gr = 0;
     for (; gr + U <= P.k; gr += U) {
       // one compare per group: the largest face distance of its U pairs
@@ -260,9 +263,15 @@ __global__ __launch_bounds__(kBatchGatherBlock) void batch_gather_kernel(
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           if (!(pfast & (d[u] < P.lim))) {
-            const int ex = cell_exact_call(c0, c1, c2, R + 12 * (gr + u), O + 4 * (gr + u), P.W,
-                                           P.half, P.h, P.lg);
-            f[u] = cell_load(F + (long long)(gr + u) * P.cells, ex);
+            if (nrec < kGatherSlots) {
+              recs[nrec * kBatchGatherBlock + threadIdx.x] = (unsigned)(gr + u);
+              f[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+              const int ex = cell_exact_call(c0, c1, c2, R + 12 * (gr + u), O + 4 * (gr + u), P.W,
+                                             P.half, P.h, P.lg);
+              f[u] = cell_load(F + (long long)(gr + u) * P.cells, ex);
+            }
+            ++nrec;
           }
         }
       }
@@ -276,6 +285,16 @@ __global__ __launch_bounds__(kBatchGatherBlock) void batch_gather_kernel(
     for (; gr < P.k; ++gr) {
       const int cell = cell_index(c0, c1, c2, pfast, FR + 12 * gr, R + 12 * gr, O + 4 * gr, P);
       const float4 f = cell_load(F + (long long)gr * P.cells, cell);
+      tx += f.x;
+      ty += f.y;
+      tz += f.z;
+    }
+    const int mine = min(nrec, kGatherSlots);
+    for (int r = 0; r < mine; ++r) {
+      const int gd = (int)recs[r * kBatchGatherBlock + threadIdx.x];
+      const int ex =
+          cell_exact_call(c0, c1, c2, R + 12 * gd, O + 4 * gd, P.W, P.half, P.h, P.lg);
+      const float4 f = cell_load(F + (long long)gd * P.cells, ex);
       tx += f.x;
       ty += f.y;
       tz += f.z;

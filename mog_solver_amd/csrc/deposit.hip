@@ -443,7 +443,9 @@ __device__ __forceinline__ void wave_aggregate_add(unsigned int* hist, int cell,
 // Deposit over positions in Morton order (spos[t] = pos[order[t]]): the
 // lanes of a wave are neighbours in space and mostly share cells, so LDS
 // adds are aggregated per wave.  With CACHE the cell of every (grid, slot)
-// pair is stored in cells_out[cache_index(t, grid, k)] for the gather.
+// pair is stored in cells_out[cache_index(t, grid, k)] for the gather, with
+// bit 15 set where the fast path could not decide (the gather defers those
+// pairs; g <= 32, so cells fit 15 bits).
 template <int G, bool CACHE>
 __global__ __launch_bounds__(1024) void deposit_sorted_kernel(
     const float4* __restrict__ spos, long long n, long long chunk,
@@ -484,8 +486,11 @@ __global__ __launch_bounds__(1024) void deposit_sorted_kernel(
     for (int gi = 0; gi < G; ++gi) {
       if (gi < ng) {
         const int gr = grid0 + gi;
-        const int cell = cell_index(c0, c1, c2, pfast, F[gi], rot + 12 * gr, off + 4 * gr, P);
-        if (CACHE && live) cells_out[cache_index(p, gr, P.k)] = (unsigned short)cell;
+        int cell;
+        const int ok = cell_fast(c0, c1, c2, F[gi], P.lim, P.lg, &cell) & pfast;
+        if (!ok) cell = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half, P.h, P.lg);
+        if (CACHE && live)
+          cells_out[cache_index(p, gr, P.k)] = (unsigned short)(cell | (ok ? 0 : 0x8000));
         if (glob) atomicAdd(&grids[(long long)gr * cells + cell], milli);
         wave_aggregate_add(hist + gi * cells, cell, lds_v);
       }

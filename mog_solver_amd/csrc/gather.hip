@@ -35,6 +35,15 @@ __device__ __forceinline__ void accumulate_rotated(const float* __restrict__ R, 
 // SH: the fast cells come as byte offsets into the float4 grid through the
 // shifted fields of cellmath.h (cell_fast_o), for transforms that satisfy
 // field_shift_ok(lg, 4, ...); otherwise as indices (cell_fast_d).
+//
+// Undecided pairs (a face within the fast path's margin, or a particle
+// outside the admitted bound) are deferred: inside the rarely taken branch a
+// lane only records the grid in its LDS slots and drops the fast fetch;
+// after the grid loop it evaluates the exact cells of its records and adds
+// their forces.  A wave then runs about one exact evaluation per particle
+// pass (the most records of any of its lanes) instead of one per group in
+// which any lane needs one.  The forces are summed in grid order except the
+// deferred ones, which come last.
 template <bool WORLD, bool LDS3, bool SH>
 __global__ __launch_bounds__(256) void gather_update_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, long long n,
@@ -42,7 +51,8 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
     const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode,
     const int* __restrict__ order, long long* __restrict__ zero, long long zero_count) {
   constexpr int U = 4;
-  extern __shared__ float adds[];  // LDS3: adds[3 g + m] = frot[12 g + 4 m + 3]
+  extern __shared__ float adds[];  // LDS3: adds[3 g + m] = frot[12 g + 4 m + 3]; then the records
+  unsigned* const recs = reinterpret_cast<unsigned*>(adds + (LDS3 ? 3 * P.k : 0));
   // side job: clear `zero` (the density grids the next step deposits into)
   for (long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x; z < zero_count;
        z += (long long)gridDim.x * blockDim.x)
@@ -70,12 +80,13 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
     FS.m012 = to_vgpr(FS.m012);
   }
   float tx = 0.f, ty = 0.f, tz = 0.f;
+  int nrec = 0;  // undecided pairs met so far
   int grid = 0;
   for (; grid + U <= P.k; grid += U) {
     // the U fast cells (always in range) and their force fetches first, then
-    // one wave-uniform test: only a wave with an undecided pair re-fetches
-    // those pairs from the exact cells (the exact sequence is a call, whose
-    // entry waits for every load in flight)
+    // one wave-uniform test: only a wave with an undecided pair enters the
+    // branch, which records those pairs (the exact sequence is a call, whose
+    // entry waits for every load in flight: it runs after the loop)
     // one compare per group: the largest face distance of its U pairs
     unsigned cell[U];  // byte offsets under SH
     float d[U];
@@ -105,9 +116,15 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
       for (int u = 0; u < U; ++u) {
         if (!(pfast & (d[u] < P.lim))) {
           const int gr = grid + u;
-          const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half,
-                                         P.h, P.lg);
-          F[u] = cell_load(force + (long long)gr * P.cells, ex);
+          if (nrec < kGatherSlots) {
+            recs[nrec * blockDim.x + threadIdx.x] = (unsigned)gr;
+            F[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+          } else {
+            const int ex = cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half,
+                                           P.h, P.lg);
+            F[u] = cell_load(force + (long long)gr * P.cells, ex);
+          }
+          ++nrec;
         }
       }
     }
@@ -134,6 +151,21 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
       accumulate_rotated(rot + 12 * grid, F, tx, ty, tz);
     }
   }
+  // the deferred pairs, in the order they were met
+  const int mine = min(nrec, kGatherSlots);
+  for (int r = 0; r < mine; ++r) {
+    const int gr = (int)recs[r * blockDim.x + threadIdx.x];
+    const int ex =
+        cell_exact_call(c0, c1, c2, rot + 12 * gr, off + 4 * gr, P.W, P.half, P.h, P.lg);
+    const float4 F = cell_load(force + (long long)gr * P.cells, ex);
+    if (WORLD) {
+      tx += F.x;
+      ty += F.y;
+      tz += F.z;
+    } else {
+      accumulate_rotated(rot + 12 * gr, F, tx, ty, tz);
+    }
+  }
   const float ax = tx / P.fk, ay = ty / P.fk, az = tz / P.fk;
   if (mode == 1) {
     avg_out[i] = make_float4(ax, ay, az, 0.0f);
@@ -147,31 +179,60 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
 
 // Gather + update from the cell cache of the step's sorted deposit: slot t
 // reads its k cells with coalesced uint16 loads (no cell arithmetic), the
-// world-frame forces of those cells, then updates particle order[t].
+// world-frame forces of those cells, then updates particle order[t].  Pairs
+// the fast path could not decide (bit 15 of the cached cell) are summed as
+// gather_update_kernel sums them: the first kGatherSlots of them in the
+// grids below k - k % 4 after all others, the rest in grid order, so both
+// gathers give the same bits.
 __global__ __launch_bounds__(256) void gather_cached_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, long long n, const int* __restrict__ order,
     const unsigned short* __restrict__ cells, const float4* __restrict__ force, Params P) {
   constexpr int U = 8;
+  __shared__ unsigned recs[kGatherSlots * 256];
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  const int k4 = P.k & ~3;  // the grids gather_update_kernel takes in groups of 4
   float tx = 0.f, ty = 0.f, tz = 0.f;
+  int nrec = 0;
+  auto add = [&](int grid, unsigned c) {
+    if ((c & 0x8000u) && grid < k4) {
+      if (nrec < kGatherSlots) {
+        recs[nrec * blockDim.x + threadIdx.x] = (unsigned)grid;
+        ++nrec;
+        return;
+      }
+      ++nrec;
+    }
+    const float4 F = force[(long long)grid * P.cells + (c & 0x7FFFu)];
+    tx += F.x;
+    ty += F.y;
+    tz += F.z;
+  };
   int grid = 0;
   for (; grid + U <= P.k; grid += U) {
-    int c[U];
+    unsigned c[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) c[u] = cells[cache_index(t, grid + u, P.k)];
-    float4 F[U];
+    if (__ballot(((c[0] | c[1] | c[2] | c[3] | c[4] | c[5] | c[6] | c[7]) & 0x8000u) != 0)) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) F[u] = force[(long long)(grid + u) * P.cells + c[u]];
+      for (int u = 0; u < U; ++u) add(grid + u, c[u]);
+    } else {
+      float4 F[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      tx += F[u].x;
-      ty += F[u].y;
-      tz += F[u].z;
+      for (int u = 0; u < U; ++u) F[u] = force[(long long)(grid + u) * P.cells + c[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        tx += F[u].x;
+        ty += F[u].y;
+        tz += F[u].z;
+      }
     }
   }
-  for (; grid < P.k; ++grid) {
-    const float4 F = force[(long long)grid * P.cells + cells[cache_index(t, grid, P.k)]];
+  for (; grid < P.k; ++grid) add(grid, cells[cache_index(t, grid, P.k)]);
+  const int mine = min(nrec, kGatherSlots);
+  for (int r = 0; r < mine; ++r) {
+    const int gr = (int)recs[r * blockDim.x + threadIdx.x];
+    const float4 F = force[(long long)gr * P.cells + (cells[cache_index(t, gr, P.k)] & 0x7FFFu)];
     tx += F.x;
     ty += F.y;
     tz += F.z;
@@ -216,11 +277,13 @@ hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long 
   const long long blocks = (n + 255) / 256;
   const size_t lds = (size_t)P.k * 3 * sizeof(float);
   const bool lds3 = lds <= 16 * 1024;  // k <= 1365: a few KB per workgroup
+  const size_t lds_recs = (size_t)kGatherSlots * 256 * sizeof(unsigned);
   const bool sh = P.shift_ok & kShiftF4;
   const dim3 grid((unsigned)blocks);
-#define MOG_GATHER(W3, L3, S3)                                                                    \
-  hipLaunchKernelGGL((gather_update_kernel<W3, L3, S3>), grid, dim3(256), L3 ? lds : 0, s, pos, vel, \
-                     n, rot, off, frot, force, P, avg_out, mode, order, zero, zero_count)
+#define MOG_GATHER(W3, L3, S3)                                                             \
+  hipLaunchKernelGGL((gather_update_kernel<W3, L3, S3>), grid, dim3(256),                 \
+                     (L3 ? lds : 0) + lds_recs, s, pos, vel, n, rot, off, frot, force, P, \
+                     avg_out, mode, order, zero, zero_count)
   if (world && lds3 && sh) MOG_GATHER(true, true, true);
   else if (world && lds3) MOG_GATHER(true, true, false);
   else if (world && sh) MOG_GATHER(true, false, true);

@@ -268,6 +268,10 @@ struct DepositTask {
 constexpr int kBatchLdsCells = 32768;
 // Particles per gather block; a block never straddles two simulations.
 constexpr int kBatchGatherBlock = 256;
+// Deferred undecided pairs per lane of a gather (grid indices in LDS): the
+// exact cells of a lane's first kGatherSlots undecided pairs are evaluated
+// after its grid loop, the rest on the spot.
+constexpr int kGatherSlots = 4;
 
 // The simulation owning gather block `blk` (sims ordered by blk_off); the
 // result is uniform across a workgroup that passes a uniform blk.
