@@ -37,6 +37,7 @@ struct mog_ctx {
   long long* d_grids;
   long long* d_grids_next;
   float2* d_S1;
+  long long s1_elems;  // float2 elements of d_S1
   float2* d_S2;
   float4* d_force;
   float2* d_tw;
@@ -354,6 +355,17 @@ void build_conv_tables(const mog_config* c, float2* out) {
     out[i] = make_float2((float)yz.real(), (float)yz.imag());
     out[g3 + i] = make_float2((float)x.real(), (float)x.imag());
   }
+  if (g == 32) {  // half-spectrum multipliers [d][kz][ky][kx <= g/2] of the g = 32 solve
+    const int hx = g / 2 + 1;
+    float2* half = out + 2 * g3;
+    for (int d = 0; d < 3; ++d)
+      for (int iz = 0; iz < g; ++iz)
+        for (int iy = 0; iy < g; ++iy)
+          for (int ix = 0; ix < hx; ++ix) {
+            const std::complex<double> t = scale * K[d][((size_t)iz * g + iy) * g + ix];
+            half[(((size_t)d * g + iz) * g + iy) * hx + ix] = make_float2((float)t.real(), (float)t.imag());
+          }
+  }
 }
 
 Params make_params(const mog_config* c) {
@@ -544,7 +556,8 @@ hipError_t solve_dispatch(mog_ctx* ctx, const long long* grids, float4* force, i
     return launch_solve_mfma(P, ctx->cfg.precision, grids, ctx->d_S1, ctx->d_S2, force, ctx->d_Wf,
                              ctx->d_Wi, ctx->d_gtab, ctx->d_kd, ctx->d_conv, ctx->need_x,
                              ctx->d_rot, world, s);
-  return launch_solve(P, grids, ctx->d_S1, ctx->d_S2, force, ctx->d_tw, ctx->d_gtab, ctx->d_kd,
+  return launch_solve(P, grids, ctx->d_S1, ctx->s1_elems, ctx->d_S2, force, ctx->d_tw,
+                      ctx->d_gtab, ctx->d_kd,
                       ctx->d_conv, ctx->need_x, ctx->d_rot, world, s);
 }
 
@@ -662,6 +675,7 @@ struct mog_batch {
   float* d_frot;
   long long* d_grids;
   float2* d_S1;
+  long long s1_elems;
   float2* d_S2;
   float4* d_force;
   float2* d_tw;
@@ -800,8 +814,16 @@ hipError_t batch_enqueue(mog_batch* b, float* pos, float* vel, hipStream_t s) {
     const mog_batch::SolveClass& sc = b->cls[win.cls];
     e = launch_batch_solve(b->d_sims, b->d_cls_sim + sc.start, b->d_cls_off + sc.start, sc.count,
                            win.slab0, win.slabs, b->d_win_soff + w * (size_t)b->nsims, sc.gmax,
-                           b->d_grids, b->d_S1, b->d_S2, b->d_force, b->d_tw, b->d_gtab, b->d_kd,
-                           b->d_conv, b->d_rot, s);
+                           b->d_grids, b->d_S1, b->s1_elems, b->d_S2, b->d_force, b->d_tw,
+                           b->d_gtab, b->d_kd, b->d_conv, b->d_rot, s);
+  }
+  for (int ci = 0; ci < kBatchClasses && e == hipSuccess; ++ci) {
+    const mog_batch::SolveClass& sc = b->cls[ci];
+    if (sc.count == 0 || !batch_class_fused(sc.gmax)) continue;
+    e = launch_batch_solve(b->d_sims, b->d_cls_sim + sc.start, b->d_cls_off + sc.start, sc.count,
+                           0, sc.slabs, nullptr, sc.gmax, b->d_grids, b->d_S1, b->s1_elems,
+                           b->d_S2, b->d_force, b->d_tw, b->d_gtab, b->d_kd, b->d_conv, b->d_rot,
+                           s);
   }
   if (e == hipSuccess)
     e = launch_batch_gather((float4*)pos, (float4*)vel, b->gather_blocks, b->d_sims, b->nsims,
@@ -851,10 +873,14 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
     return fail(nullptr, MOG_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
   }
   const size_t k = (size_t)cfg->n_grids, g = (size_t)cfg->grid_size, cells = k * g * g * g;
+  // the matrix-core passes use S1 as one complex field; the g = 32 vector
+  // solve as one scratch slot per workgroup
+  c->s1_elems = (long long)cells;
+  if (g == 32) c->s1_elems = std::max(c->s1_elems, rh_scratch_elems((long long)k));
   if (dalloc(&c->d_rot, k * 12) != hipSuccess || dalloc(&c->d_off, k * 4) != hipSuccess ||
       dalloc(&c->d_frot, k * 12) != hipSuccess ||
       dalloc(&c->d_grids, cells) != hipSuccess || dalloc(&c->d_grids_next, cells) != hipSuccess ||
-      dalloc(&c->d_S1, cells) != hipSuccess ||
+      dalloc(&c->d_S1, (size_t)c->s1_elems) != hipSuccess ||
       dalloc(&c->d_S2, c->need_x ? cells : 1) != hipSuccess ||
       dalloc(&c->d_force, cells) != hipSuccess || dalloc(&c->d_tw, g) != hipSuccess ||
       dalloc(&c->d_gtab, g * g * g) != hipSuccess || dalloc(&c->d_kd, g) != hipSuccess ||
@@ -886,9 +912,9 @@ int mog_create(const mog_config* cfg, mog_ctx** out) {
       cfg->compat == MOG_COMPAT_PHYSICAL) {
     // the reference runs the FFT solver for either key (main.py:671-672);
     // the real-space convolution is the physical mode's meaning of the key
-    std::vector<float2> conv(2 * g * g * g);
+    std::vector<float2> conv((size_t)conv_table_elems((int)g));
     build_conv_tables(cfg, conv.data());
-    if (dalloc(&c->d_conv, 2 * g * g * g) != hipSuccess) e = hipErrorOutOfMemory;
+    if (dalloc(&c->d_conv, conv.size()) != hipSuccess) e = hipErrorOutOfMemory;
     if (e == hipSuccess)
       e = hipMemcpy(c->d_conv, conv.data(), conv.size() * sizeof(float2), hipMemcpyHostToDevice);
   }
@@ -1294,7 +1320,8 @@ int mog_closure_features(mog_ctx* ctx, const float* pos, int64_t n, double rho0,
                        (unsigned long long*)ctx->d_grids, ctx->tune, s);
   if (e == hipSuccess) e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
   if (e == hipSuccess)
-    e = launch_solve(P, ctx->d_grids, ctx->d_S1, ctx->d_S2, ctx->d_force_dc, ctx->d_tw,
+    e = launch_solve(P, ctx->d_grids, ctx->d_S1, ctx->s1_elems, ctx->d_S2, ctx->d_force_dc,
+                     ctx->d_tw,
                      ctx->d_gtab_dc, ctx->d_kd, nullptr, ctx->need_x, ctx->d_rot, 1, s);
   const int* order = nullptr;
   if (e == hipSuccess) e = refresh_order(ctx, pos, n, s, &order);
@@ -1535,7 +1562,7 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
     grid_off += c->n_grids;
     tab_off += (int)(g * g * g);
     line_off += (int)g;
-    if (conv) conv_off += 2 * g * g * g;
+    if (conv) conv_off += conv_table_elems((int)g);
   }
   if (p_off > 2147483647LL || cell_off > 2147483647LL) {
     delete b;
@@ -1555,10 +1582,14 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
     all_off.insert(all_off.end(), cls_off[ci].begin(), cls_off[ci].end());
   }
   b->grids_total = grid_off;
-  // solve windows: whole grids of one class, at most kBatchWindowCells
-  // cells each, laid out in the window's scratch in slab order
+  // solve windows of the classes g <= 8 (the slab passes): whole grids of
+  // one class, at most kBatchWindowCells cells each, laid out in the
+  // window's scratch in slab order.  The g = 16 and g = 32 classes run one
+  // launch each over all of their grids (one workgroup per grid).
   std::vector<long long> win_soff;
+  long long win_max = 0;
   for (int ci = 0; ci < kBatchClasses; ++ci) {
+    if (batch_class_fused(cls_gmax[ci])) continue;
     long long used = 0;
     for (size_t jj = 0; jj < cls_sim[ci].size(); ++jj) {
       const mog_config* c = &cfgs[cls_sim[ci][jj]];
@@ -1578,6 +1609,7 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
         }
         b->windows.back().slabs += g;
         used += g3;
+        win_max = std::max(win_max, used);
       }
     }
   }
@@ -1592,9 +1624,10 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
   A(dalloc(&b->d_off, (size_t)grid_off * 4));
   A(dalloc(&b->d_frot, (size_t)grid_off * 12));
   A(dalloc(&b->d_grids, (size_t)cell_off));
-  const long long win_cells = std::min(cell_off, kBatchWindowCells);
-  A(dalloc(&b->d_S1, (size_t)win_cells));
-  A(dalloc(&b->d_S2, (size_t)win_cells));
+  const long long g32_grids = cls_gmax[4] == 32 ? cls_slabs[4] / 32 : 0;
+  b->s1_elems = std::max(win_max, g32_grids > 0 ? rh_scratch_elems(g32_grids) : 0LL);
+  A(dalloc(&b->d_S1, (size_t)std::max(b->s1_elems, 1LL)));
+  A(dalloc(&b->d_S2, (size_t)std::max(win_max, 1LL)));
   A(dalloc(&b->d_win_soff, win_soff.size() > 0 ? win_soff.size() : 1));
   A(dalloc(&b->d_force, (size_t)cell_off));
   A(dalloc(&b->d_tw, (size_t)line_off));

@@ -167,10 +167,21 @@ hipError_t launch_gather_cached(const Params& P, float4* pos, float4* vel, long 
                                 const float4* force, hipStream_t s);
 // conv != NULL: multiply by the spectrum of a real-space kernel instead of
 // the Green's function x i*k (conv[0..g^3) = T_y + i T_z, conv[g^3..) = T_x).
-hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
-                        float4* force, const float2* tw, const float* gtab, const float* kd,
-                        const float2* conv, int need_x, const float* rot, int world,
-                        hipStream_t s);
+hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, long long s1_elems,
+                        float2* S2, float4* force, const float2* tw, const float* gtab,
+                        const float* kd, const float2* conv, int need_x, const float* rot,
+                        int world, hipStream_t s);
+// float2 elements of S1 the g = 32 solve needs for `ngrids` grids (one
+// scratch slot per workgroup, at most one workgroup per CU)
+long long rh_scratch_elems(long long ngrids);
+// Entries of the convolution solver's multiplier table of one simulation:
+// T_y + i T_z and T_x over the full spectrum (2 g^3), then for g = 32 the
+// three half-spectrum multipliers T_x, T_y, T_z (kx <= g/2) of the g = 32
+// kernel.
+inline long long conv_table_elems(int g) {
+  const long long g3 = (long long)g * g * g;
+  return 2 * g3 + (g == 32 ? 3LL * g * g * (g / 2 + 1) : 0);
+}
 // MFMA spectral solve (g >= 16): precision 0 FP32, 1 BF16X3, 2 FP16.
 // Wf / Wi: forward / inverse DFT tables, each [Re W | -Im W] then
 // [Im W | Re W] as g rows of 2g floats.
@@ -323,14 +334,18 @@ hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const D
 // grid `gmax`.
 hipError_t launch_batch_solve(const BatchSim* sims, const int* cls_sim, const long long* cls_off,
                               int ncls, long long slab0, long long slabs, const long long* soff,
-                              int gmax, const long long* grids, float2* S1, float2* S2,
-                              float4* force, const float2* tw, const float* gtab, const float* kd,
-                              const float2* conv, const float* rot, hipStream_t s);
-// Cells of the batch's spectral scratch window: the solve runs window by
-// window through one reused pair of complex buffers (2 x 32 MiB), which so
-// stay in the Infinity Cache instead of streaming every simulation's
-// scratch through HBM.
+                              int gmax, const long long* grids, float2* S1, long long s1_elems,
+                              float2* S2, float4* force, const float2* tw, const float* gtab,
+                              const float* kd, const float2* conv, const float* rot,
+                              hipStream_t s);
+// Cells of the batch's spectral scratch window for the grid sizes that run
+// the slab passes (g <= 8): their solve runs window by window through one
+// reused pair of complex buffers, which so stay in the Infinity Cache
+// instead of streaming every simulation's scratch through HBM.
 constexpr long long kBatchWindowCells = 1ll << 22;
+// Grid-size classes whose solve is one launch over all of the class's
+// grids (one workgroup per grid, no windows).
+inline bool batch_class_fused(int gmax) { return gmax == 16 || gmax == 32; }
 hipError_t launch_batch_gather(float4* pos, float4* vel, long long blocks, const BatchSim* sims,
                                int nsims, const float* rot, const float* off, const float* frot,
                                const float4* force, const int* order, hipStream_t s);

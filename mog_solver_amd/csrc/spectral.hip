@@ -5,11 +5,13 @@
 // fourier.glsl, the Green's multiply and i*k gradients of greens.glsl:25-61,
 // the three inverse transforms (main.py:638-646) and pack.glsl.
 //
-// Each length-g transform is a dense DFT contraction with a twiddle table
-// (FP64-derived, stored FP32) for g <= 8, and for g >= 16 a radix-4/2
-// Stockham FFT over all lines of the slab at once, in LDS (g log2 g instead
-// of g^2 complex multiply-adds per line).  The 3D transform of a grid is
-// split into three passes over an L2-resident complex scratch buffer:
+// g = 16 and g = 32 run one workgroup per grid with register FFTs (the
+// grid kernels below).  For the other sizes each length-g transform is a
+// dense DFT contraction with a twiddle table (FP64-derived, stored FP32) for
+// g <= 8, and for g = 64 a radix-8/4/2 Stockham FFT over all lines of the
+// slab at once, in LDS (g log2 g instead of g^2 complex multiply-adds per
+// line); the 3D transform of a grid is then split into three passes over an
+// L2-resident complex scratch buffer:
 //   A  per (grid, z) slab: real x-DFT, y-DFT                      -> S1
 //   B  per (grid, ky) slab: z-DFT, Green x i*k, inverse z-DFT      -> S1 (y+iz), S2 (x)
 //   C  per (grid, z) slab: inverse y-DFT, inverse x-DFT, pack float4
@@ -475,23 +477,17 @@ __global__ __launch_bounds__(256) void spec_inv_xy_kernel(const float2* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Register-FFT passes for g = 16 and 32: every length-g line is transformed
-// by one thread in registers (regfft.h), so an axis costs one LDS (or
-// global) read and one write per element and no barrier inside it.  The
-// same three passes as above, with the same data layout in S1/S2:
-//   A  per (grid, z) slab: x- then y-transform                  -> S1
-//   B  per (grid, ky, kx) line along z: forward z, Green x i*k, inverse z of
-//      the y+iz spectrum (and of the x spectrum)                -> S1, S2
-//   C  per (grid, z) slab: inverse y, inverse x, pack float4
-// Pass B needs no LDS: a wave's lanes take consecutive kx, so its loads and
-// stores of one z plane are contiguous.  Passes A and C stage rows in LDS
-// with a padded row stride of g + 1 (lanes of a row read hit distinct
-// banks) and take 256 / g (A) or 128 / g (C, two fields) slabs per
-// workgroup of 256 threads.
+// Register-FFT solves for g = 16 and 32: every length-g line is transformed
+// by one thread in registers (regfft.h), so an axis costs one LDS read and
+// one write per element and no barrier inside it, and the whole solve of a
+// grid runs in one workgroup (rf_grid_kernel, g = 16: the full complex
+// spectrum in LDS; rh_grid_kernel, g = 32: the half spectrum of the real
+// density in LDS, the three force components through a per-workgroup
+// scratch slot).
 // ---------------------------------------------------------------------------
 // Where slab t of a launch lives: its simulation's arrays and tables and
-// the slab's index inside that simulation (slab = grid * g + z in passes A
-// and C, grid * g + ky in pass B).
+// the slab's index inside that simulation (the grid kernels ask for slab
+// grid * g: local / g is the grid).
 struct RfSlab {
   const long long* mass;
   float2* S;
@@ -540,46 +536,6 @@ struct RfBatch {
   __device__ RfSlab operator()(long long t) const;
 };
 
-template <int G, typename R>
-__device__ __forceinline__ void rf_fwd_xy(const R& res, long long t0, long long nslabs,
-                                          long long blk) {
-  constexpr int SL = 256 / G, GG = G * G, RS = G + 1;
-  extern __shared__ float2 sm[];
-  __shared__ const long long* s_mass[SL];
-  __shared__ float2* s_S[SL];
-  const long long first = blk * SL;
-  if (threadIdx.x < SL && first + threadIdx.x < nslabs) {
-    const RfSlab a = res(t0 + first + threadIdx.x);
-    s_mass[threadIdx.x] = a.mass + a.local * GG;
-    s_S[threadIdx.x] = a.S + a.local * GG;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < SL * GG; i += blockDim.x) {
-    const int sl = i / GG, r = (i / G) % G, c = i % G;
-    const float v = first + sl < nslabs ? (float)s_mass[sl][r * G + c] : 0.f;
-    sm[(sl * G + r) * RS + c] = make_float2(v, 0.f);
-  }
-  __syncthreads();
-  const int sl = threadIdx.x / G, line = threadIdx.x % G;
-  float2 v[G];
-  float2* row = sm + (sl * G + line) * RS;  // x line: row `line` of slab sl
-#pragma unroll
-  for (int x = 0; x < G; ++x) v[x] = row[x];
-  reg_fft<G, false>(v);
-#pragma unroll
-  for (int x = 0; x < G; ++x) row[x] = v[x];
-  __syncthreads();
-  float2* col = sm + sl * G * RS + line;  // y line: column kx = `line`
-#pragma unroll
-  for (int y = 0; y < G; ++y) v[y] = col[y * RS];
-  reg_fft<G, false>(v);
-  if (first + sl < nslabs) {
-    float2* dst = s_S[sl];
-#pragma unroll
-    for (int ky = 0; ky < G; ++ky) dst[ky * G + line] = v[ky];
-  }
-}
-
 // The spectral multiply of one z line (ky, kx) of a grid's spectrum:
 // a[kz] (in: the density spectrum) becomes the packed y + i z force
 // spectrum, xs[kz] the x force spectrum; Green's function times i k
@@ -606,107 +562,6 @@ __device__ __forceinline__ void rf_multiply(const RfSlab& a0, int ky, int kx, fl
       xs[kz] = make_float2(-(pi * kxv), pr * kxv);
     }
   }
-}
-
-// Pass B: line `line` of the launch = (slab t0 + line / G, kx = line % G).
-template <int G, typename R>
-__device__ __forceinline__ void rf_z_green(const R& res, long long t0, long long nlines,
-                                           long long line) {
-  constexpr int GG = G * G;
-  if (line >= nlines) return;
-  const RfSlab a0 = res(t0 + line / G);
-  const long long grid = a0.local / G;
-  const int ky = (int)(a0.local % G), kx = (int)(line % G);
-  const long long at0 = grid * G * GG + ky * G + kx;
-  float2 a[G], xs[G];
-#pragma unroll
-  for (int z = 0; z < G; ++z) a[z] = a0.S[at0 + (long long)z * GG];
-  reg_fft<G, false>(a);
-  rf_multiply<G>(a0, ky, kx, a, xs);
-  reg_fft<G, true>(a);
-#pragma unroll
-  for (int z = 0; z < G; ++z) a0.S[at0 + (long long)z * GG] = a[z];
-  if (a0.need_x) {
-    reg_fft<G, true>(xs);
-#pragma unroll
-    for (int z = 0; z < G; ++z) a0.S2[at0 + (long long)z * GG] = xs[z];
-  }
-}
-
-// Pass C; world != 0 stores M F with the simulation's rotation rows.
-template <int G, typename R>
-__device__ __forceinline__ void rf_inv_xy(const R& res, long long t0, long long nslabs, int world,
-                                          long long blk) {
-  constexpr int SL = 128 / G, GG = G * G, RS = G + 1;
-  extern __shared__ float2 sm[];  // [sl][field][y][x], rows padded to RS
-  __shared__ float4* s_force[SL];
-  __shared__ const float* s_rot[SL];
-  __shared__ int s_need_x[SL];
-  const long long first = blk * SL;
-  const int f = (threadIdx.x / G) & 1, sl = threadIdx.x / (2 * G), line = threadIdx.x % G;
-  const bool valid = first + sl < nslabs;
-  RfSlab a = {};
-  if (valid) a = res(t0 + first + sl);
-  if (valid && f == 0 && line == 0) {
-    s_force[sl] = a.force + a.local * GG;
-    s_rot[sl] = a.rot + 12 * (a.local / G);
-    s_need_x[sl] = a.need_x;
-  }
-  float2* rows = sm + (sl * 2 + f) * G * RS;
-  float2 v[G];
-  const bool busy = valid && (f == 0 || a.need_x);
-  // inverse y of column kx = `line` of field f (f = 1: the x spectrum)
-  if (busy) {
-    const float2* src = (f == 0 ? a.S : a.S2) + a.local * GG;
-#pragma unroll
-    for (int ky = 0; ky < G; ++ky) v[ky] = src[ky * G + line];
-    reg_fft<G, true>(v);
-#pragma unroll
-    for (int y = 0; y < G; ++y) rows[y * RS + line] = v[y];
-  }
-  __syncthreads();
-  // inverse x of row y = `line`
-  if (busy) {
-    float2* row = rows + line * RS;
-#pragma unroll
-    for (int x = 0; x < G; ++x) v[x] = row[x];
-    reg_fft<G, true>(v);
-#pragma unroll
-    for (int x = 0; x < G; ++x) row[x] = v[x];
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < SL * GG; i += blockDim.x) {
-    const int s2 = i / GG, y = (i / G) % G, x = i % G;
-    if (first + s2 >= nslabs) break;
-    const float2 tz = sm[((s2 * 2) * G + y) * RS + x];
-    const float fy = tz.x, fz = tz.y;
-    const float fx = s_need_x[s2] ? sm[((s2 * 2 + 1) * G + y) * RS + x].x : fz;
-    const float mag = sqrtf(fmaf(fx, fx, fmaf(fy, fy, fz * fz)));
-    float4 out;
-    if (world) {
-      const float* Rr = s_rot[s2];
-      out = make_float4(fmaf(Rr[0], fx, fmaf(Rr[1], fy, Rr[2] * fz)),
-                        fmaf(Rr[4], fx, fmaf(Rr[5], fy, Rr[6] * fz)),
-                        fmaf(Rr[8], fx, fmaf(Rr[9], fy, Rr[10] * fz)), mag);
-    } else {
-      out = make_float4(fx, fy, fz, mag);
-    }
-    s_force[s2][y * G + x] = out;
-  }
-}
-
-template <int G, typename R>
-__global__ __launch_bounds__(256) void rf_fwd_xy_kernel(R res, long long t0, long long nslabs) {
-  rf_fwd_xy<G>(res, t0, nslabs, blockIdx.x);
-}
-template <int G, typename R>
-__global__ __launch_bounds__(256) void rf_z_green_kernel(R res, long long t0, long long nlines) {
-  rf_z_green<G>(res, t0, nlines, (long long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-template <int G, typename R>
-__global__ __launch_bounds__(256) void rf_inv_xy_kernel(R res, long long t0, long long nslabs,
-                                                        int world) {
-  rf_inv_xy<G>(res, t0, nslabs, world, blockIdx.x);
 }
 
 // The whole solve of one g = 16 grid in one workgroup (512 threads, the two
@@ -814,6 +669,358 @@ __global__ __launch_bounds__(512) void rf_grid_kernel(R res, long long g0, int w
   }
 }
 
+// ---------------------------------------------------------------------------
+// g = 32: the whole solve of one grid in one workgroup, on the half spectrum.
+// The density is real, so its spectrum is fixed by kx = 0 .. 16 (17 x 32 x 32
+// complex = 139 KB, which fits the 160 KB of LDS where the full 256 KB
+// spectrum does not), and each force component is real, so each comes from
+// its own half spectrum.  Per grid:
+//   0  density -> float, rows [z][y][x] in LDS (row stride 34 floats)
+//   1  real x transforms, two rows per complex FFT:  c = DFT(a + i b),
+//      A(k) = (C(k) + conj C(-k)) / 2,  B(k) = (C(k) - conj C(-k)) / 2i,
+//      kx = 0 .. 16 written over the rows (17 complex = 34 floats)
+//   2  y transforms of the columns kx = 1 .. 15, and of the real columns
+//      kx = 0 and 16 packed into one complex FFT and split as in 1
+//   3  per z line (ky, kx <= 16): forward z, Green's function (or the
+//      convolution multipliers), and for each component d the spectrum
+//      i k_d phi (Nyquist entry of k_d zero, as in the other paths),
+//      inverse z -> this workgroup's slot of global scratch
+//   4  in rounds of 12 z slabs (the slot's slabs copied back into LDS):
+//      inverse y (the kx = 0 and 16 columns, Hermitian in ky, packed into
+//      one complex FFT whose real and imaginary parts are their two real
+//      inverses), inverse x of two rows per complex FFT from their half
+//      spectra (Z(k) = X_a(k) + i X_b(k), Z(32 - k) = conj X_a(k) + i
+//      conj X_b(k), IDFT Z = x_a + i x_b), then |F| and the packed float4
+//      store (world frame: M F).
+// The transforms are the unnormalised DFTs of the other paths (the constant
+// factors are in gtab), so the forces agree with them to rounding.  Scratch
+// per workgroup: three components x 32 slabs x 544 complex (kRhSlot).
+// Workgroups loop over grids (grid-stride), one per CU at this LDS size.
+// ---------------------------------------------------------------------------
+constexpr int kRhThreads = 576;      // 544 z lines in phase 3; 512 tasks elsewhere
+constexpr int kRhRow = 34;           // floats per row: 32 reals or 17 complex
+// floats per slab (per slab and component in phase 4): 32 rows, padded by 8
+// floats so that the slabs or components one wave spans fall in different
+// LDS banks (a multiple of 4 floats: the copies of phase 4 store float4)
+constexpr int kRhArea = 32 * kRhRow + 8;
+constexpr int kRhLines = 32 * 17;    // complex per half-spectrum slab
+constexpr long long kRhSlot = 3LL * 32 * kRhLines;  // float2 of scratch per workgroup
+constexpr int kRhRound = 12;         // slabs per round of phase 4
+constexpr size_t kRhLds = (size_t)kRhRound * 3 * kRhArea * sizeof(float);  // 157,824 B
+
+// The thread index, opaque to the compiler.  rh_grid_kernel takes it afresh
+// at the start of each phase, so that the addresses a phase derives from it
+// are formed there rather than hoisted out of the grid loop and held in
+// registers through every other phase.
+__device__ __forceinline__ int phase_index() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+// The spectra A, B of two real sequences a, b from C = DFT(a + i b) at k and -k.
+__device__ __forceinline__ void split_real_pair(float2 c, float2 cm, float2& A, float2& B) {
+  A = make_float2(0.5f * (c.x + cm.x), 0.5f * (c.y - cm.y));
+  B = make_float2(0.5f * (c.y + cm.y), 0.5f * (cm.x - c.x));
+}
+
+template <typename R>
+__global__ __launch_bounds__(kRhThreads) void rh_grid_kernel(R res, long long g0,
+                                                             long long ngrids, int world,
+                                                             float2* __restrict__ scratch) {
+  constexpr int G = 32, H = 17, GG = G * G;
+  extern __shared__ float L[];
+  float2* const L2 = reinterpret_cast<float2*>(L);
+  float2* const slot = scratch + (long long)blockIdx.x * kRhSlot;
+  int t;
+  for (long long gi = blockIdx.x; gi < ngrids; gi += gridDim.x) {
+    const RfSlab a = res((g0 + gi) * G);
+    const long long cell0 = a.local * GG;  // local = grid * G
+    const long long grid = a.local / G;
+    const bool need_x = a.need_x;
+    t = phase_index();
+    // 0: density -> float rows, in batches of loads that are all in flight
+    // before the first conversion waits
+    {
+      constexpr int NP = G * GG / 2, NL = (NP + kRhThreads - 1) / kRhThreads, NB = 8;
+      const longlong2* m2 = reinterpret_cast<const longlong2*>(a.mass + cell0);
+#pragma unroll
+      for (int r0 = 0; r0 < NL; r0 += NB) {
+        longlong2 m[NB];
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {  // clamped, unconditional loads
+          const int i = t + (r0 + r) * kRhThreads;
+          if (r0 + r < NL) m[r] = m2[min(i, NP - 1)];
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+          const int i = t + (r0 + r) * kRhThreads;
+          if (r0 + r < NL && i < NP) {
+            const int c = 2 * i, x = c & (G - 1), y = (c >> 5) & (G - 1), z = c >> 10;
+            *reinterpret_cast<float2*>(L + z * kRhArea + y * kRhRow + x) =
+                make_float2((float)m[r].x, (float)m[r].y);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    __syncthreads();
+    float2 v[G];
+    t = phase_index();
+    // 1: x transforms, rows p and p + 16 of slab z per thread
+    if (t < 512) {
+      const int z = t >> 4, p = t & 15;
+      float* ra = L + z * kRhArea + p * kRhRow;
+      float* rb = ra + 16 * kRhRow;
+#pragma unroll
+      for (int x = 0; x < G; ++x) v[x] = make_float2(ra[x], rb[x]);
+      reg_fft<G, false>(v);
+      float2* ha = reinterpret_cast<float2*>(ra);
+      float2* hb = reinterpret_cast<float2*>(rb);
+#pragma unroll
+      for (int k = 0; k <= G / 2; ++k) {
+        float2 A, B;
+        split_real_pair(v[k], v[(G - k) & (G - 1)], A, B);
+        ha[k] = A;
+        hb[k] = B;
+      }
+    }
+    __syncthreads();
+    t = phase_index();
+    // 2: y transforms: threads 0 .. 479 take column q = 1 .. 15 of slab z,
+    // threads 480 .. 511 the real columns 0 and 16 of slab z packed into one
+    // complex FFT (the two kinds in different waves but one, so that a wave
+    // runs one of the two bodies)
+    if (t < 480) {
+      const int z = t / 15, q = 1 + t - 15 * z;
+      float2* col = L2 + z * (kRhArea / 2) + q;
+#pragma unroll
+      for (int y = 0; y < G; ++y) v[y] = col[y * H];
+      reg_fft<G, false>(v);
+#pragma unroll
+      for (int k = 0; k < G; ++k) col[k * H] = v[k];
+    } else if (t < 512) {
+      float2* col = L2 + (t - 480) * (kRhArea / 2);
+#pragma unroll
+      for (int y = 0; y < G; ++y) v[y] = make_float2(col[y * H].x, col[y * H + G / 2].x);
+      reg_fft<G, false>(v);
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        float2 A, B;
+        split_real_pair(v[k], v[(G - k) & (G - 1)], A, B);
+        col[k * H] = A;
+        col[k * H + G / 2] = B;
+      }
+    }
+    __syncthreads();
+    t = phase_index();
+    // 3a: forward z transforms of the 544 lines, times Green's function (the
+    // convolution solver multiplies per component in 3b), back into LDS.
+    // Table and scratch addresses are a wave-uniform base per kz (or z) plus
+    // the lane's offset, so that the unrolled loads need no address registers
+    // of their own.
+    const float2* tb = a.conv ? a.conv + 2 * G * GG : nullptr;  // half-spectrum multipliers
+    if (t < kRhLines) {
+      const int ky = t / H, kx = t - ky * H;
+      int lo = ky * G + kx;
+      // opaque to the compiler: the 32 table addresses are formed here, not
+      // hoisted out of the grid loop (where they would take 64 registers)
+      asm volatile("" : "+v"(lo));
+#pragma unroll
+      for (int z = 0; z < G; ++z) v[z] = L2[z * (kRhArea / 2) + t];
+      reg_fft<G, false>(v);
+      if (!tb) {
+#pragma unroll
+        for (int kz = 0; kz < G; ++kz) {
+          const float gf = (a.gtab + kz * GG)[lo];
+          v[kz] = make_float2(v[kz].x * gf, v[kz].y * gf);
+        }
+      }
+#pragma unroll
+      for (int kz = 0; kz < G; ++kz) L2[kz * (kRhArea / 2) + t] = v[kz];
+    }
+    __syncthreads();
+    t = phase_index();
+    // 3b: per component d and line: i k_d times the spectrum, inverse z -> scratch
+    if (t < kRhLines) {
+      const int ky = t / H, kx = t - ky * H;
+      for (int d = need_x ? 0 : 1; d < 3; ++d) {
+        if (tb) {
+#pragma unroll
+          for (int kz = 0; kz < G; ++kz) {
+            const float2 u = L2[kz * (kRhArea / 2) + t];
+            const float2 m = (tb + (d * G + kz) * kRhLines)[t];
+            v[kz] = make_float2(fmaf(u.x, m.x, -(u.y * m.y)), fmaf(u.x, m.y, u.y * m.x));
+          }
+        } else {
+          const float ks = a.kd[d == 0 ? kx : ky];
+#pragma unroll
+          for (int kz = 0; kz < G; ++kz) {
+            const float2 u = L2[kz * (kRhArea / 2) + t];
+            const float kv = d == 2 ? a.kd[kz] : ks;
+            v[kz] = make_float2(-(u.y * kv), u.x * kv);
+          }
+        }
+        reg_fft<G, true>(v);
+#pragma unroll
+        for (int z = 0; z < G; ++z) (slot + (d * G + z) * kRhLines)[t] = v[z];
+      }
+    }
+    __syncthreads();
+    t = phase_index();
+    // 4: inverse y and x, pack
+    float4* const out = a.force + cell0;
+    const float* Rr = a.rot + 12 * grid;
+    for (int z0 = 0; z0 < G; z0 += kRhRound) {
+      const int S = min(kRhRound, G - z0);
+      constexpr int A4 = kRhLines / 2;  // float4 per slab of scratch
+      constexpr int NL = (3 * kRhRound * A4 + kRhThreads - 1) / kRhThreads, NB = 9;
+#pragma unroll
+      for (int r0 = 0; r0 < NL; r0 += NB) {
+        float4 c[NB];  // a batch of loads in flight before the first LDS store
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {  // clamped, unconditional loads
+          const int i = min(t + (r0 + r) * kRhThreads, 3 * S * A4 - 1);
+          const int d = i / (S * A4), q4 = i - d * (S * A4);
+          if (r0 + r < NL) c[r] = reinterpret_cast<const float4*>(slot + (d * G + z0) * kRhLines)[q4];
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+          const int i = t + (r0 + r) * kRhThreads;
+          if (r0 + r < NL && i < 3 * S * A4) {
+            const int d = i / (S * A4), q4 = i - d * (S * A4), s = q4 / A4, j = q4 - s * A4;
+            reinterpret_cast<float4*>(L + (3 * s + d) * kRhArea)[j] = c[r];
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      __syncthreads();
+      // inverse y: threads 0 .. 45 S - 1 take column q = 1 .. 15 of (slab s,
+      // component d), the next 3 S threads the Hermitian columns 0 and 16 of
+      // (s, d) packed into one complex FFT
+      if (t < 45 * S) {
+        const int sd = t / 15, q = 1 + t - 15 * sd;
+        if (sd % 3 > 0 || need_x) {
+          float2* col = L2 + sd * (kRhArea / 2) + q;
+#pragma unroll
+          for (int k = 0; k < G; ++k) v[k] = col[k * H];
+          reg_fft<G, true>(v);
+#pragma unroll
+          for (int y = 0; y < G; ++y) col[y * H] = v[y];
+        }
+      } else if (t < 48 * S) {
+        const int sd = t - 45 * S;
+        if (sd % 3 > 0 || need_x) {
+          float2* col = L2 + sd * (kRhArea / 2);
+#pragma unroll
+          for (int k = 0; k < G; ++k) {
+            const float2 y0 = col[k * H], y16 = col[k * H + G / 2];
+            v[k] = make_float2(y0.x - y16.y, y0.y + y16.x);
+          }
+          reg_fft<G, true>(v);
+#pragma unroll
+          for (int y = 0; y < G; ++y) {
+            col[y * H] = make_float2(v[y].x, 0.f);
+            col[y * H + G / 2] = make_float2(v[y].y, 0.f);
+          }
+        }
+      }
+      __syncthreads();
+      const int s = t / 48, rem = t - 48 * s, d = rem >> 4, q = rem & 15;
+      const bool act = s < S && (d > 0 || need_x);
+      float2* const base = L2 + (3 * s + d) * (kRhArea / 2);
+      if (act) {  // inverse x of rows q and q + 16
+        float2* ra = base + q * H;
+        float2* rb = base + (q + 16) * H;
+#pragma unroll
+        for (int k = 0; k <= G / 2; ++k) {
+          const float2 A = ra[k], B = rb[k];
+          v[k] = make_float2(A.x - B.y, A.y + B.x);
+          if (k > 0 && k < G / 2) v[G - k] = make_float2(A.x + B.y, B.x - A.y);
+        }
+        reg_fft<G, true>(v);
+        float* fa = reinterpret_cast<float*>(ra);
+        float* fb = reinterpret_cast<float*>(rb);
+#pragma unroll
+        for (int x = 0; x < G; ++x) {
+          fa[x] = v[x].x;
+          fb[x] = v[x].y;
+        }
+      }
+      __syncthreads();
+      // pack, PB cells per thread at a time (their LDS reads in flight together)
+      constexpr int NPK = (kRhRound * GG + kRhThreads - 1) / kRhThreads, PB = 4;
+#pragma unroll
+      for (int r0 = 0; r0 < NPK; r0 += PB) {
+        float fx[PB], fy[PB], fz[PB];
+#pragma unroll
+        for (int r = 0; r < PB; ++r) {
+          const int i = min(t + (r0 + r) * kRhThreads, S * GG - 1);
+          const float* Fa = L + 3 * (i >> 10) * kRhArea + ((i >> 5) & (G - 1)) * kRhRow + (i & (G - 1));
+          fy[r] = Fa[kRhArea];
+          fz[r] = Fa[2 * kRhArea];
+          fx[r] = need_x ? Fa[0] : fz[r];
+        }
+#pragma unroll
+        for (int r = 0; r < PB; ++r) {
+          const int i = t + (r0 + r) * kRhThreads;
+          if (r0 + r < NPK && i < S * GG) {
+            const float mag = sqrtf(fmaf(fx[r], fx[r], fmaf(fy[r], fy[r], fz[r] * fz[r])));
+            float4 o;
+            if (world) {
+              o = make_float4(fmaf(Rr[0], fx[r], fmaf(Rr[1], fy[r], Rr[2] * fz[r])),
+                              fmaf(Rr[4], fx[r], fmaf(Rr[5], fy[r], Rr[6] * fz[r])),
+                              fmaf(Rr[8], fx[r], fmaf(Rr[9], fy[r], Rr[10] * fz[r])), mag);
+            } else {
+              o = make_float4(fx[r], fy[r], fz[r], mag);
+            }
+            out[z0 * GG + i] = o;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Workgroups of rh_grid_kernel for `ngrids` grids and `scratch_elems` float2
+// of scratch: one per grid up to the CU count (one workgroup fits a CU) and
+// the slots the scratch holds.
+static long long rh_slots(long long ngrids, long long scratch_elems) {
+  static int cus[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (cus[dev] <= 0) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
+      c = 256;
+    cus[dev] = c;
+  }
+  return std::min(std::min(ngrids, (long long)cus[dev]), scratch_elems / kRhSlot);
+}
+
+long long rh_scratch_elems(long long ngrids) {
+  int dev = 0, c = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
+    c = 256;
+  return std::min(ngrids, (long long)c) * kRhSlot;
+}
+
+template <typename R>
+static hipError_t launch_rh_grids(const R& res, long long g0, long long ngrids, int world,
+                                  float2* scratch, long long scratch_elems, hipStream_t s) {
+  if (ngrids <= 0) return hipSuccess;
+  const long long wgs = rh_slots(ngrids, scratch_elems);
+  if (wgs <= 0) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute((const void*)rh_grid_kernel<R>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRhLds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((rh_grid_kernel<R>), dim3((unsigned)wgs), dim3(kRhThreads), kRhLds, s, res,
+                     g0, ngrids, world, scratch);
+  return hipGetLastError();
+}
+
 // One rf_grid_kernel workgroup per grid, grids [g0, g0 + ngrids) of `res`.
 template <int G, typename R>
 static hipError_t launch_rf_grids(const R& res, long long g0, long long ngrids, int world,
@@ -828,40 +1035,14 @@ static hipError_t launch_rf_grids(const R& res, long long g0, long long ngrids, 
   return hipGetLastError();
 }
 
-// The three register-FFT passes over slabs [t0, t0 + nslabs) of `res`.
-template <int G, typename R>
-static hipError_t launch_rf_passes(const R& res, long long t0, long long nslabs, int world,
-                                   hipStream_t s) {
-  if (nslabs <= 0) return hipSuccess;
-  const long long lines = nslabs * G;
-  const size_t ldsA = (size_t)(256 / G) * G * (G + 1) * sizeof(float2);
-  const size_t ldsC = (size_t)(128 / G) * 2 * G * (G + 1) * sizeof(float2);
-  hipError_t e;
-  if ((e = hipFuncSetAttribute((const void*)rf_fwd_xy_kernel<G, R>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsA)) != hipSuccess)
-    return e;
-  if ((e = hipFuncSetAttribute((const void*)rf_inv_xy_kernel<G, R>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsC)) != hipSuccess)
-    return e;
-  hipLaunchKernelGGL((rf_fwd_xy_kernel<G, R>), dim3((unsigned)((nslabs + 256 / G - 1) / (256 / G))),
-                     dim3(256), ldsA, s, res, t0, nslabs);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL((rf_z_green_kernel<G, R>), dim3((unsigned)((lines + 255) / 256)), dim3(256), 0,
-                     s, res, t0, lines);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL((rf_inv_xy_kernel<G, R>), dim3((unsigned)((nslabs + 128 / G - 1) / (128 / G))),
-                     dim3(256), ldsC, s, res, t0, nslabs, world);
-  return hipGetLastError();
-}
-
 template <int G>
-static hipError_t launch_rf_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
-                                  float4* force, const float* gtab, const float* kd,
-                                  const float2* conv, int need_x, const float* rot, int world,
-                                  hipStream_t s) {
+static hipError_t launch_rf_solve(const Params& P, const long long* grids, float2* S1,
+                                  long long s1_elems, float2* S2, float4* force, const float* gtab,
+                                  const float* kd, const float2* conv, int need_x, const float* rot,
+                                  int world, hipStream_t s) {
   const RfSingle res = {grids, S1, S2, force, gtab, kd, conv, rot, need_x};
   if (G == 16) return launch_rf_grids<16>(res, 0, P.k, world, s);
-  return launch_rf_passes<G>(res, 0, (long long)P.k * G, world, s);
+  return launch_rh_grids(res, 0, P.k, world, S1, s1_elems, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -894,8 +1075,9 @@ __device__ __forceinline__ int class_slot(const long long* __restrict__ cls_off,
 __device__ RfSlab RfBatch::operator()(long long t) const {
   const int j = class_slot(cls_off, ncls, t);
   const BatchSim& b = sims[cls_sim[j]];
-  return {mass + b.cell_off,          S + soff[j],
-          S2 + soff[j],               force + b.cell_off,
+  const long long so = soff ? soff[j] : 0;  // (no scratch offsets: kernels without scratch)
+  return {mass + b.cell_off,          S + so,
+          S2 + so,                    force + b.cell_off,
           gtab + b.tab_off,           kd + b.line_off,
           b.conv_off >= 0 ? conv + b.conv_off : nullptr,
           rot + 12 * b.grid_off,      t - cls_off[j],
@@ -967,17 +1149,18 @@ static size_t solve_lds(int g, int pass) {
 
 hipError_t launch_batch_solve(const BatchSim* sims, const int* cls_sim, const long long* cls_off,
                               int ncls, long long slab0, long long slabs, const long long* soff,
-                              int gmax, const long long* grids, float2* S1, float2* S2,
-                              float4* force, const float2* tw, const float* gtab, const float* kd,
-                              const float2* conv, const float* rot, hipStream_t s) {
+                              int gmax, const long long* grids, float2* S1, long long s1_elems,
+                              float2* S2, float4* force, const float2* tw, const float* gtab,
+                              const float* kd, const float2* conv, const float* rot,
+                              hipStream_t s) {
   if (slabs <= 0 || ncls <= 0) return hipSuccess;
-  // a class of one grid size 16 or 32 runs the register-FFT passes, the
-  // same arithmetic as the single-simulation solve
+  // a class of grid size 16 or 32 runs the single-simulation kernels (one
+  // workgroup per grid, same arithmetic) over all of its grids at once
   if (gmax == 16 || gmax == 32) {
     const RfBatch res = {sims, cls_sim, cls_off, ncls, soff, grids, S1, S2, force, gtab, kd, conv, rot};
-    // (windows hold whole grids: slab0 and slabs are multiples of g)
+    // (slab0 and slabs are multiples of g: whole grids)
     return gmax == 16 ? launch_rf_grids<16>(res, slab0 / 16, slabs / 16, 1, s)
-                      : launch_rf_passes<32>(res, slab0, slabs, 1, s);
+                      : launch_rh_grids(res, slab0 / 32, slabs / 32, 1, S1, s1_elems, s);
   }
   const size_t ldsA = solve_lds(gmax, 0), ldsB = solve_lds(gmax, 1), ldsC = solve_lds(gmax, 2);
   hipError_t e;
@@ -1002,15 +1185,17 @@ hipError_t launch_batch_solve(const BatchSim* sims, const int* cls_sim, const lo
   return hipGetLastError();
 }
 
-hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, float2* S2,
-                        float4* force, const float2* tw, const float* gtab, const float* kd,
-                        const float2* conv, int need_x, const float* rot, int world,
-                        hipStream_t s) {
+hipError_t launch_solve(const Params& P, const long long* grids, float2* S1, long long s1_elems,
+                        float2* S2, float4* force, const float2* tw, const float* gtab,
+                        const float* kd, const float2* conv, int need_x, const float* rot,
+                        int world, hipStream_t s) {
   const int g = P.g, lg = P.lg;
   if (g == 16)
-    return launch_rf_solve<16>(P, grids, S1, S2, force, gtab, kd, conv, need_x, rot, world, s);
+    return launch_rf_solve<16>(P, grids, S1, s1_elems, S2, force, gtab, kd, conv, need_x, rot,
+                               world, s);
   if (g == 32)
-    return launch_rf_solve<32>(P, grids, S1, S2, force, gtab, kd, conv, need_x, rot, world, s);
+    return launch_rf_solve<32>(P, grids, S1, s1_elems, S2, force, gtab, kd, conv, need_x, rot,
+                               world, s);
   const unsigned slabs = (unsigned)(P.k * g);
   const size_t ldsA = solve_lds(g, 0), ldsB = solve_lds(g, 1), ldsC = solve_lds(g, 2);
   hipError_t e;
