@@ -652,6 +652,7 @@ struct mog_batch {
   std::vector<char> have_transforms;
   long long ntotal, cells_total, gather_blocks;
   int grids_total;
+  int kmax;  // largest n_grids of the simulations
   // solve classes, one per grid size g = 2 .. 32 (class log2 g - 1):
   // simulations, slabs, LDS
   struct SolveClass {
@@ -753,14 +754,14 @@ hipError_t batch_build_tasks(mog_batch* b) {
     const BatchSim& sm = b->sims[si];
     if (sm.n == 0) continue;
     const int cells = sm.P.cells;
-    const int per = std::max(1, std::min(sm.P.k, kBatchLdsCells / cells));
+    const int per = batch_task_grids(cells);
     for (long long p0 = 0; p0 < sm.n; p0 += b->deposit_chunk)
       for (int g0 = 0; g0 < sm.P.k; g0 += per) {
         DepositTask t;
         t.sim = si;
         t.grid0 = g0;
         t.ng = std::min(per, sm.P.k - g0);
-        t.pad = 0;
+        t.G = per;
         t.p0 = p0;
         t.p1 = std::min(sm.n, p0 + b->deposit_chunk);
         tasks.push_back(t);
@@ -827,7 +828,7 @@ hipError_t batch_enqueue(mog_batch* b, float* pos, float* vel, hipStream_t s) {
   }
   if (e == hipSuccess)
     e = launch_batch_gather((float4*)pos, (float4*)vel, b->gather_blocks, b->d_sims, b->nsims,
-                            b->d_rot, b->d_off, b->d_frot, b->d_force, batch_order(b), s);
+                            b->kmax, b->d_rot, b->d_off, b->d_frot, b->d_force, batch_order(b), s);
   return e;
 }
 
@@ -1582,6 +1583,8 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
     all_off.insert(all_off.end(), cls_off[ci].begin(), cls_off[ci].end());
   }
   b->grids_total = grid_off;
+  b->kmax = 0;
+  for (int i = 0; i < n_sims; ++i) b->kmax = std::max(b->kmax, cfgs[i].n_grids);
   // solve windows of the classes g <= 8 (the slab passes): whole grids of
   // one class, at most kBatchWindowCells cells each, laid out in the
   // window's scratch in slab order.  The g = 16 and g = 32 classes run one

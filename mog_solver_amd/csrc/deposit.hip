@@ -41,10 +41,14 @@ constexpr int kLaneSlots = 4;
 // UP particles per lane and register set in the LDS deposit.
 constexpr int kDepositUP = 2;
 
-// LDS-privatised deposit; grid (chunks, ceil(k / G)).  Dynamic LDS holds
-// G * cells uint32 counters followed by kLaneSlots records per lane.  The G
-// fast rows are wave-uniform and live in scalar registers for the whole
-// chunk.
+// One tile of the LDS-privatised deposit: particles [p0, p1) of one
+// simulation (rows of pos / cpos) into its grids [grid0, grid0 + ng), ng <=
+// G, through G uint32 histograms at `hist` (zero on entry; at LDS address 0
+// under SH) and kLaneSlots records per lane at `slots`.  Called by every
+// thread of the workgroup (it holds barriers).  The G fast rows are
+// wave-uniform and live in scalar (for G <= 4 under SH: vector) registers
+// for the whole tile.  Used by deposit_lds_kernel (one tile per workgroup)
+// and by the batched ensemble's persistent deposit (one tile per task).
 //
 // Every (particle, grid) pair adds the particle's 32-bit milli-mass at its
 // fast cell (cellmath.h), unconditionally: no per-pair compare or select.
@@ -61,12 +65,11 @@ constexpr int kDepositUP = 2;
 // while a chunk's sum stays below 2^32: a particle whose milli-mass exceeds
 // (2^32-1)/chunk takes the 64-bit path.
 //
-// Two loops feed the same per-particle work.  Workgroups whose chunk is a
-// whole number of (2 x UP x threads)-particle spans and whose group holds G
-// live grids (all but the last chunk of a launch sized by launch_deposit)
-// run a double-buffered loop: two register sets of UP particles each, one
+// Two loops feed the same per-particle work.  In a tile whose group holds G
+// live grids, the whole (2 x UP x threads)-particle spans of its range run a
+// double-buffered loop: two register sets of UP particles each, one
 // processed while the other loads, with unguarded loads at 32-bit byte
-// offsets from the chunk's base.  The rest run the guarded loop.
+// offsets from the range's base.  The rest run the guarded loop.
 //
 // SH: the fast cells come as LDS byte offsets through the shifted fields of
 // cellmath.h (cell_fast_o), for launches whose transforms satisfy
@@ -79,22 +82,14 @@ constexpr int kDepositUP = 2;
 // pair is one compare; the repair adds the marker back and takes such
 // particles from `pos`.
 template <int G, bool SH, bool PREP>
-__global__ __launch_bounds__(1024) void deposit_lds_kernel(
-    const float4* __restrict__ pos, const float4* __restrict__ cpos, long long n, long long chunk,
-    const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
-    Params P, unsigned long long lds_limit, unsigned long long* __restrict__ grids) {
-  extern __shared__ unsigned int hist[];
+__device__ __forceinline__ void deposit_tile(
+    unsigned int* __restrict__ hist, unsigned int* __restrict__ slots,
+    const float4* __restrict__ pos, const float4* __restrict__ cpos, long long p0, long long p1,
+    int grid0, int ng, const float* __restrict__ rot,
+    const float* __restrict__ off, const float* __restrict__ frot, const Params& P,
+    unsigned long long lds_limit, unsigned long long* __restrict__ grids, bool rezero) {
   const int cells = P.cells;
-  unsigned int* slots = hist + G * cells;  // slot r of lane t: slots[r * blockDim.x + t]
-  // tiles in chunk-major order: the groups of one chunk share an XCD's L2
-  const int groups = (P.k + G - 1) / G;
-  const unsigned int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const long long chunk_id = tile / groups;
-  const int grid0 = (int)(tile % groups) * G;
-  const int ng = min(G, P.k - grid0);
   const int total = ng * cells;
-  for (int i = threadIdx.x; i < total; i += blockDim.x) hist[i] = 0u;
-
   float F[G][12];
 #pragma unroll
   for (int gi = 0; gi < G; ++gi) {
@@ -111,8 +106,6 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
   }
   __syncthreads();
 
-  const long long p0 = chunk_id * chunk;
-  const long long p1 = min(n, p0 + chunk);
   const unsigned int lds_limit32 =
       (unsigned int)(lds_limit < 0xFFFFFFFFull ? lds_limit : 0xFFFFFFFFull);
   constexpr int UP = kDepositUP;
@@ -318,10 +311,12 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
     }
   };
 
+  // whole spans of the range (when the group holds G live grids) take the
+  // double-buffered loop, the rest the guarded one
   const long long span = 2ll * UP * blockDim.x;
-  const bool fast = ng == G && p1 - p0 == chunk && chunk % span == 0 && chunk < (1ll << 27);
+  const long long spans = (ng == G && p1 - p0 < (1ll << 27)) ? (p1 - p0) / span : 0;
   const float4* const src = PREP ? cpos : pos;  // the rows the loops read
-  if (fast) {
+  if (spans > 0) {
     const char* base = reinterpret_cast<const char*>(src + p0);
     const unsigned int lane_step = 16u * blockDim.x;  // bytes between a lane's particles
     auto load = [&](float4* q, unsigned int boff) {
@@ -329,7 +324,7 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
       for (int u = 0; u < UP; ++u)
         q[u] = *reinterpret_cast<const float4*>(base + boff + (unsigned int)u * lane_step);
     };
-    const int iters = (int)(chunk / span);
+    const int iters = (int)spans;
     unsigned int boff = 16u * threadIdx.x;
     float4 a[UP], b[UP];
     load(a, boff);
@@ -341,17 +336,19 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
       if (it + 1 < iters) load(a, boff);
       pairs(b, boff_b >> 4);
     }
-  } else {
+  }
+  if (p0 + spans * span < p1) {
     const long long step = (long long)UP * blockDim.x;
+    const long long pfirst = p0 + spans * span + threadIdx.x;
     // the next iteration's particles are loaded while this one computes;
-    // rows past the chunk read as mass 0 (they add 0 and are never repaired)
+    // rows past the range read as mass 0 (they add 0 and are never repaired)
     float4 qn[UP];
 #pragma unroll
     for (int u = 0; u < UP; ++u) {
-      const long long p = p0 + threadIdx.x + (long long)u * blockDim.x;
+      const long long p = pfirst + (long long)u * blockDim.x;
       qn[u] = p < p1 ? src[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    for (long long pb = p0 + threadIdx.x; pb < p1; pb += step) {
+    for (long long pb = pfirst; pb < p1; pb += step) {
       float4 q[UP];
 #pragma unroll
       for (int u = 0; u < UP; ++u) {
@@ -379,11 +376,115 @@ __global__ __launch_bounds__(1024) void deposit_lds_kernel(
     }
   }
   __syncthreads();
+  // flush the live counters; with `rezero` every counter the tile touched
+  // (its G histograms, live or not) is left at zero for the next tile
   unsigned long long* dst = grids + (long long)grid0 * cells;
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+  for (int i = threadIdx.x; i < (rezero ? G * cells : total); i += blockDim.x) {
     const unsigned int v = hist[i];
-    if (v) atomicAdd(&dst[i], (unsigned long long)v);
+    if (v) {
+      if (i < total) atomicAdd(&dst[i], (unsigned long long)v);
+      if (rezero) hist[i] = 0u;
+    }
   }
+}
+
+// LDS-privatised deposit; grid (chunks, ceil(k / G)) in XCD-aware,
+// chunk-major order.  Dynamic LDS holds G * cells uint32 counters followed
+// by kLaneSlots records per lane (and no static LDS: the shifted-field path
+// addresses counters by their LDS address from 0).
+template <int G, bool SH, bool PREP>
+__global__ __launch_bounds__(1024) void deposit_lds_kernel(
+    const float4* __restrict__ pos, const float4* __restrict__ cpos, long long n, long long chunk,
+    const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
+    Params P, unsigned long long lds_limit, unsigned long long* __restrict__ grids) {
+  extern __shared__ unsigned int hist[];
+  // tiles in chunk-major order: the groups of one chunk share an XCD's L2
+  const int groups = (P.k + G - 1) / G;
+  const unsigned int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const long long chunk_id = tile / groups;
+  const int grid0 = (int)(tile % groups) * G;
+  const int ng = min(G, P.k - grid0);
+  for (int i = threadIdx.x; i < ng * P.cells; i += blockDim.x) hist[i] = 0u;
+  const long long p0 = chunk_id * chunk;
+  deposit_tile<G, SH, PREP>(hist, hist + G * P.cells, pos, cpos, p0, min(n, p0 + chunk), grid0,
+                            ng, rot, off, frot, P, lds_limit, grids, false);
+}
+
+// Grids per task of the batched deposit: the largest power of two up to
+// kBatchMaxG whose histograms fit kBatchLdsCells (G <= 4 keeps the fast rows
+// in vector registers).
+int batch_task_grids(int cells) {
+  int G = 1;
+  while (G * 2 <= kBatchMaxG && (G * 2) * cells <= kBatchLdsCells) G *= 2;
+  return G;
+}
+
+// Persistent deposit of the batched ensemble over its task list (particle
+// chunk x group of grids of one simulation, ordered by cost, largest first:
+// dynamic pickup balances simulations whose n spans 50x, k 7.5x, g^3 8x).
+// A workgroup takes the next task from a device counter and runs it as one
+// deposit_tile of the single-simulation deposit, so the grids are the same
+// integers; the tile's flush leaves its counters at zero for the next task
+// (one clear per workgroup, not one per task).  Dynamic LDS: kBatchLdsCells
+// counters from LDS address 0, the lanes' records, then the task index.
+__global__ __launch_bounds__(1024) void batch_deposit_kernel(
+    const float4* __restrict__ pos, const BatchSim* __restrict__ sims,
+    const DepositTask* __restrict__ tasks, int ntasks, const float* __restrict__ rot,
+    const float* __restrict__ off, const float* __restrict__ frot,
+    unsigned long long* __restrict__ grids, unsigned long long* __restrict__ counter) {
+  extern __shared__ unsigned int hist[];
+  unsigned int* const slots = hist + kBatchLdsCells;
+  int* const cur = reinterpret_cast<int*>(slots + kLaneSlots * blockDim.x);
+  for (int i = threadIdx.x; i < kBatchLdsCells; i += blockDim.x) hist[i] = 0u;
+  for (;;) {
+    if (threadIdx.x == 0) *cur = (int)atomicAdd(counter, 1ull);
+    __syncthreads();
+    const int ti = *cur;
+    if (ti >= ntasks) break;  // uniform across the workgroup
+    const DepositTask task = tasks[ti];
+    const BatchSim& b = sims[task.sim];
+    const Params P = b.P;
+    const long long len = task.p1 - task.p0;
+    // 32-bit partials exact for the task's particles (at least 256 in the
+    // divisor: a saturated milli-mass is then never taken as small)
+    const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)(len > 256 ? len : 256);
+    const float4* sp = pos + b.p_off;
+    const float* R = rot + 12 * b.grid_off;
+    const float* O = off + 4 * b.grid_off;
+    const float* FR = frot + 12 * b.grid_off;
+    unsigned long long* gd = grids + b.cell_off;
+#define MOG_TILE(GG, SHB)                                                                     \
+  deposit_tile<GG, SHB, false>(hist, slots, sp, nullptr, task.p0, task.p1, task.grid0,       \
+                               task.ng, R, O, FR, P, lds_limit, gd, true)
+    if (P.shift_ok & kShiftLds) {
+      if (task.G == 4) MOG_TILE(4, true);
+      else if (task.G == 2) MOG_TILE(2, true);
+      else MOG_TILE(1, true);
+    } else {
+      if (task.G == 4) MOG_TILE(4, false);
+      else if (task.G == 2) MOG_TILE(2, false);
+      else MOG_TILE(1, false);
+    }
+#undef MOG_TILE
+    __syncthreads();  // LDS and the task index are reused by the next task
+  }
+}
+
+hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const DepositTask* tasks,
+                                int ntasks, const float* rot, const float* off, const float* frot,
+                                unsigned long long* grids, unsigned long long* counter,
+                                int threads, hipStream_t s) {
+  if (ntasks <= 0) return hipSuccess;
+  const size_t lds = (size_t)kBatchLdsCells * sizeof(unsigned int) +
+                     (size_t)kLaneSlots * threads * sizeof(unsigned int) + 16;
+  hipError_t e = hipFuncSetAttribute((const void*)batch_deposit_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  // one workgroup per CU holds the 128 KB histogram budget
+  const unsigned wg = (unsigned)(ntasks < 256 ? ntasks : 256);
+  hipLaunchKernelGGL(batch_deposit_kernel, dim3(wg), dim3(threads), lds, s, pos, sims, tasks,
+                     ntasks, rot, off, frot, grids, counter);
+  return hipGetLastError();
 }
 
 // Direct global deposit for grids too large to privatise in LDS (g = 64).

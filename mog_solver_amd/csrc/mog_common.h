@@ -271,12 +271,15 @@ struct BatchSim {
 // A deposit work item: particles [p0, p1) of `sim` into its grids
 // [grid0, grid0 + ng), privatised in LDS.
 struct DepositTask {
-  int sim, grid0, ng, pad;
+  int sim, grid0, ng, G;  // G: the deposit tile's histogram count (1, 2 or 4), ng <= G
   long long p0, p1;
 };
 
 // Largest LDS histogram of a batched deposit task (uint32 counters).
 constexpr int kBatchLdsCells = 32768;
+// Most grids per task of the batched deposit (fast rows in vector registers).
+constexpr int kBatchMaxG = 4;
+int batch_task_grids(int cells);
 // Particles per gather block; a block never straddles two simulations.
 constexpr int kBatchGatherBlock = 256;
 // Deferred undecided pairs per lane of a gather (grid indices in LDS): the
@@ -347,6 +350,7 @@ constexpr long long kBatchWindowCells = 1ll << 22;
 // grids (one workgroup per grid, no windows).
 inline bool batch_class_fused(int gmax) { return gmax == 16 || gmax == 32; }
 hipError_t launch_batch_gather(float4* pos, float4* vel, long long blocks, const BatchSim* sims,
-                               int nsims, const float* rot, const float* off, const float* frot,
-                               const float4* force, const int* order, hipStream_t s);
+                               int nsims, int kmax, const float* rot, const float* off,
+                               const float* frot, const float4* force, const int* order,
+                               hipStream_t s);
 }  // namespace mog
