@@ -872,23 +872,23 @@ __global__ __launch_bounds__(kRhThreads) void rh_grid_kernel(R res, long long g0
     float4* const out = a.force + cell0;
     const float* Rr = a.rot + 12 * grid;
     for (int z0 = 0; z0 < G; z0 += kRhRound) {
+      t = phase_index();  // (per round: addresses not hoisted out of the round loop)
       const int S = min(kRhRound, G - z0);
       constexpr int A4 = kRhLines / 2;  // float4 per slab of scratch
-      constexpr int NL = (3 * kRhRound * A4 + kRhThreads - 1) / kRhThreads, NB = 9;
+      constexpr int NL = (kRhRound * A4 + kRhThreads - 1) / kRhThreads;  // per component
 #pragma unroll
-      for (int r0 = 0; r0 < NL; r0 += NB) {
-        float4 c[NB];  // a batch of loads in flight before the first LDS store
+      for (int d = 0; d < 3; ++d) {
+        // component d of slabs z0 .. z0 + S - 1: a contiguous run of S * A4
+        // float4 in the slot, all loads in flight before the first LDS store
+        const float4* src = reinterpret_cast<const float4*>(slot + (d * G + z0) * kRhLines);
+        float4 c[NL];
 #pragma unroll
-        for (int r = 0; r < NB; ++r) {  // clamped, unconditional loads
-          const int i = min(t + (r0 + r) * kRhThreads, 3 * S * A4 - 1);
-          const int d = i / (S * A4), q4 = i - d * (S * A4);
-          if (r0 + r < NL) c[r] = reinterpret_cast<const float4*>(slot + (d * G + z0) * kRhLines)[q4];
-        }
+        for (int r = 0; r < NL; ++r) c[r] = src[min(t + r * kRhThreads, S * A4 - 1)];
 #pragma unroll
-        for (int r = 0; r < NB; ++r) {
-          const int i = t + (r0 + r) * kRhThreads;
-          if (r0 + r < NL && i < 3 * S * A4) {
-            const int d = i / (S * A4), q4 = i - d * (S * A4), s = q4 / A4, j = q4 - s * A4;
+        for (int r = 0; r < NL; ++r) {
+          const int i = t + r * kRhThreads;
+          if (i < S * A4) {
+            const int s = i / A4, j = i - s * A4;
             reinterpret_cast<float4*>(L + (3 * s + d) * kRhArea)[j] = c[r];
           }
         }
@@ -948,33 +948,39 @@ __global__ __launch_bounds__(kRhThreads) void rh_grid_kernel(R res, long long g0
         }
       }
       __syncthreads();
-      // pack, PB cells per thread at a time (their LDS reads in flight together)
-      constexpr int NPK = (kRhRound * GG + kRhThreads - 1) / kRhThreads, PB = 4;
+      // pack: each thread takes pairs of neighbouring cells (x, x + 1), PB
+      // pairs at a time (their LDS reads in flight together), and stores the
+      // pair's two float4 as 32 contiguous bytes
+      constexpr int NPK = (kRhRound * GG / 2 + kRhThreads - 1) / kRhThreads, PB = 2;
+      const float r0x = Rr[0], r1x = Rr[1], r2x = Rr[2], r4x = Rr[4], r5x = Rr[5], r6x = Rr[6],
+                  r8x = Rr[8], r9x = Rr[9], r10x = Rr[10];
+      auto pack1 = [&](float fx, float fy, float fz) {
+        const float mag = sqrtf(fmaf(fx, fx, fmaf(fy, fy, fz * fz)));
+        if (world)
+          return make_float4(fmaf(r0x, fx, fmaf(r1x, fy, r2x * fz)),
+                             fmaf(r4x, fx, fmaf(r5x, fy, r6x * fz)),
+                             fmaf(r8x, fx, fmaf(r9x, fy, r10x * fz)), mag);
+        return make_float4(fx, fy, fz, mag);
+      };
 #pragma unroll
       for (int r0 = 0; r0 < NPK; r0 += PB) {
-        float fx[PB], fy[PB], fz[PB];
+        float2 fx[PB], fy[PB], fz[PB];
 #pragma unroll
         for (int r = 0; r < PB; ++r) {
-          const int i = min(t + (r0 + r) * kRhThreads, S * GG - 1);
+          const int i = 2 * min(t + (r0 + r) * kRhThreads, S * GG / 2 - 1);
           const float* Fa = L + 3 * (i >> 10) * kRhArea + ((i >> 5) & (G - 1)) * kRhRow + (i & (G - 1));
-          fy[r] = Fa[kRhArea];
-          fz[r] = Fa[2 * kRhArea];
-          fx[r] = need_x ? Fa[0] : fz[r];
+          fy[r] = *reinterpret_cast<const float2*>(Fa + kRhArea);
+          fz[r] = *reinterpret_cast<const float2*>(Fa + 2 * kRhArea);
+          const float2 xv = *reinterpret_cast<const float2*>(Fa);  // (a value select, not a pointer one)
+          fx[r] = need_x ? xv : fz[r];
         }
 #pragma unroll
         for (int r = 0; r < PB; ++r) {
-          const int i = t + (r0 + r) * kRhThreads;
+          const int i = 2 * (t + (r0 + r) * kRhThreads);
           if (r0 + r < NPK && i < S * GG) {
-            const float mag = sqrtf(fmaf(fx[r], fx[r], fmaf(fy[r], fy[r], fz[r] * fz[r])));
-            float4 o;
-            if (world) {
-              o = make_float4(fmaf(Rr[0], fx[r], fmaf(Rr[1], fy[r], Rr[2] * fz[r])),
-                              fmaf(Rr[4], fx[r], fmaf(Rr[5], fy[r], Rr[6] * fz[r])),
-                              fmaf(Rr[8], fx[r], fmaf(Rr[9], fy[r], Rr[10] * fz[r])), mag);
-            } else {
-              o = make_float4(fx[r], fy[r], fz[r], mag);
-            }
-            out[z0 * GG + i] = o;
+            float4* o = out + z0 * GG + i;
+            o[0] = pack1(fx[r].x, fy[r].x, fz[r].x);
+            o[1] = pack1(fx[r].y, fy[r].y, fz[r].y);
           }
         }
       }
