@@ -37,26 +37,32 @@ FP32_VECTOR_PEAK_TFS = 157.3
 
 # HBM-side bytes per launch, from two rocprofv3 --pmc runs of this script
 # (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the dispatches)
-# with --copy-calibration 256 --no-autotune on the two-disks workload: the
-# 256 MiB copy reads as FETCH 128 MiB and WRITE 256 MiB, so FETCH_SIZE is
-# scaled by 2.0 and WRITE_SIZE by 1.0.  Figures and commands:
-# profiles/pmc_traffic_two_disks_n1e6_k240_g16.md.  Used only when the run
-# matches (workload, n, k, g, precision); the deposit's figure also needs
-# the same (grids per workgroup, workgroup target), since its re-reads of the
-# positions depend on its tiling.  The deposit phase is its prepass plus
-# the LDS kernel.
+# with --copy-calibration 256 --no-autotune on the workload: the 256 MiB
+# copy reads as FETCH 128 MiB and WRITE 256 MiB, so FETCH_SIZE is scaled by
+# 2.0 and WRITE_SIZE by 1.0.  Figures and commands:
+# profiles/pmc_traffic_two_disks_n1e6_k240_g16.md and
+# profiles/pmc_traffic_plummer_n4e6_k240_g32.md.  Used only when the run
+# matches (workload, n, k, g, precision); the deposit's figure also needs the
+# same (grids per workgroup, workgroup target), since its re-reads of the
+# positions and its flushes depend on its tiling.  The deposit phase is its
+# prepass plus the LDS kernel.
 PMC_TRAFFIC_KIB = {
     ("two_disks", 1_000_000, 240, 16, "fp32"): {
-        ("deposit", 2, 2048): (7_818.1 + 10_505.8, 15_625.0 + 35_177.7),
-        "gather_update": (123_321.4, 73_737.6),
-        "solve": (3_735.3, 14_242.9),
+        ("deposit", 2, 2048): (7_824.1 + 10_738.8, 15_625.0 + 34_463.9),
+        "gather_update": (122_836.4, 73_900.3),
+        "solve": (3_739.8, 14_242.9),
+    },
+    ("plummer", 4_000_000, 240, 32, "fp32"): {
+        ("deposit", 1, 4096): (31_262.9 + 67_088.7, 62_500.0 + 681_771.4),
+        "gather_update": (730_602.2, 312_880.0),
+        "solve": (69_804.9, 190_845.6),
     },
 }
 # Vector instructions per wave-wide particle-grid pair (SQ_INSTS_VALU / 3.75e6
 # for the kernels of the same tiling, profiles/sq_counters_two_disks_n1e6_k240_g16.md):
 # the deposit and the gather are bound by vector-instruction issue.
 SQ_VALU_PER_WAVE_PAIR = {
-    ("two_disks", 1_000_000, 240, 16, "fp32"): {("deposit", 2, 2048): 25.66, "gather_update": 31.10},
+    ("two_disks", 1_000_000, 240, 16, "fp32"): {("deposit", 2, 2048): 28.53, "gather_update": 28.33},
 }
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
@@ -312,8 +318,9 @@ def main():
     # that bounds the scatter/gather (8 B per LDS atomic read-modify-write,
     # 16 B per cell fetch) and the transforms (flop count of the path used)
     inter = float(n) * k
-    # the FP32 vector path runs radix-4/2 FFTs from g = 16 up; the matrix-core
-    # paths and smaller grids run dense DFTs
+    # the FP32 vector path runs FFTs from g = 16 up (counted as full complex
+    # 3D FFTs, 5 N log2 N flops each: one forward and three inverse); the
+    # matrix-core paths and smaller grids run dense DFTs
     fft = g >= 16 and args.precision == "fp32" and not tuning.get("spectral_mfma", 0)
     dft_flops = (4.0 * 5.0 * g**3 * math.log2(g**3) * k) if fft else 96.0 * k * g**4
     phase_fractions = {}
