@@ -661,6 +661,7 @@ struct mog_batch {
   } cls[kBatchClasses];
   int* d_cls_sim;
   long long* d_cls_off;
+  float4* d_cpos;  // the deposit's prepared rows, one per particle
   // solve windows (kBatchWindowCells): class, first slab and slab count,
   // and per window a row of n_sims scratch offsets (class slot j -> offset
   // of the simulation's grid 0 in the window's scratch)
@@ -735,7 +736,7 @@ void batch_release(mog_batch* b) {
   void* ptrs[] = {b->d_sims, b->d_rot,  b->d_off, b->d_frot, b->d_grids, b->d_S1,
                   b->d_S2,   b->d_force, b->d_tw, b->d_gtab, b->d_kd,    b->d_conv,
                   b->d_tasks, b->d_counter, b->d_order, b->d_idx, b->d_keys_a, b->d_keys_b,
-                  b->d_sort_tmp, b->d_cls_sim, b->d_cls_off, b->d_win_soff};
+                  b->d_sort_tmp, b->d_cls_sim, b->d_cls_off, b->d_win_soff, b->d_cpos};
   for (void* q : ptrs) (void)hipFree(q);
 }
 
@@ -807,8 +808,9 @@ hipError_t batch_enqueue(mog_batch* b, float* pos, float* vel, hipStream_t s) {
   hipError_t e = launch_fill_zero(b->d_grids, b->cells_total, s);
   if (e == hipSuccess) e = launch_fill_zero((long long*)b->d_counter, 1, s);
   if (e == hipSuccess)
-    e = launch_batch_deposit((const float4*)pos, b->d_sims, b->d_tasks, b->ntasks, b->d_rot,
-                             b->d_off, b->d_frot, (unsigned long long*)b->d_grids, b->d_counter,
+    e = launch_batch_deposit((const float4*)pos, b->d_cpos, b->d_sims, b->nsims, b->gather_blocks,
+                             b->deposit_chunk, b->d_tasks, b->ntasks, b->d_rot, b->d_off,
+                             b->d_frot, (unsigned long long*)b->d_grids, b->d_counter,
                              b->deposit_threads, s);
   for (size_t w = 0; w < b->windows.size() && e == hipSuccess; ++w) {
     const mog_batch::SolveWindow& win = b->windows[w];
@@ -1638,6 +1640,7 @@ int mog_batch_create(const mog_config* cfgs, const int64_t* n, int32_t n_sims, m
   A(dalloc(&b->d_kd, (size_t)line_off));
   A(dalloc(&b->d_conv, (size_t)(conv_off > 0 ? conv_off : 1)));
   A(dalloc(&b->d_counter, 1));
+  A(dalloc(&b->d_cpos, (size_t)std::max(p_off, 1LL)));
   A(dalloc(&b->d_cls_sim, (size_t)n_sims));
   A(dalloc(&b->d_cls_off, (size_t)n_sims));
   b->resort_every = 16;

@@ -38,6 +38,23 @@ typedef __attribute__((address_space(3))) unsigned int lds_u32;
 
 // Deferred-pair records per lane of the LDS deposit (see deposit_lds_kernel).
 constexpr int kLaneSlots = 4;
+
+// Rows of the prepared deposit: (c0, c1, c2, milli) with c = p - W/2 and the
+// 32-bit milli-mass, or kPrepMarker where the fast path cannot take the
+// particle (outside |c_j| <= CB, non-finite, or a milli-mass above the
+// partials' limit).  The milli-mass and the admission test are those of the
+// unprepared loop (same float32 operations, saturating conversion).
+constexpr unsigned int kPrepMarker = 0xFFFFFFFFu;
+
+__device__ __forceinline__ float4 prepared_row(const float4 q, const Params& P,
+                                               unsigned int lds_limit32) {
+  const float c0 = q.x - P.half, c1 = q.y - P.half, c2 = q.z - P.half;
+  const float mm = q.w * 1000.0f;
+  unsigned int m;
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(m) : "v"(mm));
+  const bool take = (m <= lds_limit32) & (bool)particle_fast(c0, c1, c2, P.CB);
+  return make_float4(c0, c1, c2, __uint_as_float(take ? m : kPrepMarker));
+}
 // UP particles per lane and register set in the LDS deposit.
 constexpr int kDepositUP = 2;
 
@@ -419,6 +436,23 @@ int batch_task_grids(int cells) {
   return G;
 }
 
+// Prepared rows of every particle of the batch (the rows of
+// deposit_prep_kernel with each simulation's own centre and admission
+// bound), one workgroup of kBatchGatherBlock threads per gather block (a
+// block lies inside one simulation).  lds_limit32: the partials' limit of
+// the batch's largest task, (2^32 - 1) / max(deposit chunk, 256); a task
+// of fewer particles has a higher limit, so the rows a task finds unmarked
+// fit its partials.
+__global__ __launch_bounds__(kBatchGatherBlock) void batch_prep_kernel(
+    const float4* __restrict__ pos, const BatchSim* __restrict__ sims, int nsims,
+    unsigned int lds_limit32, float4* __restrict__ cpos) {
+  const BatchSim& b = sims[sim_of_block(sims, nsims, blockIdx.x)];
+  const long long j = (blockIdx.x - b.blk_off) * kBatchGatherBlock + threadIdx.x;
+  if (j >= b.n) return;
+  const long long i = b.p_off + j;
+  cpos[i] = prepared_row(pos[i], b.P, lds_limit32);
+}
+
 // Persistent deposit of the batched ensemble over its task list (particle
 // chunk x group of grids of one simulation, ordered by cost, largest first:
 // dynamic pickup balances simulations whose n spans 50x, k 7.5x, g^3 8x).
@@ -428,9 +462,9 @@ int batch_task_grids(int cells) {
 // (one clear per workgroup, not one per task).  Dynamic LDS: kBatchLdsCells
 // counters from LDS address 0, the lanes' records, then the task index.
 __global__ __launch_bounds__(1024) void batch_deposit_kernel(
-    const float4* __restrict__ pos, const BatchSim* __restrict__ sims,
-    const DepositTask* __restrict__ tasks, int ntasks, const float* __restrict__ rot,
-    const float* __restrict__ off, const float* __restrict__ frot,
+    const float4* __restrict__ pos, const float4* __restrict__ cpos,
+    const BatchSim* __restrict__ sims, const DepositTask* __restrict__ tasks, int ntasks,
+    const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
     unsigned long long* __restrict__ grids, unsigned long long* __restrict__ counter) {
   extern __shared__ unsigned int hist[];
   unsigned int* const slots = hist + kBatchLdsCells;
@@ -449,13 +483,14 @@ __global__ __launch_bounds__(1024) void batch_deposit_kernel(
     // divisor: a saturated milli-mass is then never taken as small)
     const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)(len > 256 ? len : 256);
     const float4* sp = pos + b.p_off;
+    const float4* cp = cpos + b.p_off;
     const float* R = rot + 12 * b.grid_off;
     const float* O = off + 4 * b.grid_off;
     const float* FR = frot + 12 * b.grid_off;
     unsigned long long* gd = grids + b.cell_off;
 #define MOG_TILE(GG, SHB)                                                                     \
-  deposit_tile<GG, SHB, false>(hist, slots, sp, nullptr, task.p0, task.p1, task.grid0,       \
-                               task.ng, R, O, FR, P, lds_limit, gd, true)
+  deposit_tile<GG, SHB, true>(hist, slots, sp, cp, task.p0, task.p1, task.grid0, task.ng, R,  \
+                              O, FR, P, lds_limit, gd, true)
     if (P.shift_ok & kShiftLds) {
       if (task.G == 4) MOG_TILE(4, true);
       else if (task.G == 2) MOG_TILE(2, true);
@@ -470,11 +505,18 @@ __global__ __launch_bounds__(1024) void batch_deposit_kernel(
   }
 }
 
-hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const DepositTask* tasks,
+hipError_t launch_batch_deposit(const float4* pos, float4* cpos, const BatchSim* sims, int nsims,
+                                long long blocks, long long chunk, const DepositTask* tasks,
                                 int ntasks, const float* rot, const float* off, const float* frot,
                                 unsigned long long* grids, unsigned long long* counter,
                                 int threads, hipStream_t s) {
   if (ntasks <= 0) return hipSuccess;
+  const unsigned int lds_limit32 =
+      (unsigned int)(0xFFFFFFFFull / (unsigned long long)(chunk > 256 ? chunk : 256));
+  hipLaunchKernelGGL(batch_prep_kernel, dim3((unsigned)blocks), dim3(kBatchGatherBlock), 0, s, pos,
+                     sims, nsims, lds_limit32, cpos);
+  hipError_t pe = hipGetLastError();
+  if (pe != hipSuccess) return pe;
   const size_t lds = (size_t)kBatchLdsCells * sizeof(unsigned int) +
                      (size_t)kLaneSlots * threads * sizeof(unsigned int) + 16;
   hipError_t e = hipFuncSetAttribute((const void*)batch_deposit_kernel,
@@ -482,8 +524,8 @@ hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const D
   if (e != hipSuccess) return e;
   // one workgroup per CU holds the 128 KB histogram budget
   const unsigned wg = (unsigned)(ntasks < 256 ? ntasks : 256);
-  hipLaunchKernelGGL(batch_deposit_kernel, dim3(wg), dim3(threads), lds, s, pos, sims, tasks,
-                     ntasks, rot, off, frot, grids, counter);
+  hipLaunchKernelGGL(batch_deposit_kernel, dim3(wg), dim3(threads), lds, s, pos,
+                     (const float4*)cpos, sims, tasks, ntasks, rot, off, frot, grids, counter);
   return hipGetLastError();
 }
 
@@ -622,26 +664,13 @@ hipError_t launch_permute(const float4* pos, const int* order, long long n, floa
 
 static const size_t kMaxLds = 160 * 1024;
 
-// Rows of the prepared deposit: (c0, c1, c2, milli) with c = p - W/2 and the
-// 32-bit milli-mass, or kPrepMarker where the fast path cannot take the
-// particle (outside |c_j| <= CB, non-finite, or a milli-mass above the
-// partials' limit).  The milli-mass and the admission test are those of the
-// unprepared loop (same float32 operations, saturating conversion).
-constexpr unsigned int kPrepMarker = 0xFFFFFFFFu;
-
 __global__ __launch_bounds__(256) void deposit_prep_kernel(const float4* __restrict__ pos,
                                                            long long n, Params P,
                                                            unsigned int lds_limit32,
                                                            float4* __restrict__ cpos) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float4 q = pos[i];
-  const float c0 = q.x - P.half, c1 = q.y - P.half, c2 = q.z - P.half;
-  const float mm = q.w * 1000.0f;
-  unsigned int m;
-  asm("v_cvt_u32_f32 %0, %1" : "=v"(m) : "v"(mm));
-  const bool take = (m <= lds_limit32) & (bool)particle_fast(c0, c1, c2, P.CB);
-  cpos[i] = make_float4(c0, c1, c2, __uint_as_float(take ? m : kPrepMarker));
+  cpos[i] = prepared_row(pos[i], P, lds_limit32);
 }
 
 template <int G>

@@ -325,7 +325,12 @@ hipError_t launch_mog2d_deposit(const float4* pos, long long n, int k, int lg, f
                                 const float* rot, const float* off, const float* frot, float lim,
                                 long long* grids, hipStream_t s);
 
-hipError_t launch_batch_deposit(const float4* pos, const BatchSim* sims, const DepositTask* tasks,
+// The batch's prepass (prepared rows into cpos, one row per particle; the
+// partials' limit of a task of `chunk` particles) and its persistent deposit
+// over the task list; blocks: the batch's gather blocks (kBatchGatherBlock
+// particles of one simulation each).
+hipError_t launch_batch_deposit(const float4* pos, float4* cpos, const BatchSim* sims, int nsims,
+                                long long blocks, long long chunk, const DepositTask* tasks,
                                 int ntasks, const float* rot, const float* off, const float* frot,
                                 unsigned long long* grids, unsigned long long* counter,
                                 int threads, hipStream_t s);
