@@ -132,6 +132,27 @@ def test_solve_matches_fp64(torch_cuda, g, compat, mfma):
         assert np.array_equal(got[..., 0], got[..., 2])
 
 
+# g = 32 with more grids than CUs: the grid solve's persistent workgroups
+# take several grids each and reuse their scratch slot (one per workgroup).
+def test_solve_g32_more_grids_than_cus(torch_cuda):
+    torch = torch_cuda
+    k, g, W, G = 300, 32, 100.0, 100.0
+    assert k > torch.cuda.get_device_properties(0).multi_processor_count
+    rng = np.random.default_rng(32)
+    n = 20000
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = np.mod(rng.normal(50, 12, (n, 3)), W)
+    pos[:, 3] = 1.0
+    s, rot, off = make_solver(k, g, W, G)
+    grids = O.deposit(pos, rot, off, O.Params.make(k, g, W))
+    got = s.solve(dev(torch, grids)).cpu().numpy().astype(np.float64)
+    sample = np.arange(0, k, 7)  # among them grids a workgroup takes second
+    want = O.solve(grids[sample], g, W, G)
+    for j, i in enumerate(sample):
+        err = O.rel_rms(got[i, ..., :3].reshape(-1, 3), want[j, ..., :3].reshape(-1, 3))
+        assert err < GRID_TOL, f"grid {i}: rel rms {err}"
+
+
 def clustered_grids(k, g, W, seed, n=40000):
     rng = np.random.default_rng(seed)
     pos = np.zeros((n, 4), np.float32)
