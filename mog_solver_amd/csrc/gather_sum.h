@@ -62,6 +62,10 @@ __device__ __forceinline__ void gather_sum(float c0, float c1, float c2, int pfa
     FS.m01 = to_vgpr(FS.m01);
     FS.m012 = to_vgpr(FS.m012);
   }
+  // a wave with a particle outside the fast path takes the branch below for
+  // every group (its lanes decide per pair); one wave-uniform flag, taken
+  // once, so that the per-group test is a compare and a scalar branch
+  const bool wave_slow = __builtin_amdgcn_ballot_w64(!pfast) != 0;
   int nrec = 0;  // undecided pairs met so far
   int grid = 0;
   for (; grid + U <= P.k; grid += U) {
@@ -89,11 +93,10 @@ __device__ __forceinline__ void gather_sum(float c0, float c1, float c2, int pfa
       }
       dmax = fmaxf(dmax, d[u]);
     }
-    const bool rare = !(pfast & (dmax < P.lim));
     float4 F[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) F[u] = cell_load_b(force + (long long)(grid + u) * P.cells, cell[u]);
-    if (__ballot(rare)) {
+    if (wave_slow || __builtin_amdgcn_ballot_w64(!(dmax < P.lim)) != 0) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (!(pfast & (d[u] < P.lim))) {
