@@ -44,10 +44,13 @@ struct mog_ctx {
   float* d_gtab;
   float* d_kd;
   float2* d_conv;  // real-space kernel spectrum (solver convolution, compat physical)
-  // closure features: deconvolved Green's function and world-frame force
-  // grids (allocated by the first mog_closure_features call)
+  // closure features: deconvolved Green's function, world-frame force
+  // grids, per-cell force and mass, per-cell interpolation blocks (8 float4)
+  // (allocated by the first mog_closure_features call)
   float* d_gtab_dc;
   float4* d_force_dc;
+  float4* d_fm;
+  float4* d_blk;
   // FP64 diagnostics (allocated by the first mog_diagnostics call):
   // twiddles exp(-2 pi i j / g), the energy table T(k), reduction scratch
   double2* d_diag_tw;
@@ -413,6 +416,8 @@ void release(mog_ctx* c) {
   (void)hipFree(c->d_conv);
   (void)hipFree(c->d_gtab_dc);
   (void)hipFree(c->d_force_dc);
+  (void)hipFree(c->d_fm);
+  (void)hipFree(c->d_blk);
   (void)hipFree(c->d_diag_tw);
   (void)hipFree(c->d_diag_tab);
   (void)hipFree(c->d_diag_scratch);
@@ -1306,11 +1311,17 @@ int mog_closure_features(mog_ctx* ctx, const float* pos, int64_t n, double rho0,
           dc[at] = (float)((double)gt[at] / (win(ix) * win(iy) * win(iz)));
         }
     if (dalloc(&ctx->d_gtab_dc, dc.size()) != hipSuccess ||
-        dalloc(&ctx->d_force_dc, (size_t)P.k * P.cells) != hipSuccess) {
+        dalloc(&ctx->d_force_dc, (size_t)P.k * P.cells) != hipSuccess ||
+        dalloc(&ctx->d_fm, (size_t)P.k * P.cells) != hipSuccess ||
+        dalloc(&ctx->d_blk, (size_t)P.k * P.cells * 8) != hipSuccess) {
       (void)hipFree(ctx->d_gtab_dc);
       (void)hipFree(ctx->d_force_dc);
+      (void)hipFree(ctx->d_fm);
+      (void)hipFree(ctx->d_blk);
       ctx->d_gtab_dc = nullptr;
       ctx->d_force_dc = nullptr;
+      ctx->d_fm = nullptr;
+      ctx->d_blk = nullptr;
       return fail(ctx, MOG_ENOMEM, "mog_closure_features: deconvolved grids");
     }
     hipError_t ec = hipMemcpy(ctx->d_gtab_dc, dc.data(), dc.size() * sizeof(float),
@@ -1318,11 +1329,23 @@ int mog_closure_features(mog_ctx* ctx, const float* pos, int64_t n, double rho0,
     if (ec != hipSuccess) return hip_fail(ctx, ec, "mog_closure_features");
   }
   hipError_t e = launch_fill_zero(ctx->d_grids, (long long)P.k * P.cells, s);
+  // the few grids of a closure context spread the workgroup target over
+  // many small chunks: keep each workgroup's pairs well above its counters,
+  // whose flush otherwise costs as much as the deposit
+  DepositTune dt = ctx->tune;
+  dt.min_chunk = 4ll * P.cells;
   if (e == hipSuccess)
     e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                       (unsigned long long*)ctx->d_grids, ctx->tune, s);
-  if (e == hipSuccess) e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
-  if (e == hipSuccess)
+                       (unsigned long long*)ctx->d_grids, dt, s);
+  // the plain and the deconvolved force grids: one launch of both solves
+  // where the grid solve of this size runs them (g = 16, vector FFT path)
+  const bool pair = P.g == 16 && !ctx->spectral_mfma && ctx->cfg.precision == MOG_PRECISION_FP32;
+  if (e == hipSuccess && pair)
+    e = launch_solve_pair(P, ctx->d_grids, ctx->d_force, ctx->d_gtab, ctx->d_conv,
+                          ctx->d_force_dc, ctx->d_gtab_dc, nullptr, ctx->d_kd, ctx->need_x,
+                          ctx->d_rot, 1, s);
+  if (e == hipSuccess && !pair) e = solve_dispatch(ctx, ctx->d_grids, ctx->d_force, 1, s);
+  if (e == hipSuccess && !pair)
     e = launch_solve(P, ctx->d_grids, ctx->d_S1, ctx->s1_elems, ctx->d_S2, ctx->d_force_dc,
                      ctx->d_tw,
                      ctx->d_gtab_dc, ctx->d_kd, nullptr, ctx->need_x, ctx->d_rot, 1, s);
@@ -1331,7 +1354,7 @@ int mog_closure_features(mog_ctx* ctx, const float* pos, int64_t n, double rho0,
   if (e == hipSuccess)
     e = launch_closure_features(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
                                 ctx->d_force, ctx->d_force_dc, ctx->d_grids, (float)rho0, order,
-                                (float4*)feat, (float4*)mean, s);
+                                ctx->d_fm, ctx->d_blk, (float4*)feat, (float4*)mean, s);
   return e == hipSuccess ? MOG_OK : hip_fail(ctx, e, "mog_closure_features");
 }
 

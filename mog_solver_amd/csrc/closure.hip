@@ -120,12 +120,84 @@ __device__ void wg_gemm(const Mat A, const Mat B, float* C, int ldc, int M, int 
 // ---------------------------------------------------------------------------
 // Features of a k-grid ensemble (context grids and world-frame force grids)
 // ---------------------------------------------------------------------------
+// Per cell of the k context grids (one thread per cell): fm[c] = (F_c.xyz,
+// mass of the cell x 1e-3, the value the features' rho sums), and the
+// trilinear form of the deconvolved field D over the octant above the cell:
+// with D_j at the corner j = (jx, jy, jz) = (j & 1, j >> 1 & 1, j >> 2)
+// (cell c + jx x + jy y + jz z, periodic),
+//   D(c + f) = a0 + a1 fx + a2 fy + a3 fz + a4 fx fy + a5 fx fz + a6 fy fz
+//              + a7 fx fy fz,   f in [0, 1)^3,
+// i.e. the sum over corners of D_j times the weights (fx or 1 - fx)(fy or
+// 1 - fy)(fz or 1 - fz).  A cell's block holds a_i.xyz at float 3 i + m
+// (24 floats) and 8 floats of padding: 128 bytes, so a particle reads its
+// interpolation with six 16-byte loads from one address.
+__global__ __launch_bounds__(256) void closure_grids_kernel(const float4* __restrict__ force,
+                                                            const float4* __restrict__ force_dc,
+                                                            const long long* __restrict__ grids,
+                                                            long long total, int lg,
+                                                            float4* __restrict__ fm,
+                                                            float4* __restrict__ blk) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= total) return;
+  const int gm = (1 << lg) - 1;
+  const long long cells = 1ll << (3 * lg);
+  const long long base = c & ~(cells - 1);
+  const int cell = (int)(c & (cells - 1));
+  const int x = cell & gm, y = (cell >> lg) & gm, z = cell >> (2 * lg);
+  const float4 F = force[c];
+  fm[c] = make_float4(F.x, F.y, F.z, (float)grids[c] * 1e-3f);
+  float d[8][3];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int xx = (x + (j & 1)) & gm, yy = (y + ((j >> 1) & 1)) & gm, zz = (z + (j >> 2)) & gm;
+    const float4 v = force_dc[base + ((((long long)zz << lg) | yy) << lg | xx)];
+    d[j][0] = v.x;
+    d[j][1] = v.y;
+    d[j][2] = v.z;
+  }
+  float a[32];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    a[m] = d[0][m];
+    a[3 + m] = d[1][m] - d[0][m];
+    a[6 + m] = d[2][m] - d[0][m];
+    a[9 + m] = d[4][m] - d[0][m];
+    a[12 + m] = (d[3][m] - d[1][m]) - (d[2][m] - d[0][m]);
+    a[15 + m] = (d[5][m] - d[1][m]) - (d[4][m] - d[0][m]);
+    a[18 + m] = (d[6][m] - d[2][m]) - (d[4][m] - d[0][m]);
+    a[21 + m] = ((d[7][m] - d[6][m]) - (d[5][m] - d[4][m])) - ((d[3][m] - d[2][m]) - (d[1][m] - d[0][m]));
+  }
+#pragma unroll
+  for (int i = 24; i < 32; ++i) a[i] = 0.f;
+  float4* out = blk + 8 * c;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) out[i] = make_float4(a[4 * i], a[4 * i + 1], a[4 * i + 2], a[4 * i + 3]);
+}
+
+// The trilinear form of one component at f (closure_grids_kernel), Horner in
+// fz, then fy, then fx.
+__device__ __forceinline__ float trilinear(float a0, float a1, float a2, float a3, float a4,
+                                           float a5, float a6, float a7, float fx, float fy,
+                                           float fz) {
+  const float tx = fmaf(fmaf(a7, fz, a4), fy, fmaf(a5, fz, a1));
+  const float t0 = fmaf(fmaf(a6, fz, a2), fy, fmaf(a3, fz, a0));
+  return fmaf(tx, fx, t0);
+}
+
+// Per particle, over the k grids: the NGP cell of the fast path (the force
+// and cell mass from fm), the continuous cell coordinate u = l/h - 1/2 (cell
+// centres at integers: the fast row's w) and the interpolation block of the
+// cell floor(u).  NGP pairs the fast path cannot decide are deferred as in
+// the force gather: recorded in the lane's LDS slots, their exact cells
+// evaluated after the grid loop.  SH: NGP and block offsets through the
+// shifted cell fields (kShiftF4 and kShiftBlk); otherwise from indices.
+template <bool SH>
 __global__ __launch_bounds__(256) void closure_features_kernel(
     const float4* __restrict__ pos, long long n, const float* __restrict__ rot,
     const float* __restrict__ off, const float* __restrict__ frot,
-    const float4* __restrict__ force, const float4* __restrict__ force_dc,
-    const long long* __restrict__ grids, Params P, float rho0, const int* __restrict__ order,
-    float4* __restrict__ feat, float4* __restrict__ mean_out) {
+    const float4* __restrict__ fm, const float4* __restrict__ blk, Params P, float rho0,
+    const int* __restrict__ order, float4* __restrict__ feat, float4* __restrict__ mean_out) {
+  __shared__ unsigned recs[kGatherSlots * 256];
   const long long t = (long long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
   // visiting order (spatially sorted when given): neighbouring lanes, and
@@ -134,21 +206,80 @@ __global__ __launch_bounds__(256) void closure_features_kernel(
   const float4 p = pos[i];
   const float c0 = p.x - P.half, c1 = p.y - P.half, c2 = p.z - P.half;
   const int pfast = particle_fast(c0, c1, c2, P.CB);
+  FieldShift FS = field_shift(P.lg, 4), FB = field_shift(P.lg, 7);
+  if constexpr (SH) {  // vector-register operands issue at the full rate
+    FS.k0 = to_vgpr(FS.k0);
+    FS.k1 = to_vgpr(FS.k1);
+    FS.k2 = to_vgpr(FS.k2);
+    FS.m0 = to_vgpr(FS.m0);
+    FS.m01 = to_vgpr(FS.m01);
+    FS.m012 = to_vgpr(FS.m012);
+    FB.k0 = to_vgpr(FB.k0);
+    FB.k1 = to_vgpr(FB.k1);
+    FB.k2 = to_vgpr(FB.k2);
+    FB.m0 = to_vgpr(FB.m0);
+    FB.m01 = to_vgpr(FB.m01);
+    FB.m012 = to_vgpr(FB.m012);
+  }
   const int gm = P.g - 1;
-  // Sums are taken about the first grid's force (shifted data), so the
+  const bool wave_slow = __builtin_amdgcn_ballot_w64(!pfast) != 0;
+  const int lane = threadIdx.x;
+  // Sums are taken about a force of the first grid (shifted data), so the
   // variance does not cancel catastrophically when the spread is small.
   float bx = 0.f, by = 0.f, bz = 0.f;
   float sx = 0.f, sy = 0.f, sz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, sm = 0.f;
   float cx = 0.f, cy = 0.f, cz = 0.f;
+  int nrec = 0;
   for (int g = 0; g < P.k; ++g) {
     const float* F = frot + 12 * g;
-    const int cell = cell_index(c0, c1, c2, pfast, F, rot + 12 * g, off + 4 * g, P);
-    const long long base = (long long)g * P.cells;
-    const float4 f = force[base + cell];
-    if (g == 0) {
+    const float w0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, F[3])));
+    const float w1 = fmaf(F[6], c2, fmaf(F[5], c1, fmaf(F[4], c0, F[7])));
+    const float w2 = fmaf(F[10], c2, fmaf(F[9], c1, fmaf(F[8], c0, F[11])));
+    const float fl0 = floorf(w0), fl1 = floorf(w1), fl2 = floorf(w2);
+    const float4* Fg = fm + (long long)g * P.cells;
+    const float4* Bg = blk + (long long)g * P.cells * 8;
+    unsigned o;
+    float d;
+    float4 a[6];
+    if constexpr (SH) {
+      d = cell_fast_o(c0, c1, c2, F, F[3], F[7], F[11], FS, 0u, &o);
+      const float u0 = fl0 + FB.k0, u1 = fl1 + FB.k1, u2 = fl2 + FB.k2;
+      unsigned b0, b1, b2;
+      memcpy(&b0, &u0, 4);
+      memcpy(&b1, &u1, 4);
+      memcpy(&b2, &u2, 4);
+      const unsigned ob = and_or(bit_select(bit_select(b0, b1, FB.m0), b2, FB.m01), FB.m012, 0u);
+      const float4* B = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(Bg) + ob);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) a[j] = B[j];
+    } else {
+      int cell;
+      d = cell_fast_d(c0, c1, c2, F, F[3], F[7], F[11], P.lg, &cell);
+      o = (unsigned)cell << 4;
+      const long long cb = (((long long)((int)fl2 & gm) << P.lg | ((int)fl1 & gm)) << P.lg) |
+                           ((int)fl0 & gm);
+      const float4* B = Bg + 8 * cb;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) a[j] = B[j];
+    }
+    float4 f = cell_load_b(Fg, o);
+    if (g == 0) {  // any force of the particle's grids serves as the shift
       bx = f.x;
       by = f.y;
       bz = f.z;
+    }
+    if (wave_slow || __builtin_amdgcn_ballot_w64(!(d < P.lim)) != 0) {
+      if (!(pfast & (d < P.lim))) {
+        if (nrec < kGatherSlots) {  // adds nothing now: zero deviation, no mass
+          recs[nrec * 256 + lane] = (unsigned)g;
+          f = make_float4(bx, by, bz, 0.f);
+        } else {
+          const int ex =
+              cell_exact_call(c0, c1, c2, rot + 12 * g, off + 4 * g, P.W, P.half, P.h, P.lg);
+          f = cell_load(Fg, ex);
+        }
+        ++nrec;
+      }
     }
     const float ex = f.x - bx, ey = f.y - by, ez = f.z - bz;
     sx += ex;
@@ -157,28 +288,26 @@ __global__ __launch_bounds__(256) void closure_features_kernel(
     qx = fmaf(ex, ex, qx);
     qy = fmaf(ey, ey, qy);
     qz = fmaf(ez, ez, qz);
-    sm += (float)grids[base + cell] * 1e-3f;
-    // continuous cell coordinate u = l/h - 1/2 (cell centres at integers)
-    // from the fast row, then trilinear weights over the 8 surrounding
-    // cell centres of the periodic grid
-    const float u0 = fmaf(F[2], c2, fmaf(F[1], c1, fmaf(F[0], c0, F[3])));
-    const float u1 = fmaf(F[6], c2, fmaf(F[5], c1, fmaf(F[4], c0, F[7])));
-    const float u2 = fmaf(F[10], c2, fmaf(F[9], c1, fmaf(F[8], c0, F[11])));
-    const float fl0 = floorf(u0), fl1 = floorf(u1), fl2 = floorf(u2);
-    const float w0 = u0 - fl0, w1 = u1 - fl1, w2 = u2 - fl2;
-    const int x0 = (int)fl0 & gm, y0 = (int)fl1 & gm, z0 = (int)fl2 & gm;
-    const int x1 = (x0 + 1) & gm, y1 = (y0 + 1) & gm, z1 = (z0 + 1) & gm;
-    const float4* D = force_dc + base;
-#pragma unroll
-    for (int corner = 0; corner < 8; ++corner) {
-      const int xx = corner & 1 ? x1 : x0, yy = corner & 2 ? y1 : y0, zz = corner & 4 ? z1 : z0;
-      const float w = (corner & 1 ? w0 : 1.f - w0) * (corner & 2 ? w1 : 1.f - w1) *
-                      (corner & 4 ? w2 : 1.f - w2);
-      const float4 d = D[(((zz << P.lg) | yy) << P.lg) | xx];
-      cx = fmaf(w, d.x, cx);
-      cy = fmaf(w, d.y, cy);
-      cz = fmaf(w, d.z, cz);
-    }
+    sm += f.w;
+    const float fx = w0 - fl0, fy = w1 - fl1, fz = w2 - fl2;
+    cx += trilinear(a[0].x, a[0].w, a[1].z, a[2].y, a[3].x, a[3].w, a[4].z, a[5].y, fx, fy, fz);
+    cy += trilinear(a[0].y, a[1].x, a[1].w, a[2].z, a[3].y, a[4].x, a[4].w, a[5].z, fx, fy, fz);
+    cz += trilinear(a[0].z, a[1].y, a[2].x, a[2].w, a[3].z, a[4].y, a[5].x, a[5].w, fx, fy, fz);
+  }
+  // the deferred NGP pairs
+  const int mine = min(nrec, kGatherSlots);
+  for (int r = 0; r < mine; ++r) {
+    const int g = (int)recs[r * 256 + lane];
+    const int ec = cell_exact_call(c0, c1, c2, rot + 12 * g, off + 4 * g, P.W, P.half, P.h, P.lg);
+    const float4 f = cell_load(fm + (long long)g * P.cells, ec);
+    const float ex = f.x - bx, ey = f.y - by, ez = f.z - bz;
+    sx += ex;
+    sy += ey;
+    sz += ez;
+    qx = fmaf(ex, ex, qx);
+    qy = fmaf(ey, ey, qy);
+    qz = fmaf(ez, ez, qz);
+    sm += f.w;
   }
   const float ik = 1.0f / (float)P.k;
   const float ux = sx * ik, uy = sy * ik, uz = sz * ik;
@@ -337,17 +466,30 @@ __global__ __launch_bounds__(256) void mlp_forward_reg_kernel(const float* __res
   const long long groups = (n + 15) >> 4;
   const int wpb = blockDim.x >> 6;
   const long long nwaves = (long long)gridDim.x * wpb;
-  for (long long gi = (long long)blockIdx.x * wpb + (threadIdx.x >> 6); gi < groups; gi += nwaves) {
+  // lane group 0 supplies inputs 0..7, group 1 inputs 8, 9 and the bias
+  // slot 10 (= 1), groups 2 and 3 zeros; the next group's inputs (and its
+  // base force) are loaded while the current group runs through the layers
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto fetch = [&](long long g, float4& a, float4& b, float4& c, float4& m) {
+    const long long p = g * 16 + r;
+    a = b = c = m = zero4;
+    if (g < groups && p < n) {
+      if (q == 0) {
+        a = feat[3 * p];
+        b = feat[3 * p + 1];
+        m = mean[p];
+      }
+      if (q <= 1) c = feat[3 * p + 2];
+    }
+  };
+  long long gi = (long long)blockIdx.x * wpb + (threadIdx.x >> 6);
+  float4 fa, fb, fc, fm;
+  fetch(gi, fa, fb, fc, fm);
+  for (; gi < groups; gi += nwaves) {
     const long long p = gi * 16 + r;
     const bool live = p < n;
-    // lane group 0 supplies inputs 0..7, group 1 inputs 8, 9 and the bias
-    // slot 10 (= 1), groups 2 and 3 zeros
-    float4 fa = make_float4(0.f, 0.f, 0.f, 0.f), fb = fa, fc = fa;
-    if (live && q == 0) {
-      fa = feat[3 * p];
-      fb = feat[3 * p + 1];
-    }
-    if (live && q <= 1) fc = feat[3 * p + 2];
+    float4 na, nb, nc, nm;
+    fetch(gi + nwaves, na, nb, nc, nm);
     v8bf x;
     if (q == 0) {
       x[0] = (__bf16)fa.x;
@@ -394,10 +536,12 @@ __global__ __launch_bounds__(256) void mlp_forward_reg_kernel(const float* __res
       y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[0], a0, b4, 0, 0, 0);
       y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[1], a1, y, 0, 0, 0);
     }
-    if (live && q == 0) {
-      const float4 m = mean[p];
-      out[p] = make_float4(fmaf(fc.z, y[0], m.x), fmaf(fc.z, y[1], m.y), fmaf(fc.z, y[2], m.z), 0.f);
-    }
+    if (live && q == 0)
+      out[p] = make_float4(fmaf(fc.z, y[0], fm.x), fmaf(fc.z, y[1], fm.y), fmaf(fc.z, y[2], fm.z), 0.f);
+    fa = na;
+    fb = nb;
+    fc = nc;
+    fm = nm;
   }
 }
 
@@ -642,10 +786,19 @@ hipError_t launch_closure_features(const Params& P, const float4* pos, long long
                                    const float* rot, const float* off, const float* frot,
                                    const float4* force, const float4* force_dc,
                                    const long long* grids, float rho0, const int* order,
-                                   float4* feat, float4* mean, hipStream_t s) {
+                                   float4* fm, float4* blk, float4* feat, float4* mean,
+                                   hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(closure_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     pos, n, rot, off, frot, force, force_dc, grids, P, rho0, order, feat, mean);
+  const long long total = (long long)P.k * P.cells;
+  hipLaunchKernelGGL(closure_grids_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                     force, force_dc, grids, total, P.lg, fm, blk);
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if ((P.shift_ok & kShiftF4) && (P.shift_ok & kShiftBlk))
+    hipLaunchKernelGGL(closure_features_kernel<true>, grid, dim3(256), 0, s, pos, n, rot, off, frot,
+                       fm, blk, P, rho0, order, feat, mean);
+  else
+    hipLaunchKernelGGL(closure_features_kernel<false>, grid, dim3(256), 0, s, pos, n, rot, off,
+                       frot, fm, blk, P, rho0, order, feat, mean);
   return hipGetLastError();
 }
 

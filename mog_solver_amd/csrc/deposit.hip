@@ -776,6 +776,7 @@ hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, co
   // takes the double-buffered, unguarded loop of deposit_lds_kernel
   const long long span = 4ll * threads;
   long long chunk = (n + chunks - 1) / chunks;
+  if (chunk < t.min_chunk) chunk = t.min_chunk;
   chunk = (chunk + span - 1) / span * span;
   chunks = (n + chunk - 1) / chunk;
   switch (G) {

@@ -31,14 +31,18 @@ struct Params {
 };
 
 // Params::shift_ok bits: the shifted fields hold for byte offsets of 4-byte
-// (LDS counters, s0 = 2) and 16-byte (float4 forces, s0 = 4) elements.
+// (LDS counters, s0 = 2) and 16-byte (float4 forces, s0 = 4) elements of
+// rint(w), and of 128-byte elements (s0 = 7) of floor(w) (|floor(w)| <=
+// |rint(w)| + 1: the closure's interpolation blocks).
 constexpr int kShiftLds = 1;
 constexpr int kShiftF4 = 2;
+constexpr int kShiftBlk = 4;
 
 // shift_ok for transforms with |R| <= rmax and |off| <= omax.
 inline int shift_flags(double W, int g, int lg, double rmax, double omax, double CB) {
   const double wb = fast_w_bound(W, g, rmax, omax, CB);
-  return (field_shift_ok(lg, 2, wb) ? kShiftLds : 0) | (field_shift_ok(lg, 4, wb) ? kShiftF4 : 0);
+  return (field_shift_ok(lg, 2, wb) ? kShiftLds : 0) | (field_shift_ok(lg, 4, wb) ? kShiftF4 : 0) |
+         (field_shift_ok(lg, 7, wb + 0.5) ? kShiftBlk : 0);
 }
 
 // The exact float32 sequence, kept out of line (one copy per translation
@@ -140,6 +144,7 @@ struct DepositTune {
   int grids_per_wg;   // grids privatised in LDS per workgroup
   int wg_target;      // aim for about this many workgroups
   int prepass = 0;    // prepared rows: 0 from 2^26 pairs, 1 always, -1 never
+  long long min_chunk = 0;  // particles per workgroup at least this many
 };
 
 hipError_t launch_fill_zero(long long* p, long long count, hipStream_t s);
@@ -220,11 +225,19 @@ hipError_t noalloc_scratch(void** ptr);
 hipError_t launch_init_perlin(float4* pos, float4* vel, long long n, float W, float noise_scale,
                               float contrast, unsigned seed, float base_mass, float spawn_buffer,
                               int sphere, hipStream_t s);
+hipError_t launch_solve_pair(const Params& P, const long long* grids, float4* force_a,
+                             const float* gtab_a, const float2* conv_a, float4* force_b,
+                             const float* gtab_b, const float2* conv_b, const float* kd,
+                             int need_x, const float* rot, int world, hipStream_t s);
+// Closure features: a per-cell pass over the k context grids writes fm
+// (k * cells float4) and blk (k * cells * 8 float4), then the per-particle
+// pass reads them.
 hipError_t launch_closure_features(const Params& P, const float4* pos, long long n,
                                    const float* rot, const float* off, const float* frot,
                                    const float4* force, const float4* force_dc,
                                    const long long* grids, float rho0, const int* order,
-                                   float4* feat, float4* mean, hipStream_t s);
+                                   float4* fm, float4* blk, float4* feat, float4* mean,
+                                   hipStream_t s);
 size_t mlp_param_count();
 hipError_t launch_direct_sum(const float4* pos, long long n, const int* targets, int nt, double W,
                              double G_eff, double eps2, double* out, hipStream_t s);

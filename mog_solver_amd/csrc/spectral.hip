@@ -517,6 +517,15 @@ struct RfSingle {
   }
 };
 
+// Two solves of one density in one launch: the first `slabs` slabs (k grids
+// x g slabs) are those of `a`, the next `slabs` those of `b` (e.g. two
+// Green's functions, two outputs).
+struct RfPair {
+  RfSingle a, b;
+  long long slabs;
+  __device__ RfSlab operator()(long long t) const { return t < slabs ? a(t) : b(t - slabs); }
+};
+
 // A window of a batch grid-size class: slab t of the class belongs to class
 // slot j; its spectral scratch starts at S + soff[j] in the window buffers.
 struct RfBatch {
@@ -1049,6 +1058,20 @@ static hipError_t launch_rf_solve(const Params& P, const long long* grids, float
   const RfSingle res = {grids, S1, S2, force, gtab, kd, conv, rot, need_x};
   if (G == 16) return launch_rf_grids<16>(res, 0, P.k, world, s);
   return launch_rh_grids(res, 0, P.k, world, S1, s1_elems, s);
+}
+
+// Two g = 16 solves of the same density grids in one launch of 2k
+// workgroups: force_a from (gtab_a, conv_a), force_b from (gtab_b, conv_b).
+// Other sizes return hipErrorNotSupported (the caller runs two solves).
+hipError_t launch_solve_pair(const Params& P, const long long* grids, float4* force_a,
+                             const float* gtab_a, const float2* conv_a, float4* force_b,
+                             const float* gtab_b, const float2* conv_b, const float* kd,
+                             int need_x, const float* rot, int world, hipStream_t s) {
+  if (P.g != 16) return hipErrorNotSupported;
+  const RfPair res = {{grids, nullptr, nullptr, force_a, gtab_a, kd, conv_a, rot, need_x},
+                      {grids, nullptr, nullptr, force_b, gtab_b, kd, conv_b, rot, need_x},
+                      (long long)P.k * 16};
+  return launch_rf_grids<16>(res, 0, 2ll * P.k, world, s);
 }
 
 // ---------------------------------------------------------------------------
