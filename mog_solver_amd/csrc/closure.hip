@@ -679,6 +679,361 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(const float* __restrict_
   if (threadIdx.x == 0) atomicAdd(loss, LS[0]);
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident BF16 training pass (the production training kernel)
+// ---------------------------------------------------------------------------
+// One wave takes 16 particles per iteration through the forward of
+// mlp_forward_reg_kernel (same fragments, same layout: lane (q, r) holds
+// features 16t + 4q + i of particle r), then the backward in that layout:
+//   dy  = 2 w (y - t)                                   (lanes q = 0)
+//   d3  = (W4 dy) . silu'(z3),  d2 = (W3 d3) . silu'(z2),  d1 = (W2 d2) . silu'(z1)
+// as v_mfma_f32_16x16x32_bf16 products with "transposed" weight fragments
+// (A[r][k] = W[16t + r][kperm(c, q, j)]), so the deltas chain like the
+// activations with no data movement.  The weight gradients sum over
+// particles, which sit on the lane axis: each factor is transposed through a
+// wave-private LDS tile (rows = the 16 particles, 68 bf16 per row so that
+// the four lane groups read disjoint banks) and multiplied with
+// v_mfma_f32_16x16x16_bf16 (K = the 16 particles) into per-wave FP32
+// accumulators (160 registers).  Layer 1's bias gradient is row 10 of the
+// W1 product (the input slot holding 1); the other bias gradients are
+// per-lane sums of the deltas.  At the end the waves of a workgroup add
+// their accumulators into one LDS copy, which is flushed to `grad` with
+// global atomics.  The weight fragments live in LDS, built once per
+// workgroup.
+typedef short v4s __attribute__((ext_vector_type(4)));
+constexpr int kFrW1 = 0, kFrW2 = 4, kFrW3 = 12, kFrW4 = 20, kFrW4T = 22, kFrW3T = 26,
+              kFrW2T = 34, kFrags = 42;
+constexpr int kTR = 68;                 // transpose tile row stride (bf16)
+constexpr int kTile = 16 * kTR;         // one transpose tile (bf16 elements)
+constexpr size_t kTrainLds = (size_t)kFrags * 64 * 16 + (size_t)9 * 64 * 16 +
+                             (size_t)4 * 2 * kTile * 2 + (size_t)kParams * 4 + 16;
+
+__device__ __forceinline__ short bf16_bits(float x) {
+  return __builtin_bit_cast(short, (__bf16)x);
+}
+
+// the 16x16x16 operand of tile `tile` of a transpose tile: lane (q, r) gets
+// feature 16 tile + r of particles 4q .. 4q + 3
+__device__ __forceinline__ v4s tr_operand(const unsigned short* tb, int tile, int q, int r) {
+  const unsigned short* b = tb + 4 * q * kTR + 16 * tile + r;
+  v4s v;
+  v[0] = (short)b[0];
+  v[1] = (short)b[kTR];
+  v[2] = (short)b[2 * kTR];
+  v[3] = (short)b[3 * kTR];
+  return v;
+}
+
+// four bf16 values at an 8-byte aligned tile position (row r * kTR + 4 m)
+__device__ __forceinline__ void tr_put4(unsigned short* at, uint2 v) {
+  *reinterpret_cast<uint2*>(at) = v;
+}
+
+// a lane's 16 features (two chunks in kperm order) into row r of a tile
+__device__ __forceinline__ void tr_put_chunks(unsigned short* tb, const v8bf& c0, const v8bf& c1,
+                                              int q, int r) {
+  unsigned short* row = tb + r * kTR + 4 * q;
+  const uint4 a = __builtin_bit_cast(uint4, c0), b = __builtin_bit_cast(uint4, c1);
+  tr_put4(row, make_uint2(a.x, a.y));
+  tr_put4(row + 16, make_uint2(a.z, a.w));
+  tr_put4(row + 32, make_uint2(b.x, b.y));
+  tr_put4(row + 48, make_uint2(b.z, b.w));
+}
+
+// orders a wave's LDS writes and reads of its transpose tiles in the
+// compiler (the hardware runs one wave's LDS instructions in order)
+__device__ __forceinline__ void tile_fence() { asm volatile("" ::: "memory"); }
+
+__device__ __forceinline__ v8bf bf16_chunk(const v4f& lo, const v4f& hi) {
+  v8bf b;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    b[i] = (__bf16)lo[i];
+    b[4 + i] = (__bf16)hi[i];
+  }
+  return b;
+}
+
+__global__ __launch_bounds__(256) void mlp_train_reg_kernel(const float* __restrict__ prm,
+                                                            const float4* __restrict__ feat,
+                                                            const float4* __restrict__ mean,
+                                                            const float4* __restrict__ target,
+                                                            long long n, float wnorm,
+                                                            float* __restrict__ grad,
+                                                            float* __restrict__ loss) {
+  extern __shared__ __align__(16) char tsm[];
+  v8bf* const FR = reinterpret_cast<v8bf*>(tsm);                      // kFrags x 64 lanes
+  v4f* const BI = reinterpret_cast<v4f*>(FR + kFrags * 64);           // b2[4], b3[4], b4 x 64
+  unsigned short* const TB = reinterpret_cast<unsigned short*>(BI + 9 * 64);
+  float* const G = reinterpret_cast<float*>(TB + 4 * 2 * kTile);       // kParams
+  float* const LS = G + kParams;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4, r = lane & 15;
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) G[i] = 0.f;
+  if (threadIdx.x == 0) LS[0] = 0.f;
+  // fragments: wave w builds those with index = w (mod 4), each lane its own
+  for (int f = wave; f < kFrags; f += 4) {
+    v8bf v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = 8 * q + j;
+      float x = 0.f;
+      if (f < kFrW2) {
+        const int t = f - kFrW1;
+        x = k < kIn ? prm[oW1 + k * kH + 16 * t + r] : (k == kIn ? prm[ob1 + 16 * t + r] : 0.f);
+      } else if (f < kFrW3) {
+        const int t = (f - kFrW2) >> 1, c = (f - kFrW2) & 1;
+        x = prm[oW2 + kperm(c, q, j) * kH + 16 * t + r];
+      } else if (f < kFrW4) {
+        const int t = (f - kFrW3) >> 1, c = (f - kFrW3) & 1;
+        x = prm[oW3 + kperm(c, q, j) * kH + 16 * t + r];
+      } else if (f < kFrW4T) {
+        const int c = f - kFrW4;
+        x = r < kOut ? prm[oW4 + kperm(c, q, j) * kOut + r] : 0.f;
+      } else if (f < kFrW3T) {
+        const int t = f - kFrW4T;
+        x = k < kOut ? prm[oW4 + (16 * t + r) * kOut + k] : 0.f;
+      } else if (f < kFrW2T) {
+        const int t = (f - kFrW3T) >> 1, c = (f - kFrW3T) & 1;
+        x = prm[oW3 + (16 * t + r) * kH + kperm(c, q, j)];
+      } else {
+        const int t = (f - kFrW2T) >> 1, c = (f - kFrW2T) & 1;
+        x = prm[oW2 + (16 * t + r) * kH + kperm(c, q, j)];
+      }
+      v[j] = (__bf16)x;
+    }
+    FR[f * 64 + lane] = v;
+  }
+  if (wave == 0) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      v4f a, b;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = prm[ob2 + 16 * t + 4 * q + i];
+        b[i] = prm[ob3 + 16 * t + 4 * q + i];
+      }
+      BI[t * 64 + lane] = a;
+      BI[(4 + t) * 64 + lane] = b;
+    }
+    v4f c;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[i] = (4 * q + i < kOut) ? prm[ob4 + 4 * q + i] : 0.f;
+    BI[8 * 64 + lane] = c;
+  }
+  __syncthreads();
+  unsigned short* const tA = TB + wave * 2 * kTile;
+  unsigned short* const tB = tA + kTile;
+  const v4f zero = {0.f, 0.f, 0.f, 0.f};
+  v4f gW1[4], gW2[4][4], gW3[4][4], gW4[4], gb2[4], gb3[4], gb4 = zero;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    gW1[a] = gW4[a] = gb2[a] = gb3[a] = zero;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) gW2[a][b] = gW3[a][b] = zero;
+  }
+  float lacc = 0.f;
+  const long long groups = (n + 15) >> 4;
+  const long long nwaves = (long long)gridDim.x * (blockDim.x >> 6);
+  // the next group's inputs are loaded while the current one is processed
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto fetch = [&](long long g, float4& a, float4& b, float4& c, float4& m, float4& tg) {
+    const long long pp = g * 16 + r;
+    a = b = c = m = tg = zero4;
+    if (g < groups && pp < n) {
+      if (q == 0) {
+        a = feat[3 * pp];
+        b = feat[3 * pp + 1];
+        m = mean[pp];
+        tg = target[pp];
+      }
+      if (q <= 1) c = feat[3 * pp + 2];
+    }
+  };
+  long long gi = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+  float4 fa, fb, fc, fm, ft;
+  fetch(gi, fa, fb, fc, fm, ft);
+  for (; gi < groups; gi += nwaves) {
+    const long long p = gi * 16 + r;
+    const bool live = p < n;
+    float4 na, nb, nc, nm, nt4;
+    fetch(gi + nwaves, na, nb, nc, nm, nt4);
+    v8bf x;
+    if (q == 0) {
+      x[0] = (__bf16)fa.x;
+      x[1] = (__bf16)fa.y;
+      x[2] = (__bf16)fa.z;
+      x[3] = (__bf16)fa.w;
+      x[4] = (__bf16)fb.x;
+      x[5] = (__bf16)fb.y;
+      x[6] = (__bf16)fb.z;
+      x[7] = (__bf16)fb.w;
+    } else {
+      x[0] = (__bf16)fc.x;
+      x[1] = (__bf16)fc.y;
+      x[2] = (__bf16)(q == 1 ? 1.f : 0.f);
+#pragma unroll
+      for (int j = 3; j < 8; ++j) x[j] = (__bf16)0.f;
+    }
+    // forward
+    v4f z1[4], z2[4], z3[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      z1[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW1 + t) * 64 + lane], x, zero, 0, 0, 0);
+    const v8bf a1c0 = silu_chunk(z1[0], z1[1]), a1c1 = silu_chunk(z1[2], z1[3]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      z2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW2 + 2 * t) * 64 + lane], a1c0,
+                                                      BI[t * 64 + lane], 0, 0, 0);
+      z2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW2 + 2 * t + 1) * 64 + lane], a1c1,
+                                                      z2[t], 0, 0, 0);
+    }
+    const v8bf a2c0 = silu_chunk(z2[0], z2[1]), a2c1 = silu_chunk(z2[2], z2[3]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      z3[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW3 + 2 * t) * 64 + lane], a2c0,
+                                                      BI[(4 + t) * 64 + lane], 0, 0, 0);
+      z3[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW3 + 2 * t + 1) * 64 + lane], a2c1,
+                                                      z3[t], 0, 0, 0);
+    }
+    const v8bf a3c0 = silu_chunk(z3[0], z3[1]), a3c1 = silu_chunk(z3[2], z3[3]);
+    v4f y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[kFrW4 * 64 + lane], a3c0,
+                                                    BI[8 * 64 + lane], 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW4 + 1) * 64 + lane], a3c1, y, 0, 0, 0);
+    // loss and dL/dy (lanes q = 0 hold outputs 0..2 of particle r)
+    v4f dy = zero;
+    if (live && q == 0) {
+      const float S = fc.z, w = S * S * wnorm;
+      const float e0 = y[0] - (ft.x - fm.x) / S, e1 = y[1] - (ft.y - fm.y) / S,
+                  e2 = y[2] - (ft.z - fm.z) / S;
+      lacc += w * (e0 * e0 + e1 * e1 + e2 * e2);
+      dy[0] = 2.f * w * e0;
+      dy[1] = 2.f * w * e1;
+      dy[2] = 2.f * w * e2;
+    }
+    v8bf dyb;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) dyb[j] = (__bf16)(j < 4 ? dy[j & 3] : 0.f);
+    // backward through the layers
+    v4f d[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      d[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW4T + t) * 64 + lane], dyb, zero, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d[t][i] *= dsilu<true>(z3[t][i]);
+      gb3[t] += d[t];
+    }
+    const v8bf d3c0 = bf16_chunk(d[0], d[1]), d3c1 = bf16_chunk(d[2], d[3]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      d[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW3T + 2 * t) * 64 + lane], d3c0, zero, 0, 0, 0);
+      d[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW3T + 2 * t + 1) * 64 + lane], d3c1, d[t], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d[t][i] *= dsilu<true>(z2[t][i]);
+      gb2[t] += d[t];
+    }
+    const v8bf d2c0 = bf16_chunk(d[0], d[1]), d2c1 = bf16_chunk(d[2], d[3]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      d[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW2T + 2 * t) * 64 + lane], d2c0, zero, 0, 0, 0);
+      d[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(FR[(kFrW2T + 2 * t + 1) * 64 + lane], d2c1, d[t], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d[t][i] *= dsilu<true>(z1[t][i]);
+    }
+    const v8bf d1c0 = bf16_chunk(d[0], d[1]), d1c1 = bf16_chunk(d[2], d[3]);
+    gb4 += dy;
+    // weight gradients: K = the 16 particles, through the wave's two tiles
+    {  // W4: a3 (64 rows) x dy (outputs 0..15, 3 used)
+      tr_put_chunks(tA, a3c0, a3c1, q, r);
+      const unsigned lo = (unsigned short)bf16_bits(dy[0]) | ((unsigned)(unsigned short)bf16_bits(dy[1]) << 16);
+      const unsigned hi = (unsigned short)bf16_bits(dy[2]) | ((unsigned)(unsigned short)bf16_bits(dy[3]) << 16);
+      tr_put4(tB + r * kTR + 4 * q, make_uint2(lo, hi));
+      tile_fence();
+      const v4s b = tr_operand(tB, 0, q, r);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+        gW4[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tr_operand(tA, mt, q, r), b, gW4[mt], 0, 0, 0);
+      tile_fence();
+    }
+    {  // W3: a2 x d3
+      tr_put_chunks(tA, a2c0, a2c1, q, r);
+      tr_put_chunks(tB, d3c0, d3c1, q, r);
+      tile_fence();
+      v4s b[4];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) b[nt] = tr_operand(tB, nt, q, r);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const v4s a = tr_operand(tA, mt, q, r);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+          gW3[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nt], gW3[mt][nt], 0, 0, 0);
+      }
+      tile_fence();
+    }
+    {  // W2: a1 x d2
+      tr_put_chunks(tA, a1c0, a1c1, q, r);
+      tr_put_chunks(tB, d2c0, d2c1, q, r);
+      tile_fence();
+      v4s b[4];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) b[nt] = tr_operand(tB, nt, q, r);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const v4s a = tr_operand(tA, mt, q, r);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+          gW2[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nt], gW2[mt][nt], 0, 0, 0);
+      }
+      tile_fence();
+    }
+    {  // W1 and b1: x (inputs 0..15, slot 10 = 1) x d1
+      if (q <= 1) {
+        const uint4 xb = __builtin_bit_cast(uint4, x);
+        tr_put4(tA + r * kTR + 8 * q, make_uint2(xb.x, xb.y));
+        tr_put4(tA + r * kTR + 8 * q + 4, make_uint2(xb.z, xb.w));
+      }
+      tr_put_chunks(tB, d1c0, d1c1, q, r);
+      tile_fence();
+      const v4s a = tr_operand(tA, 0, q, r);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt)
+        gW1[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, tr_operand(tB, nt, q, r), gW1[nt], 0, 0, 0);
+      tile_fence();
+    }
+    fa = na;
+    fb = nb;
+    fc = nc;
+    fm = nm;
+    ft = nt4;
+  }
+  // the waves' accumulators into the workgroup's LDS copy
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * q + i;  // within a 16-row tile
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        atomicAdd(&G[oW2 + (16 * a + row) * kH + 16 * b + r], gW2[a][b][i]);
+        atomicAdd(&G[oW3 + (16 * a + row) * kH + 16 * b + r], gW3[a][b][i]);
+      }
+      if (row < kIn) atomicAdd(&G[oW1 + row * kH + 16 * a + r], gW1[a][i]);
+      if (row == kIn) atomicAdd(&G[ob1 + 16 * a + r], gW1[a][i]);
+      if (r < kOut) atomicAdd(&G[oW4 + (16 * a + row) * kOut + r], gW4[a][i]);
+      atomicAdd(&G[ob2 + 16 * a + row], gb2[a][i]);
+      atomicAdd(&G[ob3 + 16 * a + row], gb3[a][i]);
+    }
+  }
+  if (q == 0) {
+#pragma unroll
+    for (int i = 0; i < kOut; ++i) atomicAdd(&G[ob4 + i], gb4[i]);
+  }
+  atomicAdd(LS, lacc);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) atomicAdd(&grad[i], G[i]);
+  if (threadIdx.x == 0) atomicAdd(loss, LS[0]);
+}
+
 // Adam on fp32 parameters (bias-corrected).
 __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ prm,
                                                    const float* __restrict__ grad,
@@ -829,19 +1184,28 @@ hipError_t launch_mlp_train(const float* prm, const float4* feat, const float4* 
                             const float4* target, long long n, float wnorm, float* grad,
                             float* loss, bool bf16, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (bf16) {
+    // one workgroup of 4 waves per CU (the kernel holds its gradient
+    // accumulators in registers: one wave per SIMD), 16 particles per wave
+    // iteration
+    hipError_t e = hipFuncSetAttribute((const void*)mlp_train_reg_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTrainLds);
+    if (e != hipSuccess) return e;
+    const long long groups = (n + 15) / 16, want = (groups + 3) / 4;
+    const unsigned blocks = (unsigned)(want < 256 ? want : 256);
+    hipLaunchKernelGGL(mlp_train_reg_kernel, dim3(blocks), dim3(256), kTrainLds, s, prm, feat, mean,
+                       target, n, wnorm, grad, loss);
+    return hipGetLastError();
+  }
   const size_t lds = (size_t)(TILE * kXS + 3 * TILE * kH + TILE * 16 + 2 * TILE * kH + kParams + 1) *
                      sizeof(float);
-  const void* fn = bf16 ? (const void*)mlp_train_kernel<true> : (const void*)mlp_train_kernel<false>;
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  hipError_t e = hipFuncSetAttribute((const void*)mlp_train_kernel<false>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (e != hipSuccess) return e;
   const long long tiles = (n + TILE - 1) / TILE;
   const unsigned blocks = (unsigned)(tiles < 256 ? tiles : 256);
-  if (bf16)
-    hipLaunchKernelGGL(mlp_train_kernel<true>, dim3(blocks), dim3(256), lds, s, prm, feat, mean,
-                       target, n, wnorm, grad, loss);
-  else
-    hipLaunchKernelGGL(mlp_train_kernel<false>, dim3(blocks), dim3(256), lds, s, prm, feat, mean,
-                       target, n, wnorm, grad, loss);
+  hipLaunchKernelGGL(mlp_train_kernel<false>, dim3(blocks), dim3(256), lds, s, prm, feat, mean,
+                     target, n, wnorm, grad, loss);
   return hipGetLastError();
 }
 
