@@ -36,13 +36,21 @@ def clustered(n, seed=5, W=100.0):
     return workloads.plummer_spheres(n=n, n_spheres=4, scale=6.0, world_size=W, seed=seed)
 
 
-@pytest.mark.parametrize("compat", ["reference", "physical"])
-def test_features_match_oracle(torch_cuda, compat):
+@pytest.mark.parametrize("compat,g", [("reference", 16), ("physical", 16), ("reference", 8),
+                                      ("reference", 32), ("reference", 64)])
+def test_features_match_oracle(torch_cuda, compat, g):
+    """g = 16: both solves in one launch; g = 8, 16: cells and interpolation
+    blocks through the shifted fields; g = 32, 64: from indices.  A few
+    particles lie just outside the box (not admitted to the fast path)."""
     from mog_solver_amd.closure import closure_features
 
     torch = torch_cuda
-    k, g, W, G = 4, 16, 100.0, 100.0
+    k, W, G = 4, 100.0, 100.0
     pos, _ = clustered(4096)
+    rng = np.random.default_rng(g)
+    out = rng.choice(pos.shape[0], 64, replace=False)
+    pos[out, rng.integers(0, 3, 64)] = np.where(rng.random(64) < 0.5, rng.uniform(-3.0, -0.5, 64),
+                                                rng.uniform(W + 0.5, W + 3.0, 64)).astype(np.float32)
     s, rot, off = solver(k, g, W, G, compat=compat, seed=3)
     rho0 = 4096 / g**3
     feat, mean = closure_features(s, dev(torch, pos), rho0)
