@@ -422,13 +422,33 @@ __device__ __forceinline__ v8bf silu_chunk(const v4f& lo, const v4f& hi) {
   return b;
 }
 
+// The inference works in units of -log2(e): layer l computes
+// z' = -log2(e) z (W1, b1, b2, b3 scaled by -log2(e); W2 and W3 unchanged,
+// since each layer's input a' = z' sigmoid(z) = -log2(e) silu(z) carries the
+// same factor), so that sigmoid(z) = 1 / (1 + 2^z') needs no multiply, and
+// W4 is scaled by -ln 2 to return to the output's units.
+constexpr float kNegLog2e = -1.44269504f, kNegLn2 = -0.693147181f;
+
+__device__ __forceinline__ v8bf silu_l2_chunk(const v4f& lo, const v4f& hi) {
+  v8bf b;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    b[i] = (__bf16)(lo[i] * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(lo[i])));
+    b[4 + i] = (__bf16)(hi[i] * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(hi[i])));
+  }
+  return b;
+}
+
 __global__ __launch_bounds__(256) void mlp_forward_reg_kernel(const float* __restrict__ prm,
                                                               const float4* __restrict__ feat,
                                                               const float4* __restrict__ mean,
                                                               long long n,
                                                               float4* __restrict__ out) {
   __shared__ float P[kParams];
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) P[i] = prm[i];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) {
+    const bool w4 = i >= oW4 && i < ob4, mid = (i >= oW2 && i < ob2) || (i >= oW3 && i < ob3) || i >= ob4;
+    P[i] = mid ? prm[i] : (w4 ? kNegLn2 : kNegLog2e) * prm[i];
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
   v8bf w1[4], w2[4][2], w3[4][2], w4[2];
@@ -515,7 +535,7 @@ __global__ __launch_bounds__(256) void mlp_forward_reg_kernel(const float* __res
     for (int t = 0; t < 4; ++t)
       h[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[t], x, v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     {
-      const v8bf a0 = silu_chunk(h[0], h[1]), a1 = silu_chunk(h[2], h[3]);
+      const v8bf a0 = silu_l2_chunk(h[0], h[1]), a1 = silu_l2_chunk(h[2], h[3]);
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][0], a0, b2[t], 0, 0, 0);
@@ -523,7 +543,7 @@ __global__ __launch_bounds__(256) void mlp_forward_reg_kernel(const float* __res
       }
     }
     {
-      const v8bf a0 = silu_chunk(z[0], z[1]), a1 = silu_chunk(z[2], z[3]);
+      const v8bf a0 = silu_l2_chunk(z[0], z[1]), a1 = silu_l2_chunk(z[2], z[3]);
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         h[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][0], a0, b3[t], 0, 0, 0);
@@ -532,7 +552,7 @@ __global__ __launch_bounds__(256) void mlp_forward_reg_kernel(const float* __res
     }
     v4f y;
     {
-      const v8bf a0 = silu_chunk(h[0], h[1]), a1 = silu_chunk(h[2], h[3]);
+      const v8bf a0 = silu_l2_chunk(h[0], h[1]), a1 = silu_l2_chunk(h[2], h[3]);
       y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[0], a0, b4, 0, 0, 0);
       y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[1], a1, y, 0, 0, 0);
     }
