@@ -549,12 +549,14 @@ struct RfBatch {
 // a[kz] (in: the density spectrum) becomes the packed y + i z force
 // spectrum, xs[kz] the x force spectrum; Green's function times i k
 // (update.glsl's gradient of the potential) or the convolution solver's two
-// precomputed kernels.
+// precomputed kernels.  gz holds the line's Green's function,
+// gtab[(kz G + ky) G + kx] (unused with conv), kdz the wavenumbers kd[kz],
+// kyv and kxv kd[ky] and kd[kx], all loaded by the caller.
 template <int G>
 __device__ __forceinline__ void rf_multiply(const RfSlab& a0, int ky, int kx, float2 (&a)[G],
-                                            float2 (&xs)[G]) {
+                                            float2 (&xs)[G], const float (&gz)[G],
+                                            const float (&kdz)[G], float kyv, float kxv) {
   constexpr int GG = G * G;
-  const float kyv = a0.kd[ky], kxv = a0.kd[kx];
 #pragma unroll
   for (int kz = 0; kz < G; ++kz) {
     const int fi = (kz * G + ky) * G + kx;
@@ -564,9 +566,9 @@ __device__ __forceinline__ void rf_multiply(const RfSlab& a0, int ky, int kx, fl
       a[kz] = make_float2(fmaf(acc.x, t.x, -(acc.y * t.y)), fmaf(acc.x, t.y, acc.y * t.x));
       xs[kz] = make_float2(fmaf(acc.x, u.x, -(acc.y * u.y)), fmaf(acc.x, u.y, acc.y * u.x));
     } else {
-      const float gf = a0.gtab[fi];
+      const float gf = gz[kz];  // a0.gtab[fi], loaded by the caller
       const float pr = acc.x * gf, pi = acc.y * gf;
-      const float kzv = a0.kd[kz];
+      const float kzv = kdz[kz];
       a[kz] = make_float2(fmaf(-pr, kzv, -(pi * kyv)), fmaf(pr, kyv, -(pi * kzv)));
       xs[kz] = make_float2(-(pi * kxv), pr * kxv);
     }
@@ -594,6 +596,16 @@ __global__ __launch_bounds__(512) void rf_grid_kernel(R res, long long g0, int w
   const long long grid = a.local / G;
   const int t = threadIdx.x, half = t >> 8, hi = (t >> 4) & (G - 1), lo = t & (G - 1);
   const bool need_x = a.need_x;
+  // the Green's function of this thread's z line (ky = hi, kx = lo), loaded
+  // now so that its latency passes under the forward transforms
+  float gz[G], kdz[G];
+  if (!a.conv) {
+#pragma unroll
+    for (int kz = 0; kz < G; ++kz) gz[kz] = a.gtab[(kz * G + hi) * G + lo];
+  }
+#pragma unroll
+  for (int kz = 0; kz < G; ++kz) kdz[kz] = a.kd[kz];
+  const float kdy = a.kd[hi], kdx = a.kd[lo];
   // density -> complex
   for (int i = t; i < G * GG; i += 512)
     A[(i / GG) * PL + ((i / G) % G) * RS + (i % G)] = make_float2((float)a.mass[cell0 + i], 0.f);
@@ -625,7 +637,7 @@ __global__ __launch_bounds__(512) void rf_grid_kernel(R res, long long g0, int w
   __syncthreads();
   if (half == 0 || need_x) {
     reg_fft<G, false>(v);
-    rf_multiply<G>(a, hi, lo, v, xs);
+    rf_multiply<G>(a, hi, lo, v, xs, gz, kdz, kdy, kdx);
     float2* dst = half == 0 ? zl : B + hi * RS + lo;
     if (half == 0) {
       reg_fft<G, true>(v);
