@@ -85,6 +85,12 @@ struct mog_ctx {
   // prepared rows of the LDS deposit (centred coordinates, milli-mass)
   float4* d_cpos;
   long long cpos_cap;
+  // mog_run's particle state in visiting order (the gather's reads and
+  // writes at slot t instead of scattered over the caller's arrays)
+  float4* d_ps;
+  float4* d_vs;
+  long long state_cap;
+  const float4* g_ps;  // the state buffer the captured graphs use
 
   // graph cache of mog_run: one captured step per assignment of the two
   // density buffers (gexec deposits into g_dep, gexec_alt into the other)
@@ -418,6 +424,8 @@ void release(mog_ctx* c) {
   (void)hipFree(c->d_force_dc);
   (void)hipFree(c->d_fm);
   (void)hipFree(c->d_blk);
+  (void)hipFree(c->d_ps);
+  (void)hipFree(c->d_vs);
   (void)hipFree(c->d_diag_tw);
   (void)hipFree(c->d_diag_tab);
   (void)hipFree(c->d_diag_scratch);
@@ -535,6 +543,31 @@ float4* ensure_cpos(mog_ctx* c, int64_t n) {
   return c->d_cpos;
 }
 
+// mog_run's state buffers for n particles (outside graph capture); false
+// when allocation fails (the run then gathers through the caller's arrays).
+bool ensure_state(mog_ctx* c, int64_t n) {
+  if (c->state_cap >= n && c->d_ps && c->d_vs) return true;
+  (void)hipFree(c->d_ps);
+  (void)hipFree(c->d_vs);
+  c->d_ps = c->d_vs = nullptr;
+  c->state_cap = 0;
+  if (hipMalloc((void**)&c->d_ps, (size_t)n * sizeof(float4)) != hipSuccess ||
+      hipMalloc((void**)&c->d_vs, (size_t)n * sizeof(float4)) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(c->d_ps);
+    (void)hipFree(c->d_vs);
+    c->d_ps = c->d_vs = nullptr;
+    return false;
+  }
+  c->state_cap = n;
+  return true;
+}
+
+// Whether refresh_order would rebuild the order for (pos, n).
+bool order_stale(const mog_ctx* c, const float* pos, int64_t n) {
+  return c->order_pos != pos || c->order_n != n || c->steps_since_order >= c->resort_every;
+}
+
 // Rebuild the order when it is stale for (pos, n); returns the order to use
 // (NULL: storage order).
 hipError_t refresh_order(mog_ctx* c, const float* pos, int64_t n, hipStream_t s, const int** out) {
@@ -578,7 +611,7 @@ void drop_graphs(mog_ctx* ctx) {
 // once the step is launched, not when it is only captured.
 hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const int* order,
                         long long* dep, long long* old, hipStream_t s, bool allow_sorted = true,
-                        float4* cpos = nullptr) {
+                        float4* cpos = nullptr, float4* ps = nullptr, float4* vs = nullptr) {
   const Params& P = ctx->P;
   const long long cells = (long long)P.k * P.cells;
   if (allow_sorted && order && ctx->sorted_deposit && ctx->d_spos && ctx->d_cells) {
@@ -609,7 +642,8 @@ hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const i
   e = solve_dispatch(ctx, dep, ctx->d_force, 1, s);
   if (e != hipSuccess) return e;
   return launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
-                              ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s, old, cells);
+                              ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s, old, cells,
+                              order ? ps : nullptr, order ? vs : nullptr);
 }
 
 // Which visiting order the no-allocation scratch of each device holds
@@ -1011,25 +1045,67 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
   hipError_t e = clean_spare(ctx, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "mog_run");
   float4* const cpos = ensure_cpos(ctx, n);
+  // With a visiting order and the plain deposit, the run keeps the particle
+  // state in visiting order (d_ps, d_vs): imported at the start and after
+  // each rebuild of the order, the gather reads and writes it at slot t and
+  // writes the positions back to the caller's array (the deposit's input);
+  // the velocities go back at each rebuild and at the end.  The step is the
+  // same arithmetic on the same values, so the results are those of the
+  // gather through the caller's arrays.
+  const int* const order_buf_before = ctx->d_order;  // ensure_order may reallocate it
+  const bool ordered = ensure_order(ctx, n);
+  const bool state = ordered && !(ctx->sorted_deposit && ctx->d_spos && ctx->d_cells) &&
+                     ensure_state(ctx, n);
+  float4* const ps = state ? ctx->d_ps : nullptr;
+  float4* const vs = state ? ctx->d_vs : nullptr;
+  bool imported = false;
+  // the order for step i (rebuilt when stale), with the state moved around
+  // the rebuild
+  auto prepare = [&](int i, const int** order) -> hipError_t {
+    *order = nullptr;
+    if (!ordered) return hipSuccess;
+    hipError_t ee = hipSuccess;
+    const bool rebuild = order_stale(ctx, pos, n);
+    if (state && imported && rebuild) {
+      ee = launch_state_export(vs, ctx->d_order, n, (float4*)vel, s);
+      if (ee != hipSuccess) return ee;
+      imported = false;
+    }
+    if ((ee = refresh_order(ctx, pos, n, s, order)) != hipSuccess) return ee;
+    if (state && (i == 0 || rebuild)) {
+      if ((ee = launch_state_import((const float4*)pos, (const float4*)vel, *order, n, ps, vs, s)) !=
+          hipSuccess)
+        return ee;
+      imported = true;
+    }
+    return hipSuccess;
+  };
+  auto finish = [&](hipError_t ee, const char* what) -> int {
+    if (state && imported) {
+      const hipError_t ex = launch_state_export(vs, ctx->d_order, n, (float4*)vel, s);
+      if (ee == hipSuccess) ee = ex;
+    }
+    if (ee != hipSuccess) {
+      ctx->spare_dirty = true;
+      return hip_fail(ctx, ee, what);
+    }
+    return MOG_OK;
+  };
   if (!use_graph) {
     for (int i = 0; i < nsteps; ++i) {
       const int* order = nullptr;
-      if ((e = refresh_order(ctx, pos, n, s, &order)) != hipSuccess ||
+      if ((e = prepare(i, &order)) != hipSuccess ||
           (e = enqueue_step(ctx, pos, vel, n, order, ctx->d_grids_next, ctx->d_grids, s, true,
-                            cpos)) != hipSuccess) {
-        ctx->spare_dirty = true;
-        return hip_fail(ctx, e, "mog_run");
-      }
+                            cpos, ps, vs)) != hipSuccess)
+        return finish(e, "mog_run");
       swap_grids(ctx);
     }
-    return MOG_OK;
+    return finish(hipSuccess, "mog_run");
   }
   // The captured step reads ctx->d_order; the order itself is rebuilt by
   // direct launches between graph replays (the sort is not captured).
-  const int* const order_buf_before = ctx->d_order;
-  const bool ordered = ensure_order(ctx, n);
   if (ctx->gexec && (ctx->d_order != order_buf_before || ordered != ctx->g_ordered ||
-                     ctx->g_cpos != cpos))
+                     ctx->g_cpos != cpos || ctx->g_ps != ps))
     drop_graphs(ctx);
   if (!ctx->gexec || ctx->g_pos != pos || ctx->g_vel != vel || ctx->g_n != n) {
     drop_graphs(ctx);
@@ -1047,7 +1123,7 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
         return hip_fail(ctx, e, "hipStreamBeginCapture");
       }
       hipError_t ee = enqueue_step(ctx, pos, vel, n, ordered ? ctx->d_order : nullptr, bufs[a],
-                                   bufs[1 - a], ctx->cap_stream, true, cpos);
+                                   bufs[1 - a], ctx->cap_stream, true, cpos, ps, vs);
       hipGraph_t graph = nullptr;
       e = hipStreamEndCapture(ctx->cap_stream, &graph);
       if (ee != hipSuccess || e != hipSuccess) {
@@ -1069,20 +1145,16 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
     ctx->g_n = n;
     ctx->g_ordered = ordered;
     ctx->g_cpos = cpos;
+    ctx->g_ps = ps;
   }
   for (int i = 0; i < nsteps; ++i) {
-    if (ordered) {
-      const int* order = nullptr;
-      if ((e = refresh_order(ctx, pos, n, s, &order)) != hipSuccess) return hip_fail(ctx, e, "mog_run");
-    }
+    const int* order = nullptr;
+    if ((e = prepare(i, &order)) != hipSuccess) return finish(e, "mog_run");
     hipGraphExec_t gx = ctx->d_grids_next == ctx->g_dep ? ctx->gexec : ctx->gexec_alt;
-    if ((e = hipGraphLaunch(gx, s)) != hipSuccess) {
-      ctx->spare_dirty = true;
-      return hip_fail(ctx, e, "hipGraphLaunch");
-    }
+    if ((e = hipGraphLaunch(gx, s)) != hipSuccess) return finish(e, "hipGraphLaunch");
     swap_grids(ctx);
   }
-  return MOG_OK;
+  return finish(hipSuccess, "mog_run");
 }
 
 int mog_step_noalloc(mog_ctx* ctx, float* pos, float* vel, int64_t n, void* stream) {
@@ -1161,7 +1233,14 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
   hipError_t e = hipSuccess;
   const bool sorted = order && ctx->sorted_deposit && ctx->d_spos && ctx->d_cells;
   const bool cache = sorted && ctx->cell_cache;
+  // the gather of mog_run (state in visiting order), imported and exported
+  // outside the timed phases
+  const bool state = order && !sorted && ensure_state(ctx, n);
+  float4* const ps = state ? ctx->d_ps : nullptr;
+  float4* const vs = state ? ctx->d_vs : nullptr;
   for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+  if (e == hipSuccess && state)
+    e = launch_state_import((const float4*)pos, (const float4*)vel, order, n, ps, vs, s);
   if (e == hipSuccess) e = hipEventRecord(ev[0], s);
   // the step's own sequence (its clear phase is empty: the grids it deposits
   // into were cleared by the previous step's gather)
@@ -1184,10 +1263,12 @@ int mog_step_profiled(mog_ctx* ctx, float* pos, float* vel, int64_t n, float* ph
       if (e == hipSuccess) e = launch_fill_zero(old, cells, s);
     } else {
       e = launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
-                               ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s, old, cells);
+                               ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s, old, cells,
+                               ps, vs);
     }
   }
   if (e == hipSuccess) e = hipEventRecord(ev[4], s);
+  if (e == hipSuccess && state) e = launch_state_export(vs, order, n, (float4*)vel, s);
   if (e == hipSuccess) swap_grids(ctx);
   if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
   for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventElapsedTime(&phase_ms[i], ev[i], ev[i + 1]);

@@ -13,15 +13,19 @@ namespace mog {
 
 // mode 0: update pos/vel in place; mode 1: write avgForce to avg_out.
 // `order` (optional) is the visiting order: thread t handles particle
-// order[t], so that the lanes of a wave are neighbours in space.  LDS3: the
-// fast rows' addends staged in LDS (when k * 12 bytes fit the launch's LDS
-// budget); the lanes' records of deferred pairs follow them.
+// order[t], so that the lanes of a wave are neighbours in space.  ps, vs
+// (optional, mode 0 with an order): the state in visiting order, read and
+// written at slot t; the new position also goes to pos[order[t]], vel is
+// not touched.  LDS3: the fast rows' addends staged in LDS (when k * 12
+// bytes fit the launch's LDS budget); the lanes' records of deferred pairs
+// follow them.
 template <bool WORLD, bool LDS3, bool SH>
 __global__ __launch_bounds__(256) void gather_update_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, long long n,
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
     const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode,
-    const int* __restrict__ order, long long* __restrict__ zero, long long zero_count) {
+    const int* __restrict__ order, long long* __restrict__ zero, long long zero_count,
+    float4* __restrict__ ps, float4* __restrict__ vs) {
   extern __shared__ float adds[];  // LDS3: adds[3 g + m] = frot[12 g + 4 m + 3]; then the records
   unsigned* const recs = reinterpret_cast<unsigned*>(adds + (LDS3 ? 3 * P.k : 0));
   // side job: clear `zero` (the density grids the next step deposits into)
@@ -36,7 +40,7 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
   const long long t = (long long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const long long i = order ? (long long)order[t] : t;
-  float4 p = pos[i];
+  float4 p = ps ? ps[t] : pos[i];
   const float c0 = p.x - P.half;
   const float c1 = p.y - P.half;
   const float c2 = p.z - P.half;
@@ -49,10 +53,51 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
     avg_out[i] = make_float4(ax, ay, az, 0.0f);
     return;
   }
-  float4 v = vel[i];
+  float4 v = vs ? vs[t] : vel[i];
   integrate(p, v, ax, ay, az, P);
   pos[i] = p;
-  vel[i] = v;
+  if (ps) {
+    ps[t] = p;
+    vs[t] = v;
+  } else {
+    vel[i] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void state_import_kernel(const float4* __restrict__ pos,
+                                                           const float4* __restrict__ vel,
+                                                           const int* __restrict__ order,
+                                                           long long n, float4* __restrict__ ps,
+                                                           float4* __restrict__ vs) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long long i = order[t];
+  ps[t] = pos[i];
+  vs[t] = vel[i];
+}
+
+__global__ __launch_bounds__(256) void state_export_kernel(const float4* __restrict__ vs,
+                                                           const int* __restrict__ order,
+                                                           long long n, float4* __restrict__ vel) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  vel[order[t]] = vs[t];
+}
+
+hipError_t launch_state_import(const float4* pos, const float4* vel, const int* order, long long n,
+                               float4* ps, float4* vs, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(state_import_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos,
+                     vel, order, n, ps, vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_state_export(const float4* vs, const int* order, long long n, float4* vel,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(state_export_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vs,
+                     order, n, vel);
+  return hipGetLastError();
 }
 
 // Gather + update from the cell cache of the step's sorted deposit: slot t
@@ -150,8 +195,9 @@ hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long 
                                 const float* rot, const float* off, const float* frot,
                                 const float4* force, int world, float4* avg_out, int mode,
                                 const int* order, hipStream_t s, long long* zero,
-                                long long zero_count) {
+                                long long zero_count, float4* ps, float4* vs) {
   if (n <= 0) return zero_count > 0 ? launch_fill_zero(zero, zero_count, s) : hipSuccess;
+  if ((ps || vs) && !(ps && vs && order && mode == 0)) return hipErrorInvalidValue;
   const long long blocks = (n + 255) / 256;
   const size_t lds = (size_t)P.k * 3 * sizeof(float);
   const bool lds3 = lds <= 16 * 1024;  // k <= 1365: a few KB per workgroup
@@ -161,7 +207,7 @@ hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long 
 #define MOG_GATHER(W3, L3, S3)                                                             \
   hipLaunchKernelGGL((gather_update_kernel<W3, L3, S3>), grid, dim3(256),                 \
                      (L3 ? lds : 0) + lds_recs, s, pos, vel, n, rot, off, frot, force, P, \
-                     avg_out, mode, order, zero, zero_count)
+                     avg_out, mode, order, zero, zero_count, ps, vs)
   if (world && lds3 && sh) MOG_GATHER(true, true, true);
   else if (world && lds3) MOG_GATHER(true, true, false);
   else if (world && sh) MOG_GATHER(true, false, true);

@@ -196,11 +196,23 @@ hipError_t launch_solve_mfma(const Params& P, int precision, const long long* gr
                              const float* rot, int world, hipStream_t s);
 // zero != NULL: the launch also clears zero[0, zero_count) (the step's
 // spare density grids), spread over all of its lanes.
+// ps, vs (optional, with order): the particle state in visiting order,
+// ps[t] = pos[order[t]], vs[t] = vel[order[t]]; the gather then reads and
+// writes them at slot t (coalesced) and writes only the positions back to
+// pos[order[t]] (the deposit's input), leaving vel stale until
+// launch_state_export.
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
                                 const float* rot, const float* off, const float* frot,
                                 const float4* force, int world, float4* avg_out, int mode,
                                 const int* order, hipStream_t s, long long* zero = nullptr,
-                                long long zero_count = 0);
+                                long long zero_count = 0, float4* ps = nullptr,
+                                float4* vs = nullptr);
+// ps[t] = pos[order[t]], vs[t] = vel[order[t]] for t < n.
+hipError_t launch_state_import(const float4* pos, const float4* vel, const int* order, long long n,
+                               float4* ps, float4* vs, hipStream_t s);
+// vel[order[t]] = vs[t] for t < n.
+hipError_t launch_state_export(const float4* vs, const int* order, long long n, float4* vel,
+                               hipStream_t s);
 // Morton visiting order of the particles (order.hip): temp storage size of
 // the radix sort, and the order itself (keys_a/keys_b/idx_a: n-element
 // scratch; order: n-element output).

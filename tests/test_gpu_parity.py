@@ -350,6 +350,31 @@ def test_step_matches_phases_and_graph(torch_cuda, k, g):
         assert torch.equal(p3, p4) and torch.equal(v3, v4)
 
 
+@pytest.mark.parametrize("resort", [1, 2, 5])
+def test_run_state_across_reorders(torch_cuda, resort):
+    """mog_run keeps the particle state in visiting order (imported at the
+    start and after each rebuild of the order, velocities written back at
+    each rebuild and at the end): bit-identical to single steps through the
+    caller's arrays, with the order rebuilt inside a run, eager and graph,
+    across successive calls."""
+    torch = torch_cuda
+    k, g, W = 12, 16, 100.0
+    pos, vel = workloads.two_disks(80000, W, 100.0, g, seed=8)
+    s, rot, off = make_solver(k, g, W)
+    s.set_tuning(**{**s.get_tuning(), "resort_every": resort, "sorted_deposit": 0, "cell_cache": 0})
+    p1, v1 = dev(torch, pos), dev(torch, vel)
+    for _ in range(7):
+        s.step(p1, v1)
+    p2, v2 = dev(torch, pos), dev(torch, vel)
+    s.run(p2, v2, 7, use_graph=True)
+    p3, v3 = dev(torch, pos), dev(torch, vel)
+    s.run(p3, v3, 3, use_graph=False)
+    s.run(p3, v3, 4, use_graph=True)
+    torch.cuda.synchronize()
+    assert torch.equal(p1, p2) and torch.equal(v1, v2)
+    assert torch.equal(p1, p3) and torch.equal(v1, v3)
+
+
 def test_particle_count_changes_between_steps(torch_cuda):
     """The per-step cell cache grows with n; captured graphs are rebuilt."""
     torch = torch_cuda
