@@ -737,12 +737,16 @@ struct mog_batch {
   const void* order_pos;
   int steps_since_order;
   int resort_every;
+  // mog_batch_run's state in visiting order (allocated by its first run)
+  float4* d_ps;
+  float4* d_vs;
   long long deposit_chunk;
   int deposit_threads;
   hipStream_t cap_stream;
   hipGraphExec_t gexec;
   const void* g_pos;
   const void* g_vel;
+  const void* g_ps;  // the state buffer the captured graph uses
 };
 
 struct mog2d_ctx {
@@ -775,7 +779,8 @@ void batch_release(mog_batch* b) {
   void* ptrs[] = {b->d_sims, b->d_rot,  b->d_off, b->d_frot, b->d_grids, b->d_S1,
                   b->d_S2,   b->d_force, b->d_tw, b->d_gtab, b->d_kd,    b->d_conv,
                   b->d_tasks, b->d_counter, b->d_order, b->d_idx, b->d_keys_a, b->d_keys_b,
-                  b->d_sort_tmp, b->d_cls_sim, b->d_cls_off, b->d_win_soff, b->d_cpos};
+                  b->d_sort_tmp, b->d_cls_sim, b->d_cls_off, b->d_win_soff, b->d_cpos,
+                  b->d_ps, b->d_vs};
   for (void* q : ptrs) (void)hipFree(q);
 }
 
@@ -843,7 +848,8 @@ const int* batch_order(const mog_batch* b) {
   return (b->d_order && b->resort_every > 0) ? b->d_order : nullptr;
 }
 
-hipError_t batch_enqueue(mog_batch* b, float* pos, float* vel, hipStream_t s) {
+hipError_t batch_enqueue(mog_batch* b, float* pos, float* vel, hipStream_t s,
+                         float4* ps = nullptr, float4* vs = nullptr) {
   hipError_t e = launch_fill_zero(b->d_grids, b->cells_total, s);
   if (e == hipSuccess) e = launch_fill_zero((long long*)b->d_counter, 1, s);
   if (e == hipSuccess)
@@ -869,8 +875,24 @@ hipError_t batch_enqueue(mog_batch* b, float* pos, float* vel, hipStream_t s) {
   }
   if (e == hipSuccess)
     e = launch_batch_gather((float4*)pos, (float4*)vel, b->gather_blocks, b->d_sims, b->nsims,
-                            b->kmax, b->d_rot, b->d_off, b->d_frot, b->d_force, batch_order(b), s);
+                            b->kmax, b->d_rot, b->d_off, b->d_frot, b->d_force, batch_order(b), s,
+                            batch_order(b) ? ps : nullptr, batch_order(b) ? vs : nullptr);
   return e;
+}
+
+// mog_batch_run's state buffers (outside capture); false when allocation
+// fails (the run then gathers through the caller's arrays).
+bool batch_ensure_state(mog_batch* b) {
+  if (b->d_ps && b->d_vs) return true;
+  if (hipMalloc((void**)&b->d_ps, (size_t)b->ntotal * sizeof(float4)) != hipSuccess ||
+      hipMalloc((void**)&b->d_vs, (size_t)b->ntotal * sizeof(float4)) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(b->d_ps);
+    (void)hipFree(b->d_vs);
+    b->d_ps = b->d_vs = nullptr;
+    return false;
+  }
+  return true;
 }
 
 int batch_check(mog_batch* b, const float* pos, const float* vel) {
@@ -1851,21 +1873,52 @@ int mog_batch_run(mog_batch* b, float* pos, float* vel, int32_t nsteps, int32_t 
   if (b->ntotal == 0 || nsteps == 0) return MOG_OK;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
+  // the state in visiting order, as in mog_run: imported at the start and
+  // after each rebuild of the order, velocities written back at each
+  // rebuild and at the end
+  const bool state = batch_order(b) && batch_ensure_state(b);
+  float4* const ps = state ? b->d_ps : nullptr;
+  float4* const vs = state ? b->d_vs : nullptr;
+  bool imported = false;
+  auto prepare = [&](int i) -> hipError_t {
+    hipError_t ee = hipSuccess;
+    const bool rebuild = b->order_pos != pos || b->steps_since_order >= b->resort_every;
+    if (state && imported && rebuild) {
+      if ((ee = launch_state_export(vs, b->d_order, b->ntotal, (float4*)vel, s)) != hipSuccess)
+        return ee;
+      imported = false;
+    }
+    if ((ee = batch_refresh_order(b, pos, s)) != hipSuccess) return ee;
+    if (state && (i == 0 || rebuild)) {
+      if ((ee = launch_state_import((const float4*)pos, (const float4*)vel, b->d_order, b->ntotal,
+                                    ps, vs, s)) != hipSuccess)
+        return ee;
+      imported = true;
+    }
+    return hipSuccess;
+  };
+  auto finish = [&](hipError_t ee, const char* what) -> int {
+    if (state && imported) {
+      const hipError_t ex = launch_state_export(vs, b->d_order, b->ntotal, (float4*)vel, s);
+      if (ee == hipSuccess) ee = ex;
+    }
+    if (ee != hipSuccess) return fail(nullptr, MOG_EHIP, "%s: %s", what, hipGetErrorString(ee));
+    return MOG_OK;
+  };
   if (!use_graph) {
     for (int i = 0; i < nsteps; ++i)
-      if ((e = batch_refresh_order(b, pos, s)) != hipSuccess ||
-          (e = batch_enqueue(b, pos, vel, s)) != hipSuccess)
-        return fail(nullptr, MOG_EHIP, "mog_batch_run: %s", hipGetErrorString(e));
-    return MOG_OK;
+      if ((e = prepare(i)) != hipSuccess || (e = batch_enqueue(b, pos, vel, s, ps, vs)) != hipSuccess)
+        return finish(e, "mog_batch_run");
+    return finish(hipSuccess, "mog_batch_run");
   }
-  if (!b->gexec || b->g_pos != pos || b->g_vel != vel) {
+  if (!b->gexec || b->g_pos != pos || b->g_vel != vel || b->g_ps != ps) {
     batch_drop_graph(b);
     if (!b->cap_stream &&
         (e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking)) != hipSuccess)
       return fail(nullptr, MOG_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
     if ((e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return fail(nullptr, MOG_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
-    const hipError_t ee = batch_enqueue(b, pos, vel, b->cap_stream);
+    const hipError_t ee = batch_enqueue(b, pos, vel, b->cap_stream, ps, vs);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(b->cap_stream, &graph);
     if (ee != hipSuccess) {
@@ -1881,14 +1934,13 @@ int mog_batch_run(mog_batch* b, float* pos, float* vel, int32_t nsteps, int32_t 
     }
     b->g_pos = pos;
     b->g_vel = vel;
+    b->g_ps = ps;
   }
   for (int i = 0; i < nsteps; ++i) {
-    if ((e = batch_refresh_order(b, pos, s)) != hipSuccess)
-      return fail(nullptr, MOG_EHIP, "mog_batch_run: %s", hipGetErrorString(e));
-    if ((e = hipGraphLaunch(b->gexec, s)) != hipSuccess)
-      return fail(nullptr, MOG_EHIP, "hipGraphLaunch: %s", hipGetErrorString(e));
+    if ((e = prepare(i)) != hipSuccess) return finish(e, "mog_batch_run");
+    if ((e = hipGraphLaunch(b->gexec, s)) != hipSuccess) return finish(e, "hipGraphLaunch");
   }
-  return MOG_OK;
+  return finish(hipSuccess, "mog_batch_run");
 }
 
 int mog_batch_set_tuning(mog_batch* b, int64_t deposit_chunk, int32_t deposit_threads) {

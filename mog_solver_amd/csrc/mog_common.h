@@ -379,8 +379,10 @@ constexpr long long kBatchWindowCells = 1ll << 22;
 // Grid-size classes whose solve is one launch over all of the class's
 // grids (one workgroup per grid, no windows).
 inline bool batch_class_fused(int gmax) { return gmax == 16 || gmax == 32; }
+// ps, vs (optional, with order): the ensemble's state in visiting order,
+// as for launch_gather_update.
 hipError_t launch_batch_gather(float4* pos, float4* vel, long long blocks, const BatchSim* sims,
                                int nsims, int kmax, const float* rot, const float* off,
                                const float* frot, const float4* force, const int* order,
-                               hipStream_t s);
+                               hipStream_t s, float4* ps = nullptr, float4* vs = nullptr);
 }  // namespace mog

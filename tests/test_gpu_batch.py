@@ -93,6 +93,25 @@ def test_batch_graph_and_tuning_variants_agree(torch_cuda):
         assert torch.equal(p, ref_p) and torch.equal(v, ref_v), (chunk, threads, graph)
 
 
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_run_state_across_reorders(torch_cuda, graph):
+    """A batch run keeps the ensemble's state in visiting order across a
+    rebuild of the order (every 16 steps) and writes it back at the end:
+    the same bits as single batch steps through the caller's arrays."""
+    from mog_solver_amd.batch import MogBatch
+
+    torch = torch_cuda
+    cfgs, sts = configs(), states()
+    batch = MogBatch(cfgs, [p.shape[0] for p, _ in sts], seeds=list(range(20, 20 + len(cfgs))))
+    p0, v0 = batch.pack(sts)
+    ref_p, ref_v = p0.clone(), v0.clone()
+    for _ in range(18):
+        batch.step(ref_p, ref_v)
+    p, v = p0.clone(), v0.clone()
+    batch.run(p, v, 18, use_graph=graph)
+    assert torch.equal(p, ref_p) and torch.equal(v, ref_v)
+
+
 def test_batch_rejects_unsupported(torch_cuda):
     from mog_solver_amd._abi import MogError
     from mog_solver_amd.batch import MogBatch
