@@ -563,6 +563,11 @@ bool ensure_state(mog_ctx* c, int64_t n) {
   return true;
 }
 
+// Runs of at least this many steps keep their state in visiting order: the
+// import and the export cost about two gathers' worth of scattered
+// accesses, which a step through the copies saves about once.
+constexpr int kStateMinSteps = 4;
+
 // Whether refresh_order would rebuild the order for (pos, n).
 bool order_stale(const mog_ctx* c, const float* pos, int64_t n) {
   return c->order_pos != pos || c->order_n != n || c->steps_since_order >= c->resort_every;
@@ -1067,8 +1072,9 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
   hipError_t e = clean_spare(ctx, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "mog_run");
   float4* const cpos = ensure_cpos(ctx, n);
-  // With a visiting order and the plain deposit, the run keeps the particle
-  // state in visiting order (d_ps, d_vs): imported at the start and after
+  // With a visiting order, the plain deposit and at least kStateMinSteps
+  // steps, the run keeps the particle state in visiting order (d_ps, d_vs):
+  // imported at the start and after
   // each rebuild of the order, the gather reads and writes it at slot t and
   // writes the positions back to the caller's array (the deposit's input);
   // the velocities go back at each rebuild and at the end.  The step is the
@@ -1076,8 +1082,8 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
   // gather through the caller's arrays.
   const int* const order_buf_before = ctx->d_order;  // ensure_order may reallocate it
   const bool ordered = ensure_order(ctx, n);
-  const bool state = ordered && !(ctx->sorted_deposit && ctx->d_spos && ctx->d_cells) &&
-                     ensure_state(ctx, n);
+  const bool state = nsteps >= kStateMinSteps && ordered &&
+                     !(ctx->sorted_deposit && ctx->d_spos && ctx->d_cells) && ensure_state(ctx, n);
   float4* const ps = state ? ctx->d_ps : nullptr;
   float4* const vs = state ? ctx->d_vs : nullptr;
   bool imported = false;
@@ -1876,7 +1882,7 @@ int mog_batch_run(mog_batch* b, float* pos, float* vel, int32_t nsteps, int32_t 
   // the state in visiting order, as in mog_run: imported at the start and
   // after each rebuild of the order, velocities written back at each
   // rebuild and at the end
-  const bool state = batch_order(b) && batch_ensure_state(b);
+  const bool state = nsteps >= kStateMinSteps && batch_order(b) && batch_ensure_state(b);
   float4* const ps = state ? b->d_ps : nullptr;
   float4* const vs = state ? b->d_vs : nullptr;
   bool imported = false;
