@@ -48,21 +48,21 @@ FP32_VECTOR_PEAK_TFS = 157.3
 # prepass plus the LDS kernel.
 PMC_TRAFFIC_KIB = {
     ("two_disks", 1_000_000, 240, 16, "fp32"): {
-        ("deposit", 2, 2048): (7_824.1 + 10_738.8, 15_625.0 + 34_463.9),
-        "gather_update": (122_836.4, 73_900.3),
-        "solve": (3_739.8, 14_242.9),
+        ("deposit", 2, 2048): (7_818.1 + 10_730.2, 15_625.0 + 34_463.9),
+        "gather_update": (44_247.9, 72_998.5),
+        "solve": (3_734.8, 14_242.9),
     },
     ("plummer", 4_000_000, 240, 32, "fp32"): {
-        ("deposit", 1, 4096): (31_262.9 + 67_088.7, 62_500.0 + 681_771.4),
-        "gather_update": (730_602.2, 312_880.0),
-        "solve": (69_804.9, 190_845.6),
+        ("deposit", 1, 4096): (31_256.8 + 67_334.1, 62_500.0 + 681_771.4),
+        "gather_update": (544_925.3, 313_054.0),
+        "solve": (69_800.9, 190_846.7),
     },
 }
 # Vector instructions per wave-wide particle-grid pair (SQ_INSTS_VALU / 3.75e6
 # for the kernels of the same tiling, profiles/sq_counters_two_disks_n1e6_k240_g16.md):
 # the deposit and the gather are bound by vector-instruction issue.
 SQ_VALU_PER_WAVE_PAIR = {
-    ("two_disks", 1_000_000, 240, 16, "fp32"): {("deposit", 2, 2048): 28.53, "gather_update": 28.33},
+    ("two_disks", 1_000_000, 240, 16, "fp32"): {("deposit", 2, 2048): 28.53, "gather_update": 27.88},
 }
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
