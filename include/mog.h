@@ -105,7 +105,11 @@ int mog_set_transforms(mog_ctx* ctx, const float* rot_k12, const float* off_k4);
 int mog_step(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, void* stream);
 
 /* `nsteps` consecutive steps; with use_graph != 0 one step is captured in a
- * hipGraph on first use for these (pos, vel, n, stream) and replayed. */
+ * hipGraph on first use for these (pos, vel, n, stream) and replayed.  From
+ * 4 steps on, the run works on a copy of the state in the gather's visiting
+ * order (the context's own buffers): pos stays current after every step,
+ * vel is written back at each rebuild of the order and before the call
+ * returns (stream-ordered), with the same bits as nsteps mog_step calls. */
 int mog_run(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, int32_t nsteps,
             int32_t use_graph, void* stream);
 
