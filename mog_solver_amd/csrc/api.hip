@@ -91,6 +91,7 @@ struct mog_ctx {
   float4* d_vs;
   long long state_cap;
   const float4* g_ps;  // the state buffer the captured graphs use
+  unsigned int g_prep;  // their gather writes the prepared rows (limit; 0: positions)
 
   // graph cache of mog_run: one captured step per assignment of the two
   // density buffers (gexec deposits into g_dep, gexec_alt into the other)
@@ -616,7 +617,12 @@ void drop_graphs(mog_ctx* ctx) {
 // once the step is launched, not when it is only captured.
 hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const int* order,
                         long long* dep, long long* old, hipStream_t s, bool allow_sorted = true,
-                        float4* cpos = nullptr, float4* ps = nullptr, float4* vs = nullptr) {
+                        float4* cpos = nullptr, float4* ps = nullptr, float4* vs = nullptr,
+                        unsigned int* gather_prep = nullptr) {
+  // gather_prep (with ps, vs): when the deposit reads prepared rows, it runs
+  // no prepass and the gather writes the next step's rows instead of the
+  // positions; *gather_prep is set to their milli-mass limit (0: not so)
+  if (gather_prep) *gather_prep = 0;
   const Params& P = ctx->P;
   const long long cells = (long long)P.k * P.cells;
   if (allow_sorted && order && ctx->sorted_deposit && ctx->d_spos && ctx->d_cells) {
@@ -641,14 +647,21 @@ hipError_t enqueue_step(mog_ctx* ctx, float* pos, float* vel, int64_t n, const i
     }
     return e;
   }
+  const bool prep_by_gather = gather_prep && order && ps && vs && cpos;
+  DepositTune dt = ctx->tune;
+  dt.cpos_ready = prep_by_gather;
+  unsigned int lim = 0;
   hipError_t e = launch_deposit(P, (const float4*)pos, n, ctx->d_rot, ctx->d_off, ctx->d_frot,
-                                (unsigned long long*)dep, ctx->tune, s, cpos);
+                                (unsigned long long*)dep, dt, s, cpos, &lim);
   if (e != hipSuccess) return e;
   e = solve_dispatch(ctx, dep, ctx->d_force, 1, s);
   if (e != hipSuccess) return e;
+  const bool prepped = prep_by_gather && lim != 0;
+  if (prepped) *gather_prep = lim;
   return launch_gather_update(P, (float4*)pos, (float4*)vel, n, ctx->d_rot, ctx->d_off,
                               ctx->d_frot, ctx->d_force, 1, nullptr, 0, order, s, old, cells,
-                              order ? ps : nullptr, order ? vs : nullptr);
+                              order ? ps : nullptr, order ? vs : nullptr,
+                              prepped ? cpos : nullptr, prepped ? lim : 0u);
 }
 
 // Which visiting order the no-allocation scratch of each device holds
@@ -1087,6 +1100,10 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
   float4* const ps = state ? ctx->d_ps : nullptr;
   float4* const vs = state ? ctx->d_vs : nullptr;
   bool imported = false;
+  // graph replays whose gather writes the prepared rows (prep != 0) leave
+  // pos stale as well until the export, and need the rows of the caller's
+  // positions before the first replay
+  unsigned int prep = 0;
   // the order for step i (rebuilt when stale), with the state moved around
   // the rebuild
   auto prepare = [&](int i, const int** order) -> hipError_t {
@@ -1095,7 +1112,8 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
     hipError_t ee = hipSuccess;
     const bool rebuild = order_stale(ctx, pos, n);
     if (state && imported && rebuild) {
-      ee = launch_state_export(vs, ctx->d_order, n, (float4*)vel, s);
+      ee = launch_state_export(vs, ctx->d_order, n, (float4*)vel, s, prep ? ps : nullptr,
+                               prep ? (float4*)pos : nullptr);
       if (ee != hipSuccess) return ee;
       imported = false;
     }
@@ -1104,13 +1122,17 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
       if ((ee = launch_state_import((const float4*)pos, (const float4*)vel, *order, n, ps, vs, s)) !=
           hipSuccess)
         return ee;
+      if (i == 0 && prep &&
+          (ee = launch_deposit_prep(ctx->P, (const float4*)pos, n, prep, cpos, s)) != hipSuccess)
+        return ee;
       imported = true;
     }
     return hipSuccess;
   };
   auto finish = [&](hipError_t ee, const char* what) -> int {
     if (state && imported) {
-      const hipError_t ex = launch_state_export(vs, ctx->d_order, n, (float4*)vel, s);
+      const hipError_t ex = launch_state_export(vs, ctx->d_order, n, (float4*)vel, s,
+                                                prep ? ps : nullptr, prep ? (float4*)pos : nullptr);
       if (ee == hipSuccess) ee = ex;
     }
     if (ee != hipSuccess) {
@@ -1150,8 +1172,11 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
         drop_graphs(ctx);
         return hip_fail(ctx, e, "hipStreamBeginCapture");
       }
+      unsigned int gp = 0;
       hipError_t ee = enqueue_step(ctx, pos, vel, n, ordered ? ctx->d_order : nullptr, bufs[a],
-                                   bufs[1 - a], ctx->cap_stream, true, cpos, ps, vs);
+                                   bufs[1 - a], ctx->cap_stream, true, cpos, ps, vs,
+                                   state ? &gp : nullptr);
+      ctx->g_prep = gp;
       hipGraph_t graph = nullptr;
       e = hipStreamEndCapture(ctx->cap_stream, &graph);
       if (ee != hipSuccess || e != hipSuccess) {
@@ -1175,6 +1200,7 @@ int mog_run(mog_ctx* ctx, float* pos, float* vel, int64_t n, int32_t nsteps, int
     ctx->g_cpos = cpos;
     ctx->g_ps = ps;
   }
+  prep = state ? ctx->g_prep : 0;
   for (int i = 0; i < nsteps; ++i) {
     const int* order = nullptr;
     if ((e = prepare(i, &order)) != hipSuccess) return finish(e, "mog_run");

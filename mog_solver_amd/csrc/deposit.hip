@@ -39,22 +39,6 @@ typedef __attribute__((address_space(3))) unsigned int lds_u32;
 // Deferred-pair records per lane of the LDS deposit (see deposit_lds_kernel).
 constexpr int kLaneSlots = 4;
 
-// Rows of the prepared deposit: (c0, c1, c2, milli) with c = p - W/2 and the
-// 32-bit milli-mass, or kPrepMarker where the fast path cannot take the
-// particle (outside |c_j| <= CB, non-finite, or a milli-mass above the
-// partials' limit).  The milli-mass and the admission test are those of the
-// unprepared loop (same float32 operations, saturating conversion).
-constexpr unsigned int kPrepMarker = 0xFFFFFFFFu;
-
-__device__ __forceinline__ float4 prepared_row(const float4 q, const Params& P,
-                                               unsigned int lds_limit32) {
-  const float c0 = q.x - P.half, c1 = q.y - P.half, c2 = q.z - P.half;
-  const float mm = q.w * 1000.0f;
-  unsigned int m;
-  asm("v_cvt_u32_f32 %0, %1" : "=v"(m) : "v"(mm));
-  const bool take = (m <= lds_limit32) & (bool)particle_fast(c0, c1, c2, P.CB);
-  return make_float4(c0, c1, c2, __uint_as_float(take ? m : kPrepMarker));
-}
 // UP particles per lane and register set in the LDS deposit.
 constexpr int kDepositUP = 2;
 
@@ -677,7 +661,8 @@ template <int G>
 static hipError_t launch_lds(const Params& P, const float4* pos, long long n, long long chunks,
                              long long chunk, int threads, const float* rot, const float* off,
                              const float* frot, unsigned long long* grids,
-                             unsigned short* cells_out, bool sorted, float4* cpos, hipStream_t s) {
+                             unsigned short* cells_out, bool sorted, float4* cpos, hipStream_t s,
+                             bool cpos_ready, unsigned int* prep_limit) {
   const int groups = (P.k + G - 1) / G;
   const unsigned long long lds_limit = 0xFFFFFFFFull / (unsigned long long)chunk;
   const size_t lds = (size_t)G * P.cells * 4 + (sorted ? 0 : (size_t)kLaneSlots * threads * 4);
@@ -709,9 +694,12 @@ static hipError_t launch_lds(const Params& P, const float4* pos, long long n, lo
   if constexpr (G <= 4) {
     if (prep) {
       const unsigned int lds_limit32 = (unsigned int)lds_limit;  // (2^32 - 1) / chunk
-      hipLaunchKernelGGL(deposit_prep_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                         pos, n, P, lds_limit32, cpos);
-      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if (prep_limit) *prep_limit = lds_limit32;
+      if (!cpos_ready) {  // else the rows are already in cpos (written for this step)
+        hipLaunchKernelGGL(deposit_prep_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           s, pos, n, P, lds_limit32, cpos);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+      }
       hipLaunchKernelGGL((deposit_lds_kernel<G, true, true>), grid, dim3(threads), lds, s, pos,
                          (const float4*)cpos, n, chunk, rot, off, frot, P, lds_limit, grids);
       return hipGetLastError();
@@ -733,16 +721,27 @@ static hipError_t launch_lds(const Params& P, const float4* pos, long long n, lo
   return hipGetLastError();
 }
 
+hipError_t launch_deposit_prep(const Params& P, const float4* pos, long long n,
+                               unsigned int lds_limit32, float4* cpos, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(deposit_prep_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos,
+                     n, P, lds_limit32, cpos);
+  return hipGetLastError();
+}
+
 hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const float* rot,
                           const float* off, const float* frot, unsigned long long* grids,
-                          const DepositTune& t, hipStream_t s, float4* cpos) {
-  return launch_deposit_ex(P, pos, n, rot, off, frot, grids, nullptr, false, t, s, cpos);
+                          const DepositTune& t, hipStream_t s, float4* cpos,
+                          unsigned int* prep_limit) {
+  return launch_deposit_ex(P, pos, n, rot, off, frot, grids, nullptr, false, t, s, cpos,
+                           prep_limit);
 }
 
 hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, const float* rot,
                              const float* off, const float* frot, unsigned long long* grids,
                              unsigned short* cells_out, bool sorted, const DepositTune& t,
-                             hipStream_t s, float4* cpos) {
+                             hipStream_t s, float4* cpos, unsigned int* prep_limit) {
+  if (prep_limit) *prep_limit = 0;
   if (n <= 0) return hipSuccess;
   // by default the prepass runs from 2^26 pairs, where it pays for itself
   // (two disks, 2.4e8 pairs: deposit phase -1 %; 64 Plummer spheres,
@@ -782,16 +781,16 @@ hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, co
   switch (G) {
     case 8:
       return launch_lds<8>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
-                           sorted, cpos, s);
+                           sorted, cpos, s, t.cpos_ready, prep_limit);
     case 4:
       return launch_lds<4>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
-                           sorted, cpos, s);
+                           sorted, cpos, s, t.cpos_ready, prep_limit);
     case 2:
       return launch_lds<2>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
-                           sorted, cpos, s);
+                           sorted, cpos, s, t.cpos_ready, prep_limit);
     default:
       return launch_lds<1>(P, pos, n, chunks, chunk, threads, rot, off, frot, grids, cells_out,
-                           sorted, cpos, s);
+                           sorted, cpos, s, t.cpos_ready, prep_limit);
   }
 }
 

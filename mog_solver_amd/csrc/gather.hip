@@ -25,7 +25,8 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
     const float* __restrict__ rot, const float* __restrict__ off, const float* __restrict__ frot,
     const float4* __restrict__ force, Params P, float4* __restrict__ avg_out, int mode,
     const int* __restrict__ order, long long* __restrict__ zero, long long zero_count,
-    float4* __restrict__ ps, float4* __restrict__ vs) {
+    float4* __restrict__ ps, float4* __restrict__ vs, float4* __restrict__ cpos_out,
+    unsigned int lds_limit32) {
   extern __shared__ float adds[];  // LDS3: adds[3 g + m] = frot[12 g + 4 m + 3]; then the records
   unsigned* const recs = reinterpret_cast<unsigned*>(adds + (LDS3 ? 3 * P.k : 0));
   // side job: clear `zero` (the density grids the next step deposits into)
@@ -55,7 +56,10 @@ __global__ __launch_bounds__(256) void gather_update_kernel(
   }
   float4 v = vs ? vs[t] : vel[i];
   integrate(p, v, ax, ay, az, P);
-  pos[i] = p;
+  if (cpos_out)
+    cpos_out[i] = prepared_row(p, P, lds_limit32);
+  else
+    pos[i] = p;
   if (ps) {
     ps[t] = p;
     vs[t] = v;
@@ -78,10 +82,14 @@ __global__ __launch_bounds__(256) void state_import_kernel(const float4* __restr
 
 __global__ __launch_bounds__(256) void state_export_kernel(const float4* __restrict__ vs,
                                                            const int* __restrict__ order,
-                                                           long long n, float4* __restrict__ vel) {
+                                                           long long n, float4* __restrict__ vel,
+                                                           const float4* __restrict__ ps,
+                                                           float4* __restrict__ pos) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  vel[order[t]] = vs[t];
+  const long long i = order[t];
+  vel[i] = vs[t];
+  if (ps) pos[i] = ps[t];
 }
 
 hipError_t launch_state_import(const float4* pos, const float4* vel, const int* order, long long n,
@@ -93,10 +101,10 @@ hipError_t launch_state_import(const float4* pos, const float4* vel, const int* 
 }
 
 hipError_t launch_state_export(const float4* vs, const int* order, long long n, float4* vel,
-                               hipStream_t s) {
+                               hipStream_t s, const float4* ps, float4* pos) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(state_export_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vs,
-                     order, n, vel);
+                     order, n, vel, ps, pos);
   return hipGetLastError();
 }
 
@@ -195,9 +203,11 @@ hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long 
                                 const float* rot, const float* off, const float* frot,
                                 const float4* force, int world, float4* avg_out, int mode,
                                 const int* order, hipStream_t s, long long* zero,
-                                long long zero_count, float4* ps, float4* vs) {
+                                long long zero_count, float4* ps, float4* vs,
+                                float4* cpos_out, unsigned int lds_limit32) {
   if (n <= 0) return zero_count > 0 ? launch_fill_zero(zero, zero_count, s) : hipSuccess;
   if ((ps || vs) && !(ps && vs && order && mode == 0)) return hipErrorInvalidValue;
+  if (cpos_out && !ps) return hipErrorInvalidValue;
   const long long blocks = (n + 255) / 256;
   const size_t lds = (size_t)P.k * 3 * sizeof(float);
   const bool lds3 = lds <= 16 * 1024;  // k <= 1365: a few KB per workgroup
@@ -207,7 +217,7 @@ hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long 
 #define MOG_GATHER(W3, L3, S3)                                                             \
   hipLaunchKernelGGL((gather_update_kernel<W3, L3, S3>), grid, dim3(256),                 \
                      (L3 ? lds : 0) + lds_recs, s, pos, vel, n, rot, off, frot, force, P, \
-                     avg_out, mode, order, zero, zero_count, ps, vs)
+                     avg_out, mode, order, zero, zero_count, ps, vs, cpos_out, lds_limit32)
   if (world && lds3 && sh) MOG_GATHER(true, true, true);
   else if (world && lds3) MOG_GATHER(true, true, false);
   else if (world && sh) MOG_GATHER(true, false, true);

@@ -139,12 +139,31 @@ __device__ __forceinline__ void integrate(float4& p, float4& v, float ax, float 
 // Host-side launchers (defined in the kernel translation units).
 namespace mog {
 
+// Rows of the prepared deposit: (c0, c1, c2, milli) with c = p - W/2 and the
+// 32-bit milli-mass, or kPrepMarker where the fast path cannot take the
+// particle (outside |c_j| <= CB, non-finite, or a milli-mass above the
+// partials' limit).  The milli-mass and the admission test are those of the
+// unprepared loop (same float32 operations, saturating conversion).  Written
+// by the deposit's prepass, or by a run's gather for the next step.
+constexpr unsigned int kPrepMarker = 0xFFFFFFFFu;
+
+__device__ __forceinline__ float4 prepared_row(const float4 q, const Params& P,
+                                               unsigned int lds_limit32) {
+  const float c0 = q.x - P.half, c1 = q.y - P.half, c2 = q.z - P.half;
+  const float mm = q.w * 1000.0f;
+  unsigned int m;
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(m) : "v"(mm));
+  const bool take = (m <= lds_limit32) & (bool)particle_fast(c0, c1, c2, P.CB);
+  return make_float4(c0, c1, c2, __uint_as_float(take ? m : kPrepMarker));
+}
+
 struct DepositTune {
   int threads;        // threads per workgroup
   int grids_per_wg;   // grids privatised in LDS per workgroup
   int wg_target;      // aim for about this many workgroups
   int prepass = 0;    // prepared rows: 0 from 2^26 pairs, 1 always, -1 never
   long long min_chunk = 0;  // particles per workgroup at least this many
+  bool cpos_ready = false;  // the prepared rows are already in cpos: no prepass
 };
 
 hipError_t launch_fill_zero(long long* p, long long count, hipStream_t s);
@@ -152,7 +171,13 @@ hipError_t launch_fill_zero(long long* p, long long count, hipStream_t s);
 // deposit (centred coordinates and milli-mass, written by a prepass).
 hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const float* rot,
                           const float* off, const float* frot, unsigned long long* grids,
-                          const DepositTune& t, hipStream_t s, float4* cpos = nullptr);
+                          const DepositTune& t, hipStream_t s, float4* cpos = nullptr,
+                          unsigned int* prep_limit = nullptr);
+// prep_limit (optional): set to the prepared rows' milli-mass limit when the
+// deposit reads prepared rows (with t.cpos_ready it then runs no prepass),
+// else to 0.  launch_deposit_prep writes the rows of n particles.
+hipError_t launch_deposit_prep(const Params& P, const float4* pos, long long n,
+                               unsigned int lds_limit32, float4* cpos, hipStream_t s);
 // world != 0: force grids hold M F (world frame) for the step's gather;
 // world == 0: grid-frame forces as pack.glsl writes them.
 // Deposit of positions already in Morton visiting order (sorted != 0; lanes
@@ -161,7 +186,8 @@ hipError_t launch_deposit(const Params& P, const float4* pos, long long n, const
 hipError_t launch_deposit_ex(const Params& P, const float4* pos, long long n, const float* rot,
                              const float* off, const float* frot, unsigned long long* grids,
                              unsigned short* cells_out, bool sorted, const DepositTune& t,
-                             hipStream_t s, float4* cpos = nullptr);
+                             hipStream_t s, float4* cpos = nullptr,
+                             unsigned int* prep_limit = nullptr);
 // spos[t] = pos[order[t]]
 hipError_t launch_permute(const float4* pos, const int* order, long long n, float4* spos,
                           hipStream_t s);
@@ -200,19 +226,23 @@ hipError_t launch_solve_mfma(const Params& P, int precision, const long long* gr
 // ps[t] = pos[order[t]], vs[t] = vel[order[t]]; the gather then reads and
 // writes them at slot t (coalesced) and writes only the positions back to
 // pos[order[t]] (the deposit's input), leaving vel stale until
-// launch_state_export.
+// launch_state_export.  cpos_out (optional, with ps): the next deposit's
+// prepared row goes to cpos_out[order[t]] instead (milli-mass limit
+// lds_limit32), and pos stays stale too.
 hipError_t launch_gather_update(const Params& P, float4* pos, float4* vel, long long n,
                                 const float* rot, const float* off, const float* frot,
                                 const float4* force, int world, float4* avg_out, int mode,
                                 const int* order, hipStream_t s, long long* zero = nullptr,
                                 long long zero_count = 0, float4* ps = nullptr,
-                                float4* vs = nullptr);
+                                float4* vs = nullptr, float4* cpos_out = nullptr,
+                                unsigned int lds_limit32 = 0);
 // ps[t] = pos[order[t]], vs[t] = vel[order[t]] for t < n.
 hipError_t launch_state_import(const float4* pos, const float4* vel, const int* order, long long n,
                                float4* ps, float4* vs, hipStream_t s);
-// vel[order[t]] = vs[t] for t < n.
+// vel[order[t]] = vs[t] for t < n (and pos[order[t]] = ps[t] when ps is
+// given: a run whose gather wrote prepared rows instead of positions).
 hipError_t launch_state_export(const float4* vs, const int* order, long long n, float4* vel,
-                               hipStream_t s);
+                               hipStream_t s, const float4* ps = nullptr, float4* pos = nullptr);
 // Morton visiting order of the particles (order.hip): temp storage size of
 // the radix sort, and the order itself (keys_a/keys_b/idx_a: n-element
 // scratch; order: n-element output).

@@ -350,18 +350,21 @@ def test_step_matches_phases_and_graph(torch_cuda, k, g):
         assert torch.equal(p3, p4) and torch.equal(v3, v4)
 
 
-@pytest.mark.parametrize("resort", [1, 2, 5])
-def test_run_state_across_reorders(torch_cuda, resort):
+@pytest.mark.parametrize("resort,prepass", [(1, 0), (2, 0), (5, 0), (2, 1), (5, 1)])
+def test_run_state_across_reorders(torch_cuda, resort, prepass):
     """mog_run keeps the particle state in visiting order (imported at the
     start and after each rebuild of the order, velocities written back at
     each rebuild and at the end): bit-identical to single steps through the
     caller's arrays, with the order rebuilt inside a run, eager and graph,
-    across successive calls."""
+    across successive calls.  With the deposit's prepass (forced on), the
+    graph replays' gather writes the next step's prepared rows and the
+    positions go back with the velocities."""
     torch = torch_cuda
     k, g, W = 12, 16, 100.0
     pos, vel = workloads.two_disks(80000, W, 100.0, g, seed=8)
     s, rot, off = make_solver(k, g, W)
-    s.set_tuning(**{**s.get_tuning(), "resort_every": resort, "sorted_deposit": 0, "cell_cache": 0})
+    s.set_tuning(**{**s.get_tuning(), "resort_every": resort, "sorted_deposit": 0, "cell_cache": 0,
+                    "deposit_prepass": prepass})
     p1, v1 = dev(torch, pos), dev(torch, vel)
     for _ in range(7):
         s.step(p1, v1)
