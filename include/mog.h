@@ -107,9 +107,10 @@ int mog_step(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, void* stream
 /* `nsteps` consecutive steps; with use_graph != 0 one step is captured in a
  * hipGraph on first use for these (pos, vel, n, stream) and replayed.  From
  * 4 steps on, the run works on a copy of the state in the gather's visiting
- * order (the context's own buffers): pos stays current after every step,
- * vel is written back at each rebuild of the order and before the call
- * returns (stream-ordered), with the same bits as nsteps mog_step calls. */
+ * order (the context's own buffers): vel (and, in graph replays that feed
+ * the deposit prepared rows, pos) is written back at each rebuild of the
+ * order and at the end of the call's work on the stream, with the same bits
+ * as nsteps mog_step calls; the arrays are current once that work is done. */
 int mog_run(mog_ctx* ctx, float* pos_n4, float* vel_n4, int64_t n, int32_t nsteps,
             int32_t use_graph, void* stream);
 
